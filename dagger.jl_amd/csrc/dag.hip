@@ -1,0 +1,630 @@
+// Datadeps executor: a static task DAG run on HIP streams / hipGraphs.
+//
+// Semantics follow Dagger's datadeps regions (src/datadeps/queue.jl:44-54,
+// dependency rules of get_write_deps!/get_read_deps! in
+// src/datadeps/aliasing.jl:740-773): tasks are recorded in submission
+// order; an In access waits for the last earlier overlapping writer, an
+// Out/InOut access waits for every earlier overlapping reader and writer,
+// In/In never synchronize.  Overlap is decided on byte spans of the operand
+// footprints (column-major rows/cols/ld), which for whole tiles is Dagger's
+// ContiguousAliasing of the tile array (src/memory-spaces.jl:466-470).
+//
+// Planning (done once, reused by every run):
+//  1. edges from the rules above;
+//  2. ASAP level (longest path from a source) and ALAP level (latest level
+//     that does not lengthen the DAG); slack = ALAP - ASAP;
+//  3. groups: tasks with the same ASAP level, the same kernel shape and the
+//     same criticality (slack <= lookahead) become one batched launch -- a
+//     level holds no edges, so its tasks are independent;
+//  4. critical groups go on stream 0, others on streams 1..S-1; a group
+//     waits (hipStreamWaitEvent) on the newest predecessor group of every
+//     other stream; groups are issued in level order;
+//  5. optionally the whole issue sequence is captured into a hipGraph.
+// For the tiled Cholesky this yields the classic look-ahead schedule: the
+// diagonal POTRF, the panel TRSM and the next panel's update run on the
+// critical stream while the rest of the trailing update runs beside them.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <vector>
+#include "dm_internal.h"
+#include "dag.h"
+
+namespace dm {
+
+namespace {
+
+size_t esize(int dtype) { return dtype == DM_F32 ? 4 : 8; }
+
+// Footprint of a column-major rows x cols array with leading dimension ld.
+void mat_span(const void* p, long rows, long cols, long ld, size_t es, uintptr_t& lo,
+              uintptr_t& hi) {
+  lo = (uintptr_t)p;
+  if (rows <= 0 || cols <= 0) { hi = lo; return; }
+  hi = lo + (size_t)((cols - 1) * ld + rows) * es;
+}
+
+}  // namespace
+
+int Dag::add(const dm_task& t) {
+  if (planned) return arg_fail(1, "dag already planned");
+  if (t.op < DM_OP_POTRF || t.op > DM_OP_GEMV) return arg_fail(2, "unknown op");
+  if (t.dtype != DM_F64 && t.dtype != DM_F32) return arg_fail(2, "unknown dtype");
+  TaskRec r;
+  r.t = t;
+  const size_t es = esize(t.dtype);
+  auto push = [&](const void* p, long rows, long cols, long ld, int mode) {
+    if (!p || mode == 0) return;
+    Access a;
+    mat_span(p, rows, cols, ld, es, a.lo, a.hi);
+    a.mode = mode;
+    if (a.hi > a.lo) r.acc.push_back(a);
+  };
+  const bool ta = is_trans(t.transa), tb = is_trans(t.transb);
+  switch (t.op) {
+    case DM_OP_GEMM:
+      push(t.a, ta ? t.k : t.m, ta ? t.m : t.k, t.lda, t.access[0]);
+      push(t.b, tb ? t.n : t.k, tb ? t.k : t.n, t.ldb, t.access[1]);
+      push(t.c, t.m, t.n, t.ldc, t.access[2]);
+      r.cost = 2.0 * t.m * t.n * t.k;
+      break;
+    case DM_OP_SYRK:
+      push(t.a, ta ? t.k : t.n, ta ? t.n : t.k, t.lda, t.access[0]);
+      push(t.c, t.n, t.n, t.ldc, t.access[2]);
+      r.cost = 1.0 * t.n * t.n * t.k;
+      break;
+    case DM_OP_TRSM: {
+      const long ka = is_left(t.side) ? t.m : t.n;
+      push(t.a, ka, ka, t.lda, t.access[0]);
+      push(t.c, t.m, t.n, t.ldc, t.access[2]);
+      r.cost = 1.0 * t.m * t.n * ka;
+      break;
+    }
+    case DM_OP_POTRF:
+      push(t.c, t.n, t.n, t.ldc, t.access[2] ? t.access[2] : DM_INOUT);
+      if (t.info) push(t.info, 1, 1, 1, DM_OUT);
+      r.cost = 1.0 * t.n * t.n * t.n / 3.0;
+      break;
+    case DM_OP_GEMV: {
+      // x in slot b (stride ldb), y in slot c (stride ldc)
+      const long lx = ta ? t.m : t.n, ly = ta ? t.n : t.m;
+      push(t.a, t.m, t.n, t.lda, t.access[0]);
+      push(t.b, 1, lx, t.ldb, t.access[1]);
+      push(t.c, 1, ly, t.ldc, t.access[2]);
+      r.cost = 2.0 * t.m * t.n;
+      break;
+    }
+    case DM_OP_TRSV:
+      push(t.a, t.n, t.n, t.lda, t.access[0]);
+      push(t.c, 1, t.n, t.ldc, t.access[2]);
+      r.cost = 1.0 * t.n * t.n;
+      break;
+  }
+  const int id = (int)tasks.size();
+  // ---- dependencies (datadeps rules on overlapping spans)
+  std::set<int> deps;
+  for (const Access& a : r.acc) {
+    const bool writes = (a.mode & DM_OUT) != 0;
+    auto it = regions_by_lo.lower_bound(a.lo >= max_len ? a.lo - max_len : 0);
+    for (; it != regions_by_lo.end() && it->first < a.hi; ++it) {
+      const Region& g = regions[it->second];
+      if (g.hi <= a.lo) continue;
+      if (g.last_writer >= 0) deps.insert(g.last_writer);
+      if (writes)
+        for (int rd : g.readers) deps.insert(rd);
+    }
+  }
+  deps.erase(id);
+  r.preds.assign(deps.begin(), deps.end());
+  // ---- update region state
+  for (const Access& a : r.acc) {
+    int gi = -1;
+    auto range = regions_by_lo.equal_range(a.lo);
+    for (auto it = range.first; it != range.second; ++it)
+      if (regions[it->second].hi == a.hi) { gi = it->second; break; }
+    if (gi < 0) {
+      gi = (int)regions.size();
+      Region g;
+      g.lo = a.lo;
+      g.hi = a.hi;
+      regions.push_back(g);
+      regions_by_lo.emplace(a.lo, gi);
+      max_len = std::max<uintptr_t>(max_len, a.hi - a.lo);
+    }
+    Region& g = regions[gi];
+    if (a.mode & DM_OUT) {
+      g.last_writer = id;
+      g.readers.clear();
+    } else if (g.readers.empty() || g.readers.back() != id) {
+      g.readers.push_back(id);
+    }
+  }
+  for (int p : r.preds) tasks[p].succs.push_back(id);
+  tasks.push_back(std::move(r));
+  return 0;
+}
+
+// Group key: kernel shape that can share one batched launch.
+static long shape_key(const dm_task& t) {
+  long key = (long)t.dtype << 20;
+  switch (t.op) {
+    case DM_OP_GEMM:
+    case DM_OP_SYRK: {
+      bool ta, tb;
+      if (t.op == DM_OP_GEMM) { ta = is_trans(t.transa); tb = is_trans(t.transb); }
+      else { ta = is_trans(t.transa); tb = !ta; }
+      return key | (1L << 8) | (ta ? 2 : 0) | (tb ? 1 : 0);
+    }
+    case DM_OP_TRSM: {
+      TrsmShape sh = trsm_shape(t.side, t.uplo, t.transa, t.diag);
+      return key | (2L << 8) | (sh.lower ? 8 : 0) | (sh.ttrans ? 4 : 0) | (sh.side_l ? 2 : 0) |
+             (sh.unit ? 1 : 0);
+    }
+    default:
+      return key | ((long)t.op << 8);
+  }
+}
+
+int Dag::plan(const dm_plan_opts& o) {
+  if (planned) return arg_fail(1, "dag already planned");
+  opts = o;
+  if (opts.num_streams < 1) opts.num_streams = 1;
+  if (opts.num_streams > 8) opts.num_streams = 8;
+  const int n = (int)tasks.size();
+  // ---- levels
+  int maxlev = -1;
+  for (int i = 0; i < n; ++i) {  // submission order is a topological order
+    int lv = 0;
+    for (int p : tasks[i].preds) lv = std::max(lv, tasks[p].level + 1);
+    tasks[i].level = lv;
+    maxlev = std::max(maxlev, lv);
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    int al = maxlev;
+    for (int s : tasks[i].succs) al = std::min(al, tasks[s].alap - 1);
+    tasks[i].alap = al;
+  }
+  num_levels = maxlev + 1;
+  // ---- groups
+  std::map<std::tuple<int, int, long, int>, int> gid;  // (level, crit, shape, serial) -> group
+  for (int i = 0; i < n; ++i) {
+    TaskRec& r = tasks[i];
+    const int slack = r.alap - r.level;
+    const int crit = (opts.lookahead > 0 && slack <= opts.lookahead) ? 0 : 1;
+    const bool batchable = opts.batch && (r.t.op == DM_OP_GEMM || r.t.op == DM_OP_SYRK ||
+                                          r.t.op == DM_OP_TRSM);
+    const auto key = std::make_tuple(r.level, crit, shape_key(r.t), batchable ? -1 : i);
+    auto it = gid.find(key);
+    int g;
+    if (it == gid.end()) {
+      g = (int)groups.size();
+      gid.emplace(key, g);
+      Group G;
+      G.level = r.level;
+      G.crit = crit;
+      G.op = r.t.op == DM_OP_SYRK ? DM_OP_GEMM : r.t.op;
+      G.dtype = r.t.dtype;
+      groups.push_back(G);
+    } else {
+      g = it->second;
+    }
+    groups[g].tasks.push_back(i);
+    r.group = g;
+  }
+  // issue order: by level, critical first (std::map order already is)
+  order.clear();
+  for (auto& kv : gid) order.push_back(kv.second);
+  // ---- streams
+  int rr = 0;
+  for (int g : order) {
+    Group& G = groups[g];
+    if (opts.num_streams == 1 || (opts.lookahead > 0 && G.crit == 0)) G.stream = 0;
+    else if (opts.lookahead > 0) G.stream = 1 + (rr++ % (opts.num_streams - 1));
+    else G.stream = rr++ % opts.num_streams;
+  }
+  // ---- cross-stream waits: newest predecessor group per other stream
+  std::vector<int> issue_pos(groups.size());
+  for (size_t i = 0; i < order.size(); ++i) issue_pos[order[i]] = (int)i;
+  for (int g : order) {
+    Group& G = groups[g];
+    std::vector<int> newest(opts.num_streams, -1);
+    for (int t : G.tasks)
+      for (int p : tasks[t].preds) {
+        const int pg = tasks[p].group;
+        const Group& P = groups[pg];
+        if (P.stream == G.stream) continue;
+        if (newest[P.stream] < 0 || issue_pos[pg] > issue_pos[newest[P.stream]])
+          newest[P.stream] = pg;
+      }
+    for (int sidx = 0; sidx < opts.num_streams; ++sidx)
+      if (newest[sidx] >= 0) G.waits.push_back(newest[sidx]);
+  }
+  int rc = build_device_tables();
+  if (rc) return rc;
+  rc = create_streams();
+  if (rc) return rc;
+  planned = true;
+  if (opts.use_graph) {
+    rc = capture();
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int Dag::create_streams() {
+  streams.resize(opts.num_streams);
+  for (auto& s : streams) DM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
+  events.resize(groups.size());
+  for (auto& e : events) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  fork_ev.resize(1);
+  DM_HIP(hipEventCreateWithFlags(&fork_ev[0], hipEventDisableTiming));
+  join_ev.resize(opts.num_streams);
+  for (auto& e : join_ev) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return 0;
+}
+
+int Dag::build_device_tables() {
+  // Host-side job tables for batched groups, then one upload.
+  std::vector<unsigned char> blob;
+  auto align = [&](size_t a) { while (blob.size() % a) blob.push_back(0); };
+  for (Group& G : groups) {
+    if (G.tasks.size() <= 1 && G.op != DM_OP_TRSM && G.op != DM_OP_GEMM) continue;
+    if (G.op == DM_OP_GEMM) {
+      const dm_task& t0 = tasks[G.tasks[0]].t;
+      bool ta, tb;
+      if (t0.op == DM_OP_GEMM) { ta = is_trans(t0.transa); tb = is_trans(t0.transb); }
+      else { ta = is_trans(t0.transa); tb = !ta; }
+      G.ta = ta;
+      G.tb = tb;
+      auto fill = [&](auto* dummy) -> int {
+        typedef std::remove_pointer_t<decltype(dummy)> T;
+        std::vector<GemmJob<T>> jobs;
+        int maxM = 0, maxN = 0, maxK = 0;
+        for (int ti : G.tasks) {
+          const dm_task& t = tasks[ti].t;
+          GemmJob<T> j{};
+          if (t.op == DM_OP_GEMM) {
+            j.A = (const T*)t.a; j.B = (const T*)t.b; j.C = (T*)t.c;
+            j.M = t.m; j.N = t.n; j.K = t.k;
+            j.lda = t.lda; j.ldb = t.ldb; j.ldc = t.ldc;
+            j.tri = 0;
+          } else {  // SYRK as masked GEMM of A with itself
+            j.A = (const T*)t.a; j.B = (const T*)t.a; j.C = (T*)t.c;
+            j.M = t.n; j.N = t.n; j.K = t.k;
+            j.lda = t.lda; j.ldb = t.lda; j.ldc = t.ldc;
+            j.tri = is_upper(t.uplo) ? 1 : 2;
+          }
+          j.alpha = (T)t.alpha;
+          j.beta = (T)t.beta;
+          if (j.K <= 0 || j.alpha == T(0)) { j.K = 0; j.alpha = T(0); }
+          if (j.M <= 0 || j.N <= 0) continue;
+          jobs.push_back(j);
+          maxM = std::max(maxM, j.M); maxN = std::max(maxN, j.N); maxK = std::max(maxK, j.K);
+        }
+        G.njobs = (int)jobs.size();
+        if (!G.njobs) return 0;
+        G.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, G.njobs);
+        G.total_wg = gemm_prepare_jobs<T>(jobs.data(), G.njobs, G.cfg);
+        G.vec = true;
+        for (auto& j : jobs) G.vec = G.vec && gemm_job_vec_ok<T>(ta, tb, j);
+        align(64);
+        G.table_off = blob.size();
+        const unsigned char* p = (const unsigned char*)jobs.data();
+        blob.insert(blob.end(), p, p + jobs.size() * sizeof(GemmJob<T>));
+        return 0;
+      };
+      int rc = G.dtype == DM_F64 ? fill((double*)nullptr) : fill((float*)nullptr);
+      if (rc) return rc;
+    } else if (G.op == DM_OP_TRSM) {
+      const dm_task& t0 = tasks[G.tasks[0]].t;
+      G.shape = trsm_shape(t0.side, t0.uplo, t0.transa, t0.diag);
+      auto fill = [&](auto* dummy) -> int {
+        typedef std::remove_pointer_t<decltype(dummy)> T;
+        // batched only when every job fits the LDS-resident kernel
+        int maxL = 0;
+        bool fits = true;
+        for (int ti : G.tasks) {
+          const dm_task& t = tasks[ti].t;
+          const int L = G.shape.side_l ? t.m : t.n;
+          maxL = std::max(maxL, L);
+          if (L > trsm_max_len<T>()) fits = false;
+        }
+        G.batched_trsm = fits;
+        if (!fits) return 0;
+        // one inverse-block buffer per distinct (A, L)
+        std::map<std::pair<const void*, int>, int> invidx;
+        std::vector<TrsmJob<T>> jobs;
+        long wg = 0;
+        for (int ti : G.tasks) {
+          const dm_task& t = tasks[ti].t;
+          const int L = G.shape.side_l ? t.m : t.n;
+          const int NV = G.shape.side_l ? t.n : t.m;
+          if (L <= 0 || NV <= 0) continue;
+          auto key = std::make_pair(t.a, L);
+          auto it = invidx.find(key);
+          int ii;
+          if (it == invidx.end()) {
+            ii = (int)G.inv_src.size();
+            invidx.emplace(key, ii);
+            InvSrc src;
+            src.A = t.a;
+            src.lda = t.lda;
+            src.L = L;
+            src.off = inv_bytes;
+            inv_bytes += sizeof(T) * (size_t)((L + TRSM_BS - 1) / TRSM_BS) * TRSM_BS * TRSM_BS;
+            inv_bytes = (inv_bytes + 255) & ~(size_t)255;
+            G.inv_src.push_back(src);
+          } else {
+            ii = it->second;
+          }
+          TrsmJob<T> j{};
+          j.A = (const T*)t.a; j.B = (T*)t.c;
+          j.inv = (const T*)(uintptr_t)G.inv_src[ii].off;  // patched after allocation
+          j.L = L; j.NV = NV; j.lda = t.lda; j.ldb = t.ldc;
+          j.alpha = (T)t.alpha;
+          j.wg_begin = (int)wg;
+          wg += trsm_wgs(NV);
+          jobs.push_back(j);
+        }
+        G.njobs = (int)jobs.size();
+        G.total_wg = wg;
+        G.maxL = maxL;
+        align(64);
+        G.table_off = blob.size();
+        const unsigned char* p = (const unsigned char*)jobs.data();
+        blob.insert(blob.end(), p, p + jobs.size() * sizeof(TrsmJob<T>));
+        return 0;
+      };
+      int rc = G.dtype == DM_F64 ? fill((double*)nullptr) : fill((float*)nullptr);
+      if (rc) return rc;
+    }
+  }
+  // device memory: [abort int (256 B)] [inverse blocks] [job tables]
+  const size_t tab_base = 256 + inv_bytes;
+  const size_t total = tab_base + blob.size();
+  DM_HIP(hipMalloc(&dev_mem, total));
+  abort_dev = (int*)dev_mem;
+  unsigned char* base = (unsigned char*)dev_mem;
+  // patch inverse pointers inside TRSM tables
+  for (Group& G : groups) {
+    if (G.op != DM_OP_TRSM || !G.batched_trsm || !G.njobs) continue;
+    if (G.dtype == DM_F64) {
+      auto* jobs = (TrsmJob<double>*)(blob.data() + G.table_off);
+      for (int i = 0; i < G.njobs; ++i)
+        jobs[i].inv = (const double*)(base + 256 + (uintptr_t)jobs[i].inv);
+    } else {
+      auto* jobs = (TrsmJob<float>*)(blob.data() + G.table_off);
+      for (int i = 0; i < G.njobs; ++i)
+        jobs[i].inv = (const float*)(base + 256 + (uintptr_t)jobs[i].inv);
+    }
+  }
+  for (Group& G : groups) {
+    G.table_dev = base + tab_base + G.table_off;
+    for (InvSrc& s : G.inv_src) s.dev = base + 256 + s.off;
+  }
+  if (!blob.empty()) DM_HIP(hipMemcpy(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  DM_HIP(hipMemset(dev_mem, 0, 256));
+  return 0;
+}
+
+template <typename T>
+int Dag::launch_group_t(const Group& G, hipStream_t s) {
+  const int* ab = abort_dev;
+  if (G.op == DM_OP_GEMM) {
+    if (G.njobs == 0) return 0;
+    GemmJob<T> dummy{};
+    return gemm_launch<T>(G.ta, G.tb, G.vec, G.cfg, (const GemmJob<T>*)G.table_dev, G.njobs, dummy,
+                          G.total_wg, ab, s);
+  }
+  if (G.op == DM_OP_TRSM && G.batched_trsm) {
+    for (const InvSrc& src : G.inv_src) {
+      int rc = trinv_blocks_launch<T>(G.shape, (const T*)src.A, src.lda, src.L, (T*)src.dev, ab, s);
+      if (rc) return rc;
+    }
+    if (G.njobs == 0) return 0;
+    TrsmJob<T> dummy{};
+    dummy.L = G.maxL;
+    return trsm_launch<T>(G.shape, (const TrsmJob<T>*)G.table_dev, G.njobs, dummy, G.total_wg, ab,
+                          s);
+  }
+  // Unbatched ops: one call per task in submission order.
+  for (int ti : G.tasks) {
+    const dm_task& t = tasks[ti].t;
+    int rc = 0;
+    switch (t.op) {
+      case DM_OP_POTRF: {
+        const int gofs = t.k;  // global row offset of the tile (set by the builders; 0 otherwise)
+        rc = potrf_full<T>(is_upper(t.uplo), t.n, (T*)t.c, t.ldc, t.info, abort_dev, gofs, s);
+        break;
+      }
+      case DM_OP_TRSM:
+        rc = trsm_full<T>(t.side, t.uplo, t.transa, t.diag, t.m, t.n, (T)t.alpha, (const T*)t.a,
+                          t.lda, (T*)t.c, t.ldc, ab, s);
+        break;
+      case DM_OP_GEMM:
+        rc = gemm_single<T>(is_trans(t.transa), is_trans(t.transb), t.m, t.n, t.k, (T)t.alpha,
+                            (const T*)t.a, t.lda, (const T*)t.b, t.ldb, (T)t.beta, (T*)t.c, t.ldc,
+                            0, ab, s);
+        break;
+      case DM_OP_SYRK: {
+        const bool tr = is_trans(t.transa);
+        rc = gemm_single<T>(tr, !tr, t.n, t.n, t.k, (T)t.alpha, (const T*)t.a, t.lda,
+                            (const T*)t.a, t.lda, (T)t.beta, (T*)t.c, t.ldc,
+                            is_upper(t.uplo) ? 1 : 2, ab, s);
+        break;
+      }
+      case DM_OP_GEMV:
+        if (sizeof(T) != 8) return arg_fail(0, "gemv: only FP64");
+        rc = dgemv_launch(is_trans(t.transa), t.m, t.n, t.alpha, (const double*)t.a, t.lda,
+                          (const double*)t.b, t.ldb, t.beta, (double*)t.c, t.ldc, ab, s);
+        break;
+      case DM_OP_TRSV:
+        if (t.ldc == 1)
+          rc = trsm_full<T>('L', t.uplo, t.transa, t.diag, t.n, 1, T(1), (const T*)t.a, t.lda,
+                            (T*)t.c, t.n, ab, s);
+        else
+          rc = trsm_full<T>('R', t.uplo, is_trans(t.transa) ? 'N' : 'T', t.diag, 1, t.n, T(1),
+                            (const T*)t.a, t.lda, (T*)t.c, t.ldc, ab, s);
+        break;
+    }
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int Dag::issue(hipStream_t o) {
+  // reset the abort flag, fork, run groups, join
+  int rc = fill_i32_launch(abort_dev, 0, 1, o);
+  if (rc) return rc;
+  DM_HIP(hipEventRecord(fork_ev[0], o));
+  for (hipStream_t s : streams) DM_HIP(hipStreamWaitEvent(s, fork_ev[0], 0));
+  for (int g : order) {
+    const Group& G = groups[g];
+    hipStream_t s = streams[G.stream];
+    for (int w : G.waits) DM_HIP(hipStreamWaitEvent(s, events[w], 0));
+    rc = G.dtype == DM_F64 ? launch_group_t<double>(G, s) : launch_group_t<float>(G, s);
+    if (rc) return rc;
+    DM_HIP(hipEventRecord(events[g], s));
+  }
+  for (size_t i = 0; i < streams.size(); ++i) {
+    DM_HIP(hipEventRecord(join_ev[i], streams[i]));
+    DM_HIP(hipStreamWaitEvent(o, join_ev[i], 0));
+  }
+  if (post_hook) {
+    rc = post_hook(this, o);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int Dag::capture() {
+  DM_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+  int rc = issue(origin);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(origin, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  graph = g;
+  DM_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  size_t nn = 0;
+  (void)hipGraphGetNodes(graph, nullptr, &nn);
+  graph_nodes = (int)nn;
+  return 0;
+}
+
+int Dag::run(hipStream_t s) {
+  if (!planned) return arg_fail(1, "dag not planned");
+  if (graph_exec) {
+    DM_HIP(hipGraphLaunch(graph_exec, s));
+    return 0;
+  }
+  // Eager issue: the origin stream is the caller's stream.
+  return issue(s);
+}
+
+int Dag::launches() const {
+  int n = 0;
+  for (const Group& G : groups) {
+    if (G.op == DM_OP_GEMM) n += G.njobs > 0 ? 1 : 0;
+    else if (G.op == DM_OP_TRSM && G.batched_trsm) n += (int)G.inv_src.size() + (G.njobs > 0);
+    else n += (int)G.tasks.size();
+  }
+  return n;
+}
+
+Dag::~Dag() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (auto& e : events) (void)hipEventDestroy(e);
+  for (auto& e : fork_ev) (void)hipEventDestroy(e);
+  for (auto& e : join_ev) (void)hipEventDestroy(e);
+  if (!streams.empty() || origin) (void)hipDeviceSynchronize();
+  for (auto& s : streams) (void)hipStreamDestroy(s);
+  if (origin) (void)hipStreamDestroy(origin);
+  if (dev_mem) (void)hipFree(dev_mem);
+}
+
+}  // namespace dm
+
+using namespace dm;
+
+struct dm_dag_s {
+  std::unique_ptr<Dag> d;
+};
+
+extern "C" {
+
+int dm_dag_create(dm_dag_t* dag) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  *dag = new dm_dag_s{std::unique_ptr<Dag>(new Dag())};
+  return 0;
+}
+
+int dm_dag_destroy(dm_dag_t dag) {
+  if (!dag) return 0;
+  delete dag;
+  return 0;
+}
+
+int dm_dag_add_task(dm_dag_t dag, const dm_task* t) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  if (!t) return arg_fail(2, "task is NULL");
+  return dag->d->add(*t);
+}
+
+int dm_dag_edges(dm_dag_t dag, int* pairs, int* count) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  if (!count) return arg_fail(3, "count is NULL");
+  const int cap = *count;
+  int n = 0;
+  for (size_t i = 0; i < dag->d->tasks.size(); ++i)
+    for (int p : dag->d->tasks[i].preds) {
+      if (pairs && n < cap) {
+        pairs[2 * n] = p;
+        pairs[2 * n + 1] = (int)i;
+      }
+      ++n;
+    }
+  *count = n;
+  return 0;
+}
+
+int dm_dag_plan(dm_dag_t dag, const dm_plan_opts* opts) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  dm_plan_opts o{2, 1, 1, 1};
+  if (opts) o = *opts;
+  int rc = ensure_init();
+  if (rc) return rc;
+  return dag->d->plan(o);
+}
+
+int dm_dag_run(dm_dag_t dag, dm_stream_t s) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  int rc = ensure_init();
+  if (rc) return rc;
+  return dag->d->run(stream_of(s));
+}
+
+int dm_dag_get_stats(dm_dag_t dag, dm_dag_stats* st) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  if (!st) return arg_fail(2, "stats is NULL");
+  const Dag& d = *dag->d;
+  st->num_tasks = (int)d.tasks.size();
+  int e = 0;
+  for (auto& t : d.tasks) e += (int)t.preds.size();
+  st->num_edges = e;
+  st->num_groups = (int)d.groups.size();
+  st->num_levels = d.num_levels;
+  st->num_launches = d.planned ? d.launches() : 0;
+  st->num_streams = d.opts.num_streams;
+  st->graph = d.graph_exec ? 1 : 0;
+  return 0;
+}
+
+}  // extern "C"

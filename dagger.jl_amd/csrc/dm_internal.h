@@ -1,0 +1,155 @@
+// Internal declarations shared by the HIP translation units of the
+// MI355X backend.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include "../../include/dagger_mi355x.h"
+
+struct dm_stream_s { hipStream_t h; };
+struct dm_event_s { hipEvent_t h; };
+struct dm_graph_s { hipGraph_t g; hipGraphExec_t exec; };
+
+namespace dm {
+
+// ------------------------------------------------------------ error state
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int arg_fail(int argno, const char* what);
+
+#define DM_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) return ::dm::hip_fail(e_, #call);                 \
+  } while (0)
+
+// Launch-error check after a kernel launch.
+#define DM_LAUNCH_CHECK(name)                                               \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess) return ::dm::hip_fail(e_, name);                  \
+  } while (0)
+
+int ensure_init();
+hipStream_t stream_of(dm_stream_t s);
+
+// Per-stream scratch buffer (grown on demand, never shrunk while the
+// library is loaded).  Calls ordered on one stream may reuse it.
+void* stream_workspace(hipStream_t s, size_t bytes);
+
+inline bool is_trans(char t) { return t == 'T' || t == 't' || t == 'C' || t == 'c'; }
+inline bool is_notrans(char t) { return t == 'N' || t == 'n'; }
+inline bool is_upper(char u) { return u == 'U' || u == 'u'; }
+inline bool is_lower(char u) { return u == 'L' || u == 'l'; }
+inline bool is_left(char s) { return s == 'L' || s == 'l'; }
+inline bool is_right(char s) { return s == 'R' || s == 'r'; }
+inline bool is_unit(char d) { return d == 'U' || d == 'u'; }
+inline bool is_nonunit(char d) { return d == 'N' || d == 'n'; }
+
+// ------------------------------------------------------------ GEMM family
+// One C tile update of a (possibly batched) GEMM/SYRK launch:
+//   C[0:M,0:N] = alpha * op(A) op(B) + beta * C   restricted to the
+//   triangle `tri` (0 full, 1 upper m<=n, 2 lower m>=n).
+template <typename T>
+struct GemmJob {
+  const T* A;
+  const T* B;
+  T* C;
+  int M, N, K;
+  int lda, ldb, ldc;
+  T alpha, beta;
+  int tri;
+  int tiles_m, tiles_n;  // workgroup tiles along M and N (filled by host)
+  int wg_begin;          // first workgroup of this job in a batched launch
+};
+
+// Kernel configuration chosen by the host for a launch.
+struct GemmCfgId {
+  int id;     // index into the configuration table
+  int bm, bn; // workgroup tile
+};
+template <typename T>
+GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs);
+
+// Fills tiles_m/tiles_n/wg_begin for jobs under `cfg`; returns total WGs.
+template <typename T>
+long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg);
+
+// Launches a batched GEMM over device-resident jobs (jobs_dev) or, when
+// njobs == 0, over the single job passed by value.
+// vec: every job satisfies gemm_job_vec_ok (2-wide operand loads).
+template <typename T>
+bool gemm_job_vec_ok(bool ta, bool tb, const GemmJob<T>& j);
+template <typename T>
+int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* jobs_dev,
+                int njobs, const GemmJob<T>& single, long total_wg,
+                const int* abort, hipStream_t s);
+
+// Convenience: one GEMM call (BLAS semantics incl. tri for SYRK).
+template <typename T>
+int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A,
+                int lda, const T* B, int ldb, T beta, T* C, int ldc, int tri,
+                const int* abort, hipStream_t s);
+
+// ------------------------------------------------------------ TRSM family
+// Effective left solve  Tm X = alpha V  on L-long vectors, NV of them.
+//   Tm(i,k) = ttrans ? A[k + i*lda] : A[i + k*lda], lower or upper.
+//   side L: V(i,j) = B[i + j*ldb];  side R: V(i,j) = B[j + i*ldb].
+template <typename T>
+struct TrsmJob {
+  const T* A;
+  T* B;
+  const T* inv;  // inverted 32x32 diagonal blocks of Tm (device)
+  int L, NV;
+  int lda, ldb;
+  T alpha;
+  int wg_begin;
+};
+
+struct TrsmShape {
+  bool lower;   // Tm lower triangular
+  bool ttrans;  // Tm read transposed from A
+  bool side_l;  // vectors are columns of B
+  bool unit;
+};
+TrsmShape trsm_shape(char side, char uplo, char transa, char diag);
+
+constexpr int TRSM_BS = 32;  // diagonal block inverted up front
+template <typename T>
+constexpr int trsm_max_len() { return sizeof(T) == 8 ? 1024 : 2048; }
+
+// Inverts the 32x32 diagonal blocks of Tm into inv (ceil(L/32)*1024 elems).
+template <typename T>
+int trinv_blocks_launch(const TrsmShape& sh, const T* A, int lda, int L, T* inv,
+                        const int* abort, hipStream_t s);
+// Batched / single blocked solve (L <= trsm_max_len<T>()).
+template <typename T>
+int trsm_launch(const TrsmShape& sh, const TrsmJob<T>* jobs_dev, int njobs,
+                const TrsmJob<T>& single, long total_wg, const int* abort,
+                hipStream_t s);
+int trsm_init_attributes();
+inline int trsm_wgs(int NV) { return (NV + 15) / 16; }
+
+// Full BLAS trsm (any size; host recursion above trsm_max_len).
+template <typename T>
+int trsm_full(char side, char uplo, char transa, char diag, int m, int n,
+              T alpha, const T* A, int lda, T* B, int ldb, const int* abort,
+              hipStream_t s);
+
+// ------------------------------------------------------------ POTRF
+// info: LAPACK info of this call (local order); abort (may be null): set to
+// global_offset + local order on failure and checked by every launch.
+template <typename T>
+int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
+               int global_offset, hipStream_t s);
+
+// ------------------------------------------------------------ vectors
+int dgemv_launch(bool trans, int m, int n, double alpha, const double* A,
+                 int lda, const double* x, int incx, double beta, double* y,
+                 int incy, const int* abort, hipStream_t s);
+
+int fill_i32_launch(int* dst, int value, size_t count, hipStream_t s);
+
+}  // namespace dm

@@ -1,0 +1,418 @@
+// GEMM / SYRK tile updates on CDNA4 matrix cores.
+//
+// Replaces the BLAS.gemm!/BLAS.syrk! tile calls of the tiled Cholesky
+// (src/array/cholesky.jl:47,50,92,95) and of trsm! (src/array/trsm.jl:99-183).
+//
+// Design (gfx950, wave64):
+//  * FP64 uses v_mfma_f64_16x16x4_f64, FP32 uses v_mfma_f32_16x16x4_f32.
+//    Each wave owns a (BM/WM) x (BN/WN) block of C as 16x16 MFMA tiles.
+//  * The MFMA computes C^T tiles (operand roles swapped) so a lane holds 16
+//    consecutive rows of C across a quarter-wave: column-major stores of C
+//    are 128-byte contiguous per quarter-wave.
+//  * Operand tiles are staged global -> registers -> LDS with two LDS
+//    buffers (one barrier per BK step).  The LDS layout follows the global
+//    contiguous axis: k-contiguous operands are stored [mn][BK+pad] and read
+//    as 16 rows x 4 k per MFMA; mn-contiguous operands are stored
+//    [k][MN+16] so a quarter-wave reads 128 contiguous bytes.
+//  * A launch is a batch of independent jobs (one per Dagger tile task);
+//    workgroups find their job by binary search over wg_begin, and the
+//    blockIdx -> logical workgroup map keeps consecutive tiles of one job on
+//    one XCD (blockIdx is dealt round-robin over the 8 XCDs).
+//  * SYRK is the same kernel with a triangle mask: only workgroup tiles that
+//    touch the requested triangle are launched, and diagonal tiles store
+//    masked.
+#include "dm_internal.h"
+
+namespace dm {
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T, int V> struct VecT;
+template <> struct VecT<double, 1> { typedef double t; };
+template <> struct VecT<double, 2> { typedef double t __attribute__((ext_vector_type(2))); };
+template <> struct VecT<float, 1> { typedef float t; };
+template <> struct VecT<float, 2> { typedef float t __attribute__((ext_vector_type(2))); };
+
+// Logical workgroup index such that each XCD receives a contiguous range.
+__device__ __forceinline__ int xcd_remap(int bid, int nwg) {
+  const int q = nwg >> 3, r = nwg & 7;
+  const int xcd = bid & 7, slot = bid >> 3;
+  const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return base + slot;
+}
+
+// Decode the idx-th tile (column by column) of an upper triangle of tiles.
+__device__ __forceinline__ void upper_tri_decode(int idx, int& tm, int& tn) {
+  int c = (int)((sqrtf(8.0f * (float)idx + 1.0f) - 1.0f) * 0.5f);
+  while ((c + 1) * (c + 2) / 2 <= idx) ++c;
+  while (c * (c + 1) / 2 > idx) --c;
+  tn = c;
+  tm = idx - c * (c + 1) / 2;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
+struct GemmKernel {
+  static constexpr int NT = 64 * WM * WN;
+  static constexpr int WTM = BM / WM, WTN = BN / WN;
+  static constexpr int FM = WTM / 16, FN = WTN / 16;
+  static constexpr int KP = 2;         // pad of k-contiguous rows
+  static constexpr int SK = BK + KP;   // stride of [mn][k] layout
+  static constexpr int SMA = BM + 16;  // stride of [k][m] layout
+  static constexpr int SNB = BN + 16;  // stride of [k][n] layout
+  static constexpr int A_ELEMS = TA ? BM * SK : BK * SMA;
+  static constexpr int B_ELEMS = TB ? BK * SNB : BN * SK;
+  static constexpr int LA = (BM * BK) / (VEC * NT);
+  static constexpr int LB = (BN * BK) / (VEC * NT);
+  static_assert(LA * VEC * NT == BM * BK, "A tile not divisible");
+  static_assert(LB * VEC * NT == BN * BK, "B tile not divisible");
+  static_assert(FM * 16 * WM == BM && FN * 16 * WN == BN, "wave tiling");
+  static_assert(BK % 4 == 0, "BK");
+  typedef typename Mfma<T>::v4 v4;
+  typedef typename VecT<T, VEC>::t vt;
+
+  static __device__ __forceinline__ void load_a(const T* __restrict__ A, int lda, int m0, int k0,
+                                                int M, int K, T (&ra)[LA][VEC]) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = threadIdx.x + i * NT;
+      int gm, gk;
+      if (TA) {  // A[k + m*lda], contiguous along k
+        gk = k0 + (e % (BK / VEC)) * VEC;
+        gm = m0 + e / (BK / VEC);
+        const bool ok = gm < M && gk < K;
+        const T* p = A + (size_t)gm * lda + gk;
+        if (VEC == 2) {
+          vt v = ok ? *(const vt*)p : vt{};
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) ra[i][q] = reinterpret_cast<const T*>(&v)[q];
+        } else {
+          ra[i][0] = ok ? *p : T(0);
+        }
+      } else {   // A[m + k*lda], contiguous along m
+        gm = m0 + (e % (BM / VEC)) * VEC;
+        gk = k0 + e / (BM / VEC);
+        const bool ok = gm < M && gk < K;
+        const T* p = A + (size_t)gk * lda + gm;
+        if (VEC == 2) {
+          vt v = ok ? *(const vt*)p : vt{};
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) ra[i][q] = reinterpret_cast<const T*>(&v)[q];
+        } else {
+          ra[i][0] = ok ? *p : T(0);
+        }
+      }
+    }
+  }
+  static __device__ __forceinline__ void store_a(T* __restrict__ As, const T (&ra)[LA][VEC]) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = threadIdx.x + i * NT;
+      if (TA) {
+        const int k = (e % (BK / VEC)) * VEC, m = e / (BK / VEC);
+        if (VEC == 2) *(vt*)&As[m * SK + k] = *(const vt*)&ra[i][0];
+        else As[m * SK + k] = ra[i][0];
+      } else {
+        const int m = (e % (BM / VEC)) * VEC, k = e / (BM / VEC);
+        if (VEC == 2) *(vt*)&As[k * SMA + m] = *(const vt*)&ra[i][0];
+        else As[k * SMA + m] = ra[i][0];
+      }
+    }
+  }
+  static __device__ __forceinline__ void load_b(const T* __restrict__ B, int ldb, int n0, int k0,
+                                                int N, int K, T (&rb)[LB][VEC]) {
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = threadIdx.x + i * NT;
+      int gn, gk;
+      if (!TB) {  // B[k + n*ldb], contiguous along k
+        gk = k0 + (e % (BK / VEC)) * VEC;
+        gn = n0 + e / (BK / VEC);
+        const bool ok = gn < N && gk < K;
+        const T* p = B + (size_t)gn * ldb + gk;
+        if (VEC == 2) {
+          vt v = ok ? *(const vt*)p : vt{};
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) rb[i][q] = reinterpret_cast<const T*>(&v)[q];
+        } else {
+          rb[i][0] = ok ? *p : T(0);
+        }
+      } else {    // B[n + k*ldb], contiguous along n
+        gn = n0 + (e % (BN / VEC)) * VEC;
+        gk = k0 + e / (BN / VEC);
+        const bool ok = gn < N && gk < K;
+        const T* p = B + (size_t)gk * ldb + gn;
+        if (VEC == 2) {
+          vt v = ok ? *(const vt*)p : vt{};
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) rb[i][q] = reinterpret_cast<const T*>(&v)[q];
+        } else {
+          rb[i][0] = ok ? *p : T(0);
+        }
+      }
+    }
+  }
+  static __device__ __forceinline__ void store_b(T* __restrict__ Bs, const T (&rb)[LB][VEC]) {
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = threadIdx.x + i * NT;
+      if (!TB) {
+        const int k = (e % (BK / VEC)) * VEC, n = e / (BK / VEC);
+        if (VEC == 2) *(vt*)&Bs[n * SK + k] = *(const vt*)&rb[i][0];
+        else Bs[n * SK + k] = rb[i][0];
+      } else {
+        const int n = (e % (BN / VEC)) * VEC, k = e / (BN / VEC);
+        if (VEC == 2) *(vt*)&Bs[k * SNB + n] = *(const vt*)&rb[i][0];
+        else Bs[k * SNB + n] = rb[i][0];
+      }
+    }
+  }
+  // op(A)[m][k] from LDS
+  static __device__ __forceinline__ T frag_a(const T* __restrict__ As, int m, int k) {
+    return TA ? As[m * SK + k] : As[k * SMA + m];
+  }
+  // op(B)[k][n] from LDS
+  static __device__ __forceinline__ T frag_b(const T* __restrict__ Bs, int k, int n) {
+    return TB ? Bs[k * SNB + n] : Bs[n * SK + k];
+  }
+
+  static __device__ __forceinline__ void run(const GemmJob<T>& job, int tm, int tn, T* smem) {
+    T* As0 = smem;
+    T* Bs0 = smem + A_ELEMS;
+    T* As1 = smem + A_ELEMS + B_ELEMS;
+    T* Bs1 = As1 + A_ELEMS;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm0 = (wave % WM) * WTM;
+    const int wn0 = (wave / WM) * WTN;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int M = job.M, N = job.N, K = job.K;
+
+    v4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = v4{0, 0, 0, 0};
+
+    T ra[LA][VEC], rb[LB][VEC];
+    const int nk = (K + BK - 1) / BK;
+    if (nk > 0) {
+      load_a(job.A, job.lda, m0, 0, M, K, ra);
+      load_b(job.B, job.ldb, n0, 0, N, K, rb);
+      store_a(As0, ra);
+      store_b(Bs0, rb);
+    }
+    __syncthreads();
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int kt = 0; kt < nk; ++kt) {
+      const bool odd = kt & 1;
+      const T* As = odd ? As1 : As0;
+      const T* Bs = odd ? Bs1 : Bs0;
+      const bool more = kt + 1 < nk;
+      if (more) {
+        load_a(job.A, job.lda, m0, (kt + 1) * BK, M, K, ra);
+        load_b(job.B, job.ldb, n0, (kt + 1) * BK, N, K, rb);
+      }
+#pragma unroll
+      for (int kk = 0; kk < BK / 4; ++kk) {
+        T af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = frag_a(As, wm0 + i * 16 + lr, kk * 4 + lk);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = frag_b(Bs, kk * 4 + lk, wn0 + j * 16 + lr);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mac(bf[j], af[i], acc[i][j]);
+      }
+      if (more) {
+        store_a(odd ? As0 : As1, ra);
+        store_b(odd ? Bs0 : Bs1, rb);
+      }
+      __syncthreads();
+    }
+
+    // Epilogue: lane holds C[m][n] with m = .. + lr, n = .. + 4*lk + r.
+    const T alpha = job.alpha, beta = job.beta;
+    const int tri = job.tri;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int m = m0 + wm0 + i * 16 + lr;
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn0 + j * 16 + 4 * lk + r;
+          bool ok = m < M && n < N;
+          if (tri == 1) ok = ok && m <= n;
+          else if (tri == 2) ok = ok && m >= n;
+          if (ok) {
+            T* c = job.C + (size_t)n * job.ldc + m;
+            const T v = alpha * acc[i][j][r];
+            *c = (beta == T(0)) ? v : v + beta * *c;
+          }
+        }
+      }
+    }
+  }
+};
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __restrict__ jobs,
+                                                            int njobs, GemmJob<T> single,
+                                                            const int* __restrict__ abort) {
+  typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC> G;
+  __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
+  if (abort && *abort) return;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  GemmJob<T> job;
+  int local;
+  if (njobs == 0) {
+    job = single;
+    local = wg;
+  } else {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
+    }
+    job = jobs[lo];
+    local = wg - job.wg_begin;
+  }
+  int tm, tn;
+  if (job.tri == 0) {
+    tm = local % job.tiles_m;
+    tn = local / job.tiles_m;
+  } else if (job.tri == 1) {
+    upper_tri_decode(local, tm, tn);
+  } else {
+    upper_tri_decode(local, tn, tm);
+  }
+  G::run(job, tm, tn, smem);
+}
+
+// ------------------------------------------------------------ host side
+namespace {
+struct CfgDesc { int bm, bn; };
+// 0: 128x128 (2x2 waves of 64x64), 1: 64x64 (2x2 waves of 32x32)
+const CfgDesc kCfgs[] = {{128, 128}, {64, 64}};
+}
+
+template <typename T>
+GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs) {
+  (void)K;
+  // Prefer the big tile when the launch has enough workgroups to fill the
+  // 256 CUs; otherwise trade reuse for parallelism.
+  const long big = (long)((M + 127) / 128) * ((N + 127) / 128) * (njobs > 0 ? njobs : 1);
+  const int id = big >= 192 ? 0 : 1;
+  return GemmCfgId{id, kCfgs[id].bm, kCfgs[id].bn};
+}
+
+template <typename T>
+long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg) {
+  long total = 0;
+  for (int i = 0; i < njobs; ++i) {
+    GemmJob<T>& j = jobs[i];
+    j.tiles_m = (j.M + cfg.bm - 1) / cfg.bm;
+    j.tiles_n = (j.N + cfg.bn - 1) / cfg.bn;
+    j.wg_begin = (int)total;
+    long cnt;
+    if (j.tri == 0) cnt = (long)j.tiles_m * j.tiles_n;
+    else cnt = (long)j.tiles_m * (j.tiles_m + 1) / 2;  // square (M == N)
+    if (j.M <= 0 || j.N <= 0) cnt = 0;
+    total += cnt;
+  }
+  return total;
+}
+
+namespace {
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int VEC>
+int launch_one(const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total,
+               const int* abort, hipStream_t s) {
+  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16, WM, WN, TA, TB, VEC>), dim3((unsigned)total),
+                     dim3(64 * WM * WN), 0, s, jobs, njobs, single, abort);
+  DM_LAUNCH_CHECK("gemm_kernel");
+  return 0;
+}
+template <typename T, bool TA, bool TB, int VEC>
+int launch_cfg(int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total,
+               const int* abort, hipStream_t s) {
+  if (id == 0) return launch_one<T, 128, 128, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, abort, s);
+  return launch_one<T, 64, 64, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, abort, s);
+}
+template <typename T, int VEC>
+int launch_tt(bool ta, bool tb, int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single,
+              long total, const int* abort, hipStream_t s) {
+  if (ta && tb) return launch_cfg<T, true, true, VEC>(id, jobs, njobs, single, total, abort, s);
+  if (ta) return launch_cfg<T, true, false, VEC>(id, jobs, njobs, single, total, abort, s);
+  if (tb) return launch_cfg<T, false, true, VEC>(id, jobs, njobs, single, total, abort, s);
+  return launch_cfg<T, false, false, VEC>(id, jobs, njobs, single, total, abort, s);
+}
+// Vector (2-wide) loads need: even leading dimensions, 2-element aligned
+// pointers and even extents along each operand's contiguous axis.
+template <typename T>
+bool job_vec_ok(bool ta, bool tb, const GemmJob<T>& j) {
+  const uintptr_t al = 2 * sizeof(T) - 1;
+  if (((uintptr_t)j.A & al) || ((uintptr_t)j.B & al)) return false;
+  if ((j.lda & 1) || (j.ldb & 1)) return false;
+  if (ta ? (j.K & 1) : (j.M & 1)) return false;
+  if (tb ? (j.N & 1) : (j.K & 1)) return false;
+  return true;
+}
+}  // namespace
+
+template <typename T>
+int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* jobs_dev, int njobs,
+                const GemmJob<T>& single, long total_wg, const int* abort, hipStream_t s) {
+  if (total_wg <= 0) return 0;
+  if (total_wg > 0x7fffffffL) return arg_fail(0, "gemm: too many workgroups");
+  if (vec) return launch_tt<T, 2>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, abort, s);
+  return launch_tt<T, 1>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, abort, s);
+}
+
+template <typename T>
+bool gemm_job_vec_ok(bool ta, bool tb, const GemmJob<T>& j) { return job_vec_ok(ta, tb, j); }
+
+template <typename T>
+int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A, int lda, const T* B,
+                int ldb, T beta, T* C, int ldc, int tri, const int* abort, hipStream_t s) {
+  if (M <= 0 || N <= 0) return 0;
+  GemmJob<T> j{};
+  j.A = A; j.B = B; j.C = C;
+  j.M = M; j.N = N; j.K = K;
+  j.lda = lda; j.ldb = ldb; j.ldc = ldc;
+  j.alpha = alpha; j.beta = beta; j.tri = tri;
+  if (K <= 0 || alpha == T(0)) { j.alpha = T(0); j.K = 0; }
+  GemmCfgId cfg = gemm_pick_cfg<T>(M, N, K, 1);
+  long total = gemm_prepare_jobs<T>(&j, 1, cfg);
+  return gemm_launch<T>(ta, tb, job_vec_ok(ta, tb, j), cfg, nullptr, 0, j, total, abort, s);
+}
+
+template GemmCfgId gemm_pick_cfg<double>(int, int, int, int);
+template GemmCfgId gemm_pick_cfg<float>(int, int, int, int);
+template long gemm_prepare_jobs<double>(GemmJob<double>*, int, GemmCfgId);
+template long gemm_prepare_jobs<float>(GemmJob<float>*, int, GemmCfgId);
+template int gemm_launch<double>(bool, bool, bool, GemmCfgId, const GemmJob<double>*, int,
+                                 const GemmJob<double>&, long, const int*, hipStream_t);
+template int gemm_launch<float>(bool, bool, bool, GemmCfgId, const GemmJob<float>*, int,
+                                const GemmJob<float>&, long, const int*, hipStream_t);
+template bool gemm_job_vec_ok<double>(bool, bool, const GemmJob<double>&);
+template bool gemm_job_vec_ok<float>(bool, bool, const GemmJob<float>&);
+template int gemm_single<double>(bool, bool, int, int, int, double, const double*, int,
+                                 const double*, int, double, double*, int, int, const int*,
+                                 hipStream_t);
+template int gemm_single<float>(bool, bool, int, int, int, float, const float*, int, const float*,
+                                int, float, float*, int, int, const int*, hipStream_t);
+
+}  // namespace dm

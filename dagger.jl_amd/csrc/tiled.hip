@@ -1,0 +1,439 @@
+// Whole-matrix forms: the exact task DAGs of Dagger's tiled Cholesky and
+// tiled triangular solves, built on resident tiles and run through the
+// datadeps executor with a cached plan (and hipGraph) per matrix.
+//
+//   _chol!(A, UpperTriangular)   src/array/cholesky.jl:19-63 (loop :38-53)
+//   _chol!(A, LowerTriangular)   src/array/cholesky.jl:64-108 (loop :83-98)
+//   trsm!(side, uplo, trans, diag, alpha, A, B)   src/array/trsm.jl:59-192
+//   ldiv!(C::Cholesky, B)        src/array/linalg.jl:385-401
+//
+// Tiles follow Dagger's chunk layout (each chunk a dense column-major array
+// with ld = its row count, src/array/darray.jl:568-581) packed in one HBM
+// allocation per matrix, so every task argument is a resident device
+// pointer and no per-region copies exist (the role of generate_slot!/
+// move_rewrap in src/datadeps/aliasing.jl:920-1060 disappears).
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <unordered_map>
+#include <vector>
+#include "dag.h"
+
+namespace dm {
+
+int copy_i32_launch(int* dst, const int* src, size_t count, hipStream_t s);
+
+namespace {
+
+struct TileView {
+  char* base;
+  int64_t m, n;
+  int mb, nb;
+  int dtype;
+  size_t es;
+  int mt, nt;
+  explicit TileView(const dm_tiled& t)
+      : base((char*)t.base), m(t.m), n(t.n), mb(t.mb), nb(t.nb), dtype(t.dtype),
+        es(t.dtype == DM_F32 ? 4 : 8) {
+    mt = (int)((m + mb - 1) / mb);
+    nt = (int)((n + nb - 1) / nb);
+  }
+  int rows(int i) const { return (int)std::min<int64_t>(mb, m - (int64_t)i * mb); }
+  int cols(int j) const { return (int)std::min<int64_t>(nb, n - (int64_t)j * nb); }
+  void* ptr(int i, int j) const {
+    return base + es * ((size_t)j * nb * m + (size_t)i * mb * cols(j));
+  }
+  int ld(int i) const { return rows(i); }
+};
+
+int check_tiled(const dm_tiled* A, int argno) {
+  if (!A) return arg_fail(argno, "tiled matrix is NULL");
+  if (!A->base && A->m > 0 && A->n > 0) return arg_fail(argno, "tiled base is NULL");
+  if (A->m < 0 || A->n < 0) return arg_fail(argno, "negative size");
+  if (A->mb <= 0 || A->nb <= 0) return arg_fail(argno, "block size must be positive");
+  if (A->dtype != DM_F64 && A->dtype != DM_F32) return arg_fail(argno, "unknown dtype");
+  return 0;
+}
+
+dm_task make(int op, int dtype) {
+  dm_task t;
+  std::memset(&t, 0, sizeof t);
+  t.op = op;
+  t.dtype = dtype;
+  t.side = t.uplo = t.transa = t.transb = t.diag = 'N';
+  return t;
+}
+
+void add_potrf(Dag& d, const TileView& A, int k, char uplo, int* info) {
+  dm_task t = make(DM_OP_POTRF, A.dtype);
+  t.uplo = uplo;
+  t.n = A.rows(k);
+  t.k = k * A.mb;  // global row offset (for the global info)
+  t.c = A.ptr(k, k);
+  t.ldc = A.ld(k);
+  t.info = info;
+  t.access[2] = DM_INOUT;
+  d.add(t);
+}
+
+void add_trsm(Dag& d, int dtype, char side, char uplo, char trans, char diag, double alpha,
+              const void* a, int lda, void* b, int m, int n, int ldb) {
+  dm_task t = make(DM_OP_TRSM, dtype);
+  t.side = side; t.uplo = uplo; t.transa = trans; t.diag = diag;
+  t.m = m; t.n = n;
+  t.alpha = alpha;
+  t.a = a; t.lda = lda;
+  t.c = b; t.ldc = ldb;
+  t.access[0] = DM_IN;
+  t.access[2] = DM_INOUT;
+  d.add(t);
+}
+
+void add_syrk(Dag& d, int dtype, char uplo, char trans, int n, int k, double alpha, const void* a,
+              int lda, double beta, void* c, int ldc) {
+  dm_task t = make(DM_OP_SYRK, dtype);
+  t.uplo = uplo; t.transa = trans;
+  t.n = n; t.k = k;
+  t.alpha = alpha; t.beta = beta;
+  t.a = a; t.lda = lda;
+  t.c = c; t.ldc = ldc;
+  t.access[0] = DM_IN;
+  t.access[2] = DM_INOUT;
+  d.add(t);
+}
+
+void add_gemm(Dag& d, int dtype, char ta, char tb, int m, int n, int k, double alpha,
+              const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc) {
+  dm_task t = make(DM_OP_GEMM, dtype);
+  t.transa = ta; t.transb = tb;
+  t.m = m; t.n = n; t.k = k;
+  t.alpha = alpha; t.beta = beta;
+  t.a = a; t.lda = lda;
+  t.b = b; t.ldb = ldb;
+  t.c = c; t.ldc = ldc;
+  t.access[0] = DM_IN;
+  t.access[1] = DM_IN;
+  t.access[2] = DM_INOUT;
+  d.add(t);
+}
+
+// src/array/cholesky.jl:38-53 (U) and :83-98 (L).
+void build_cholesky(Dag& d, const TileView& A, char uplo, int* info) {
+  const int mt = A.mt, nt = A.nt;
+  const int dt = A.dtype;
+  if (is_upper(uplo)) {
+    for (int k = 0; k < mt; ++k) {
+      add_potrf(d, A, k, 'U', info);
+      for (int n = k + 1; n < nt; ++n)
+        add_trsm(d, dt, 'L', 'U', 'T', 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(k, n), A.rows(k),
+                 A.cols(n), A.ld(k));
+      for (int m = k + 1; m < mt; ++m) {
+        add_syrk(d, dt, 'U', 'T', A.cols(m), A.rows(k), -1.0, A.ptr(k, m), A.ld(k), 1.0,
+                 A.ptr(m, m), A.ld(m));
+        for (int n = m + 1; n < nt; ++n)
+          add_gemm(d, dt, 'T', 'N', A.rows(m), A.cols(n), A.rows(k), -1.0, A.ptr(k, m), A.ld(k),
+                   A.ptr(k, n), A.ld(k), 1.0, A.ptr(m, n), A.ld(m));
+      }
+    }
+  } else {
+    for (int k = 0; k < mt; ++k) {
+      add_potrf(d, A, k, 'L', info);
+      for (int m = k + 1; m < mt; ++m)
+        add_trsm(d, dt, 'R', 'L', 'T', 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(m, k), A.rows(m),
+                 A.cols(k), A.ld(m));
+      for (int n = k + 1; n < nt; ++n) {
+        add_syrk(d, dt, 'L', 'N', A.rows(n), A.cols(k), -1.0, A.ptr(n, k), A.ld(n), 1.0,
+                 A.ptr(n, n), A.ld(n));
+        for (int m = n + 1; m < mt; ++m)
+          add_gemm(d, dt, 'N', 'T', A.rows(m), A.rows(n), A.cols(k), -1.0, A.ptr(m, k), A.ld(m),
+                   A.ptr(n, k), A.ld(n), 1.0, A.ptr(m, n), A.ld(m));
+      }
+    }
+  }
+}
+
+// src/array/trsm.jl:88-190, 0-based (Dagger's index (Bmt-k)+1 is Bmt-k here
+// with k counted from 1).
+void build_trsm(Dag& d, char side, char uplo, char trans, char diag, double alpha,
+                const TileView& A, const TileView& B) {
+  const int dt = B.dtype;
+  const int Bmt = B.mt, Bnt = B.nt;
+  const char tr = is_trans(trans) ? 'T' : 'N';
+  auto trsm = [&](double al, int ai, int bi, int bj) {
+    add_trsm(d, dt, side, uplo, tr, diag, al, A.ptr(ai, ai), A.ld(ai), B.ptr(bi, bj), B.rows(bi),
+             B.cols(bj), B.ld(bi));
+  };
+  // C[ci,cj] = -op(X) op(Y) + beta C, with X, Y tiles of A or B
+  auto gemm = [&](char ta, char tb, const TileView& X, int xi, int xj, const TileView& Y, int yi,
+                  int yj, double beta, int ci, int cj) {
+    const int M = B.rows(ci), N = B.cols(cj);
+    const int K = is_trans(ta) ? X.rows(xi) : X.cols(xj);
+    add_gemm(d, dt, ta, tb, M, N, K, -1.0, X.ptr(xi, xj), X.ld(xi), Y.ptr(yi, yj), Y.ld(yi), beta,
+             B.ptr(ci, cj), B.ld(ci));
+  };
+  if (is_left(side)) {
+    if (is_upper(uplo)) {
+      if (tr == 'N') {
+        for (int k = 1; k <= Bmt; ++k) {
+          const double la = k == 1 ? alpha : 1.0;
+          const int kk = Bmt - k;
+          for (int n = 0; n < Bnt; ++n) trsm(la, kk, kk, n);
+          for (int m = k + 1; m <= Bmt; ++m)
+            for (int n = 0; n < Bnt; ++n) gemm('N', 'N', A, Bmt - m, kk, B, kk, n, la, Bmt - m, n);
+        }
+      } else {
+        for (int k = 0; k < Bmt; ++k) {
+          const double la = k == 0 ? alpha : 1.0;
+          for (int n = 0; n < Bnt; ++n) trsm(la, k, k, n);
+          for (int m = k + 1; m < Bmt; ++m)
+            for (int n = 0; n < Bnt; ++n) gemm(tr, 'N', A, k, m, B, k, n, la, m, n);
+        }
+      }
+    } else {
+      if (tr == 'N') {
+        for (int k = 0; k < Bmt; ++k) {
+          const double la = k == 0 ? alpha : 1.0;
+          for (int n = 0; n < Bnt; ++n) trsm(la, k, k, n);
+          for (int m = k + 1; m < Bmt; ++m)
+            for (int n = 0; n < Bnt; ++n) gemm('N', 'N', A, m, k, B, k, n, la, m, n);
+        }
+      } else {
+        for (int k = 1; k <= Bmt; ++k) {
+          const double la = k == 1 ? alpha : 1.0;
+          const int kk = Bmt - k;
+          for (int n = 0; n < Bnt; ++n) trsm(la, kk, kk, n);
+          for (int m = k + 1; m <= Bmt; ++m)
+            for (int n = 0; n < Bnt; ++n) gemm(tr, 'N', A, kk, Bmt - m, B, kk, n, la, Bmt - m, n);
+        }
+      }
+    }
+  } else {
+    if (is_upper(uplo)) {
+      if (tr == 'N') {
+        for (int k = 0; k < Bnt; ++k) {
+          const double la = k == 0 ? alpha : 1.0;
+          for (int m = 0; m < Bmt; ++m) trsm(la, k, m, k);
+          for (int m = 0; m < Bmt; ++m)
+            for (int n = k + 1; n < Bnt; ++n) gemm('N', 'N', B, m, k, A, k, n, la, m, n);
+        }
+      } else {
+        for (int k = 1; k <= Bnt; ++k) {
+          const int kk = Bnt - k;
+          for (int m = 0; m < Bmt; ++m) {
+            trsm(alpha, kk, m, kk);
+            for (int n = k + 1; n <= Bnt; ++n)
+              gemm('N', tr, B, m, kk, A, Bnt - n, kk, 1.0, m, Bnt - n);
+          }
+        }
+      }
+    } else {
+      if (tr == 'N') {
+        for (int k = 1; k <= Bnt; ++k) {
+          const double la = k == 1 ? alpha : 1.0;
+          const int kk = Bnt - k;
+          for (int m = 0; m < Bmt; ++m) {
+            trsm(la, kk, m, kk);
+            for (int n = k + 1; n <= Bnt; ++n)
+              gemm('N', 'N', B, m, kk, A, kk, Bnt - n, la, m, Bnt - n);
+          }
+        }
+      } else {
+        for (int k = 0; k < Bnt; ++k) {
+          for (int m = 0; m < Bmt; ++m) {
+            trsm(alpha, k, m, k);
+            for (int n = k + 1; n < Bnt; ++n) gemm('N', tr, B, m, k, A, n, k, 1.0, m, n);
+          }
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ plan cache
+struct CacheEntry {
+  std::unique_ptr<Dag> dag;
+  std::vector<const void*> bases;
+};
+std::mutex g_cache_mu;
+std::unordered_map<std::string, CacheEntry> g_cache;
+
+std::string key_of(const char* kind, std::initializer_list<const void*> ptrs,
+                   std::initializer_list<int64_t> nums, const dm_plan_opts& o) {
+  std::ostringstream os;
+  os << kind;
+  for (const void* p : ptrs) os << ':' << p;
+  for (int64_t v : nums) os << ':' << v;
+  os << ':' << o.num_streams << ',' << o.batch << ',' << o.lookahead << ',' << o.use_graph;
+  return os.str();
+}
+
+dm_plan_opts opts_or_default(const dm_plan_opts* o) {
+  if (o) return *o;
+  return dm_plan_opts{2, 1, 1, 1};
+}
+
+// Looks up or builds (build(d) fills tasks) and plans a DAG.
+template <typename F>
+int get_plan(const std::string& key, std::vector<const void*> bases, const dm_plan_opts& o,
+             F build, Dag** out) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_cache.find(key);
+  if (it != g_cache.end()) {
+    *out = it->second.dag.get();
+    return 0;
+  }
+  std::unique_ptr<Dag> d(new Dag());
+  int rc = build(*d);
+  if (rc) return rc;
+  rc = d->plan(o);
+  if (rc) return rc;
+  *out = d.get();
+  CacheEntry e;
+  e.dag = std::move(d);
+  e.bases = std::move(bases);
+  g_cache.emplace(key, std::move(e));
+  return 0;
+}
+
+}  // namespace
+}  // namespace dm
+
+using namespace dm;
+
+extern "C" {
+
+int dm_cholesky_tiled(char uplo, const dm_tiled* A, int* info_dev, int* global_info_dev,
+                      const dm_plan_opts* opts, dm_stream_t s) {
+  if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(1, "uplo must be 'U' or 'L'");
+  int rc = check_tiled(A, 2);
+  if (rc) return rc;
+  if (A->m != A->n) return arg_fail(2, "matrix must be square");
+  if (A->mb != A->nb) return arg_fail(2, "tiles must be square (Blocks(nb, nb))");
+  if (!info_dev) return arg_fail(3, "info is NULL");
+  rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = stream_of(s);
+  // info = [0] (src/array/cholesky.jl:32)
+  rc = fill_i32_launch(info_dev, 0, 1, st);
+  if (rc) return rc;
+  if (A->m == 0) {
+    if (global_info_dev) return fill_i32_launch(global_info_dev, 0, 1, st);
+    return 0;
+  }
+  const dm_plan_opts o = opts_or_default(opts);
+  TileView tv(*A);
+  const std::string key = key_of("chol", {A->base, info_dev, global_info_dev},
+                                 {A->m, A->mb, A->dtype, is_upper(uplo)}, o);
+  Dag* d = nullptr;
+  rc = get_plan(key, {A->base, info_dev, global_info_dev}, o,
+                [&](Dag& dag) {
+                  build_cholesky(dag, tv, is_upper(uplo) ? 'U' : 'L', info_dev);
+                  if (global_info_dev)
+                    dag.post_hook = [global_info_dev](Dag* self, hipStream_t hs) {
+                      return copy_i32_launch(global_info_dev, self->abort_dev, 1, hs);
+                    };
+                  return 0;
+                },
+                &d);
+  if (rc) return rc;
+  return d->run(st);
+}
+
+int dm_trsm_tiled(char side, char uplo, char trans, char diag, double alpha, const dm_tiled* A,
+                  const dm_tiled* B, const dm_plan_opts* opts, dm_stream_t s) {
+  if (!is_left(side) && !is_right(side)) return arg_fail(1, "side must be 'L' or 'R'");
+  if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(2, "uplo must be 'U' or 'L'");
+  if (!is_trans(trans) && !is_notrans(trans)) return arg_fail(3, "trans must be 'N','T','C'");
+  if (!is_unit(diag) && !is_nonunit(diag)) return arg_fail(4, "diag must be 'N' or 'U'");
+  int rc = check_tiled(A, 6);
+  if (rc) return rc;
+  rc = check_tiled(B, 7);
+  if (rc) return rc;
+  if (A->m != A->n || A->mb != A->nb) return arg_fail(6, "A must be square with square tiles");
+  if (A->dtype != B->dtype) return arg_fail(7, "A and B dtypes differ");
+  if (B->mb != A->mb || B->nb != A->nb) return arg_fail(7, "B must use A's block size");
+  if (is_left(side) ? B->m != A->m : B->n != A->n) return arg_fail(7, "B does not conform");
+  rc = ensure_init();
+  if (rc) return rc;
+  if (B->m == 0 || B->n == 0) return 0;
+  const dm_plan_opts o = opts_or_default(opts);
+  TileView ta(*A), tb(*B);
+  const char up = is_upper(uplo) ? 'U' : 'L';
+  const char sd = is_left(side) ? 'L' : 'R';
+  const char tr = is_trans(trans) ? 'T' : 'N';
+  const char dg = is_unit(diag) ? 'U' : 'N';
+  char tag[16];
+  snprintf(tag, sizeof tag, "trsm%c%c%c%c", sd, up, tr, dg);
+  int64_t abits;
+  std::memcpy(&abits, &alpha, sizeof abits);
+  const std::string key =
+      key_of(tag, {A->base, B->base}, {A->m, A->mb, B->m, B->n, A->dtype, abits}, o);
+  Dag* d = nullptr;
+  rc = get_plan(key, {A->base, B->base}, o,
+                [&](Dag& dag) {
+                  build_trsm(dag, sd, up, tr, dg, alpha, ta, tb);
+                  return 0;
+                },
+                &d);
+  if (rc) return rc;
+  return d->run(stream_of(s));
+}
+
+int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B, const dm_plan_opts* opts,
+                   dm_stream_t s) {
+  if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(1, "uplo must be 'U' or 'L'");
+  int rc = check_tiled(A, 2);
+  if (rc) return rc;
+  rc = check_tiled(B, 3);
+  if (rc) return rc;
+  if (A->m != A->n || A->mb != A->nb) return arg_fail(2, "A must be square with square tiles");
+  if (A->dtype != B->dtype) return arg_fail(3, "A and B dtypes differ");
+  if (B->mb != A->mb || B->nb != A->nb) return arg_fail(3, "B must use A's block size");
+  if (B->m != A->m) return arg_fail(3, "B does not conform");
+  rc = ensure_init();
+  if (rc) return rc;
+  if (B->n == 0 || B->m == 0) return 0;
+  const dm_plan_opts o = opts_or_default(opts);
+  TileView ta(*A), tb(*B);
+  const char up = is_upper(uplo) ? 'U' : 'L';
+  const std::string key =
+      key_of(up == 'U' ? "potrsU" : "potrsL", {A->base, B->base}, {A->m, A->mb, B->n, A->dtype}, o);
+  Dag* d = nullptr;
+  rc = get_plan(key, {A->base, B->base}, o,
+                [&](Dag& dag) {
+                  // The two trsm! regions of ldiv! recorded into one DAG: the
+                  // datadeps rules order the second after the first per tile.
+                  if (up == 'U') {
+                    build_trsm(dag, 'L', 'U', 'T', 'N', 1.0, ta, tb);
+                    build_trsm(dag, 'L', 'U', 'N', 'N', 1.0, ta, tb);
+                  } else {
+                    build_trsm(dag, 'L', 'L', 'N', 'N', 1.0, ta, tb);
+                    build_trsm(dag, 'L', 'L', 'T', 'N', 1.0, ta, tb);
+                  }
+                  return 0;
+                },
+                &d);
+  if (rc) return rc;
+  return d->run(stream_of(s));
+}
+
+int dm_plan_cache_clear(void) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_cache.clear();
+  return 0;
+}
+
+int dm_forget_matrix(const void* base) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  for (auto it = g_cache.begin(); it != g_cache.end();) {
+    bool hit = false;
+    for (const void* b : it->second.bases) hit = hit || (b == base);
+    if (hit) it = g_cache.erase(it);
+    else ++it;
+  }
+  return 0;
+}
+
+}  // extern "C"

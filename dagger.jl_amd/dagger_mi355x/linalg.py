@@ -1,0 +1,282 @@
+"""cholesky! / cholesky / ldiv! / trsm! / trsv! on resident DArrays.
+
+Host mirror of Dagger's tiled linear algebra for this path:
+  * `cholesky_(A)`       LinearAlgebra.cholesky!(::DArray) -> _chol!
+                         (src/array/cholesky.jl:19-108)
+  * `cholesky(A)`        out of place via cholcopy = copy (src/array/cholesky.jl:1)
+  * `ldiv_(C, B)`        ldiv!(::Cholesky{T,<:DMatrix}, B) (src/array/linalg.jl:352-405)
+  * `trsm_`, `trsv_`     Dagger.trsm!/trsv! (src/array/trsm.jl:1-192)
+Block re-partitioning follows maybe_copy_buffered (src/array/copy.jl:3-10):
+arguments whose partitioning differs from the requested square Blocks are
+copied into a buffer of that partitioning, processed, and copied back.
+
+Two executions of the same task DAG are available:
+  * via="native": the C++ builders (dm_cholesky_tiled / dm_potrs_tiled /
+    dm_trsm_tiled) record the reference's loop nest natively and cache the
+    plan + hipGraph per matrix (the steady-state fast path);
+  * via="datadeps": the loop nest is written here with spawn/In/InOut and
+    handed to the executor, exactly as the Julia source spawns it.
+Both produce bit-identical results (same tasks, same kernels).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+from .darray import DArray, Blocks, DeviceInt
+from .datadeps import (spawn, spawn_datadeps, In, Out, InOut, BLAS, potrf_checked,
+                       PlanOptions)
+
+
+class PosDefException(Exception):
+    """LinearAlgebra.PosDefException(info)."""
+
+    def __init__(self, info):
+        super().__init__(f"matrix is not positive definite; factorization failed (info={info})")
+        self.info = info
+
+
+class Cholesky:
+    """`Cholesky{T,<:DMatrix}`: factors + uplo + info."""
+
+    def __init__(self, factors, uplo, info, global_info=0):
+        self.factors, self.uplo, self.info = factors, uplo, int(info)
+        self.global_info = int(global_info)
+
+    @property
+    def U(self):
+        F = self.factors.collect()
+        return np.triu(F) if self.uplo == "U" else np.triu(F.T)
+
+    @property
+    def L(self):
+        F = self.factors.collect()
+        return np.tril(F) if self.uplo == "L" else np.tril(F.T)
+
+    def issuccess(self):
+        return self.info == 0
+
+
+DEFAULT_OPTIONS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True)
+
+
+def _square_blocks(A):
+    bs = min(A.partitioning.blocksize)
+    return Blocks(bs, bs)
+
+
+def _maybe_copy_buffered(pairs, f):
+    """maybe_copy_buffered(A => Blocks(...), ...) (src/array/copy.jl:3-34)."""
+    if all(a.partitioning == p for a, p in pairs):
+        return f(*[a for a, _ in pairs])
+    bufs = [a.reblock(p) if a.partitioning != p else a for a, p in pairs]
+    res = f(*bufs)
+    for (a, p), b in zip(pairs, bufs):
+        if b is not a:
+            b.copy_into(a)
+            b.unsafe_free()
+    return res
+
+
+def _chol_datadeps(Ac, uplo, info, options):
+    """The loop nest of src/array/cholesky.jl:38-53 (U) / :83-98 (L)."""
+    mt, nt = Ac.shape
+    with spawn_datadeps(options=options):
+        if uplo == "U":
+            for k in range(mt):
+                spawn(potrf_checked, "U", InOut(Ac[k, k]), Out(info))
+                for n in range(k + 1, nt):
+                    spawn(BLAS.trsm, "L", "U", "T", "N", 1.0, In(Ac[k, k]), InOut(Ac[k, n]))
+                for m in range(k + 1, mt):
+                    spawn(BLAS.syrk, "U", "T", -1.0, In(Ac[k, m]), 1.0, InOut(Ac[m, m]))
+                    for n in range(m + 1, nt):
+                        spawn(BLAS.gemm, "T", "N", -1.0, In(Ac[k, m]), In(Ac[k, n]), 1.0,
+                              InOut(Ac[m, n]))
+        else:
+            for k in range(mt):
+                spawn(potrf_checked, "L", InOut(Ac[k, k]), Out(info))
+                for m in range(k + 1, mt):
+                    spawn(BLAS.trsm, "R", "L", "T", "N", 1.0, In(Ac[k, k]), InOut(Ac[m, k]))
+                for n in range(k + 1, nt):
+                    spawn(BLAS.syrk, "L", "N", -1.0, In(Ac[n, k]), 1.0, InOut(Ac[n, n]))
+                    for m in range(n + 1, mt):
+                        spawn(BLAS.gemm, "N", "T", -1.0, In(Ac[m, k]), In(Ac[n, k]), 1.0,
+                              InOut(Ac[m, n]))
+
+
+def _chol(A, uplo, via, options):
+    """LinearAlgebra._chol!(A, UpperTriangular/LowerTriangular): returns info."""
+    if A.ndim != 2 or A.m != A.n:
+        raise ValueError("DimensionMismatch: matrix is not square")
+    info = DeviceInt(1)
+    ginfo = DeviceInt(1)
+
+    def run(Ab):
+        if via == "native":
+            t = Ab.tiled()
+            o = options.c()
+            call("dm_cholesky_tiled", uplo.encode(), C.byref(t), C.c_void_p(info.ptr),
+                 C.c_void_p(ginfo.ptr), C.byref(o), None)
+            call("dm_stream_sync", None)
+        elif via == "datadeps":
+            _chol_datadeps(Ab.chunks, uplo, info, options)
+        else:
+            raise ValueError(f"unknown via={via!r}")
+
+    _maybe_copy_buffered([(A, _square_blocks(A))], run)
+    loc = int(info.get()[0])
+    g = int(ginfo.get()[0]) if via == "native" else 0
+    return loc, g
+
+
+def cholesky_(A, uplo="U", check=True, via="native", options=None):
+    """`cholesky!(A)` / `cholesky!(Hermitian(A, :L))` on a DArray, in place.
+
+    info is the reference's: the LAPACK info of the first diagonal tile
+    whose POTRF fails (tile-local, src/array/cholesky.jl:13-16,62); the
+    global leading-minor order is kept in `Cholesky.global_info`."""
+    uplo = uplo.upper()
+    if uplo not in ("U", "L"):
+        raise ValueError("uplo must be 'U' or 'L'")
+    options = options or DEFAULT_OPTIONS
+    info, ginfo = _chol(A, uplo, via, options)
+    if check and info != 0:
+        raise PosDefException(info)
+    return Cholesky(A, uplo, info, ginfo)
+
+
+def cholesky(A, uplo="U", check=True, via="native", options=None):
+    """Out-of-place `cholesky(A)`: cholcopy (a copy) then cholesky!."""
+    return cholesky_(A.copy(), uplo=uplo, check=check, via=via, options=options)
+
+
+def trsm_(side, uplo, trans, diag, alpha, A, B, via="native", options=None):
+    """Dagger.trsm!(side, uplo, trans, diag, alpha, A, B) (src/array/trsm.jl:59-192)."""
+    options = options or DEFAULT_OPTIONS
+    if via == "native":
+        ta, tb = A.tiled(), B.tiled()
+        o = options.c()
+        call("dm_trsm_tiled", side.encode(), uplo.encode(), trans.encode(), diag.encode(),
+             float(alpha), C.byref(ta), C.byref(tb), C.byref(o), None)
+        call("dm_stream_sync", None)
+        return B
+    _trsm_datadeps(side, uplo, trans, diag, alpha, A.chunks, B.chunks, options)
+    return B
+
+
+def _trsm_datadeps(side, uplo, trans, diag, alpha, Ac, Bc, options):
+    """src/array/trsm.jl:88-190 (left side; the branches ldiv! uses)."""
+    if side != "L":
+        raise NotImplementedError("datadeps trsm_ mirror covers side 'L'; use via='native'")
+    Bmt, Bnt = Bc.shape
+    with spawn_datadeps(options=options):
+        if uplo == "U" and trans == "N":
+            for k in range(1, Bmt + 1):
+                la = alpha if k == 1 else 1.0
+                kk = Bmt - k
+                for n in range(Bnt):
+                    spawn(BLAS.trsm, side, uplo, trans, diag, la, In(Ac[kk, kk]), InOut(Bc[kk, n]))
+                for m in range(k + 1, Bmt + 1):
+                    mm = Bmt - m
+                    for n in range(Bnt):
+                        spawn(BLAS.gemm, "N", "N", -1.0, In(Ac[mm, kk]), In(Bc[kk, n]), la,
+                              InOut(Bc[mm, n]))
+        elif uplo == "U":
+            for k in range(Bmt):
+                la = alpha if k == 0 else 1.0
+                for n in range(Bnt):
+                    spawn(BLAS.trsm, side, uplo, trans, diag, la, In(Ac[k, k]), InOut(Bc[k, n]))
+                for m in range(k + 1, Bmt):
+                    for n in range(Bnt):
+                        spawn(BLAS.gemm, trans, "N", -1.0, In(Ac[k, m]), In(Bc[k, n]), la,
+                              InOut(Bc[m, n]))
+        elif trans == "N":
+            for k in range(Bmt):
+                la = alpha if k == 0 else 1.0
+                for n in range(Bnt):
+                    spawn(BLAS.trsm, side, uplo, trans, diag, la, In(Ac[k, k]), InOut(Bc[k, n]))
+                for m in range(k + 1, Bmt):
+                    for n in range(Bnt):
+                        spawn(BLAS.gemm, "N", "N", -1.0, In(Ac[m, k]), In(Bc[k, n]), la,
+                              InOut(Bc[m, n]))
+        else:
+            for k in range(1, Bmt + 1):
+                la = alpha if k == 1 else 1.0
+                kk = Bmt - k
+                for n in range(Bnt):
+                    spawn(BLAS.trsm, side, uplo, trans, diag, la, In(Ac[kk, kk]), InOut(Bc[kk, n]))
+                for m in range(k + 1, Bmt + 1):
+                    mm = Bmt - m
+                    for n in range(Bnt):
+                        spawn(BLAS.gemm, trans, "N", -1.0, In(Ac[kk, mm]), In(Bc[kk, n]), la,
+                              InOut(Bc[mm, n]))
+
+
+def trsv_(uplo, trans, diag, alpha, A, b, options=None):
+    """Dagger.trsv! (src/array/trsm.jl:1-57): trsv/gemv tile tasks."""
+    options = options or PlanOptions(num_streams=1, batch=False, lookahead=0)
+    if alpha != 1.0:
+        raise NotImplementedError("trsv_ mirrors the ldiv! use (alpha = 1)")
+    Ac, Bc = A.chunks, b.chunks
+    Bnt = len(Bc)
+    with spawn_datadeps(options=options):
+        if uplo == "U" and trans == "N":
+            for k in reversed(range(Bnt)):
+                spawn(BLAS.trsv, "U", "N", diag, In(Ac[k, k]), InOut(Bc[k]))
+                for i in range(k):
+                    spawn(BLAS.gemv, "N", -1.0, In(Ac[i, k]), In(Bc[k]), 1.0, InOut(Bc[i]))
+        elif uplo == "U":
+            for k in range(Bnt):
+                spawn(BLAS.trsv, "U", trans, diag, In(Ac[k, k]), InOut(Bc[k]))
+                for i in range(k + 1, Bnt):
+                    spawn(BLAS.gemv, trans, -1.0, In(Ac[k, i]), In(Bc[k]), 1.0, InOut(Bc[i]))
+        elif trans == "N":
+            for k in range(Bnt):
+                spawn(BLAS.trsv, "L", "N", diag, In(Ac[k, k]), InOut(Bc[k]))
+                for i in range(k + 1, Bnt):
+                    spawn(BLAS.gemv, "N", -1.0, In(Ac[i, k]), In(Bc[k]), 1.0, InOut(Bc[i]))
+        else:
+            for k in reversed(range(Bnt)):
+                spawn(BLAS.trsv, "L", trans, diag, In(Ac[k, k]), InOut(Bc[k]))
+                for i in range(k):
+                    spawn(BLAS.gemv, trans, -1.0, In(Ac[k, i]), In(Bc[k]), 1.0, InOut(Bc[i]))
+    return b
+
+
+def ldiv_(F, B, via="native", options=None):
+    """`ldiv!(C::Cholesky, B)` (src/array/linalg.jl:352-405), B overwritten."""
+    options = options or DEFAULT_OPTIONS
+    pA = F.factors
+    bs = min(pA.partitioning.blocksize)
+    sq = Blocks(bs, bs)
+    if B.ndim == 1:
+        partB = Blocks(bs)
+
+        def vec(pa, pb):
+            if F.uplo == "U":
+                trsv_("U", "T", "N", 1.0, pa, pb)
+                trsv_("U", "N", "N", 1.0, pa, pb)
+            else:
+                trsv_("L", "N", "N", 1.0, pa, pb)
+                trsv_("L", "T", "N", 1.0, pa, pb)
+
+        _maybe_copy_buffered([(pA, sq), (B, partB)], vec)
+        return B
+    partB = sq
+
+    def mat(pa, pb):
+        if via == "native":
+            ta, tb = pa.tiled(), pb.tiled()
+            o = options.c()
+            call("dm_potrs_tiled", F.uplo.encode(), C.byref(ta), C.byref(tb), C.byref(o), None)
+            call("dm_stream_sync", None)
+        elif F.uplo == "U":
+            _trsm_datadeps("L", "U", "T", "N", 1.0, pa.chunks, pb.chunks, options)
+            _trsm_datadeps("L", "U", "N", "N", 1.0, pa.chunks, pb.chunks, options)
+        else:
+            _trsm_datadeps("L", "L", "N", "N", 1.0, pa.chunks, pb.chunks, options)
+            _trsm_datadeps("L", "L", "T", "N", 1.0, pa.chunks, pb.chunks, options)
+
+    _maybe_copy_buffered([(pA, sq), (B, partB)], mat)
+    return B

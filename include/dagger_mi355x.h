@@ -1,0 +1,272 @@
+/*
+ * dagger_mi355x.h — C ABI of the MI355X (gfx950) backend for Dagger.jl's
+ * tiled dense linear-algebra path: cholesky!/ldiv! on a Blocks-partitioned
+ * DArray, driven by spawn_datadeps.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status:
+ *     0      success
+ *    <0      argument error, LAPACK xerbla style: -i names the i-th argument
+ *    >0      HIP runtime error code (hipError_t value) + DM_HIP_ERROR_BASE
+ * dm_last_error_string() returns a message for the last failure on the
+ * calling host thread.
+ *
+ * Matrices are column-major with explicit leading dimensions, exactly as the
+ * BLAS/LAPACK calls they replace.  Device pointers come from dm_malloc (or any
+ * hipMalloc'd memory of the current device).  Every compute entry point is
+ * asynchronous on the given stream (NULL = the library's default stream).
+ * Calls on different streams from different host threads are safe; calls on
+ * one stream are ordered by HIP.  D->H copies synchronize their stream.
+ *
+ * Reference interfaces replaced (paths are inside JuliaParallel/Dagger.jl):
+ *   tile ops   <- the BLAS/LAPACK function objects that ext/ROCExt.jl:358-377
+ *                 remaps for a device processor, called from
+ *                 src/array/cholesky.jl:39-50,84-95 and src/array/trsm.jl:17-183
+ *   executor   <- Dagger.execute!(proc, f, args...) (src/processor.jl:40,
+ *                 ext/ROCExt.jl:338-356) and the datadeps region runner
+ *                 spawn_datadeps / distribute_tasks! (src/datadeps/queue.jl:64-471)
+ *   memory     <- move/move!/unsafe_free!/allocate_array_func for a device
+ *                 memory space (ext/ROCExt.jl:74-77,169-330, src/memory-spaces.jl:660-671)
+ *   streams    <- gpu_synchronize / with_context! (src/gpu.jl:88-120)
+ */
+#ifndef DAGGER_MI355X_H
+#define DAGGER_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DM_HIP_ERROR_BASE 1000
+
+typedef struct dm_stream_s* dm_stream_t;
+typedef struct dm_event_s* dm_event_t;
+typedef struct dm_graph_s* dm_graph_t;
+typedef struct dm_dag_s* dm_dag_t;
+
+/* element types */
+enum { DM_F64 = 0, DM_F32 = 1 };
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Select and initialise a device.  Replaces AMDGPU.device!/with_context!
+ * inside ROCExt's execute! (ext/ROCExt.jl:338-356) and gpu_synchronize
+ * (src/gpu.jl:88-117).  Idempotent. */
+int dm_init(int device);
+int dm_device_count(int* count);
+const char* dm_last_error_string(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int dm_version(void);
+
+/* Device memory.  Replaces ROCArray allocation and unsafe_free!
+ * (ext/ROCExt.jl:74-77,169-179; src/memory-spaces.jl:660-671). */
+int dm_malloc(void** ptr, size_t bytes);
+int dm_free(void* ptr);
+/* Async copies on a stream.  h2d/d2d return after enqueueing; d2h
+ * synchronizes the stream before returning (move! contract,
+ * ext/ROCExt.jl:181-192). */
+int dm_memcpy_h2d(void* dst, const void* src, size_t bytes, dm_stream_t s);
+int dm_memcpy_d2h(void* dst, const void* src, size_t bytes, dm_stream_t s);
+int dm_memcpy_d2d(void* dst, const void* src, size_t bytes, dm_stream_t s);
+/* Strided 2-D copies (column-major panels): width_bytes per column, height
+ * columns.  Used to scatter a host matrix into resident tiles and back
+ * (move(CPUProc, DevProc, ::Array), ext/ROCExt.jl:209-272). */
+int dm_memcpy2d_h2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                    size_t width_bytes, size_t height, dm_stream_t s);
+int dm_memcpy2d_d2h(void* dst, size_t dpitch, const void* src, size_t spitch,
+                    size_t width_bytes, size_t height, dm_stream_t s);
+int dm_memcpy2d_d2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                    size_t width_bytes, size_t height, dm_stream_t s);
+/* Fill an int32 device buffer with a value (a kernel, so it is safe inside
+ * graph capture). */
+int dm_fill_i32(int* dst, int value, size_t count, dm_stream_t s);
+
+/* Streams, events, graphs (ROCExt keeps one HIP stream per device,
+ * ext/ROCExt.jl:93-97,532-553; this backend uses several). */
+int dm_stream_create(dm_stream_t* s);
+int dm_stream_destroy(dm_stream_t s);
+int dm_stream_sync(dm_stream_t s);
+int dm_device_sync(void);
+int dm_event_create(dm_event_t* e);
+int dm_event_destroy(dm_event_t e);
+int dm_event_record(dm_event_t e, dm_stream_t s);
+int dm_stream_wait_event(dm_stream_t s, dm_event_t e);
+int dm_event_sync(dm_event_t e);
+int dm_event_elapsed_ms(float* ms, dm_event_t start, dm_event_t stop);
+/* Stream capture into a hipGraph; launch replays it on a stream. */
+int dm_graph_begin(dm_stream_t s);
+int dm_graph_end(dm_stream_t s, dm_graph_t* g);
+int dm_graph_launch(dm_graph_t g, dm_stream_t s);
+int dm_graph_destroy(dm_graph_t g);
+
+/* ------------------------------------------------------------- tile ops
+ * BLAS/LAPACK argument order and meaning, column-major, device pointers.
+ * These are what `Dagger.move(::CPUProc, ::DevProc, ::typeof(BLAS.gemm!))`
+ * and friends return for this backend (ext/ROCExt.jl:358-377).  Characters
+ * are 'N','T','C' (C == T for real types), 'U','L', 'L','R', 'N','U'. */
+
+/* LAPACK potrf: A = U^T U ('U') or L L^T ('L').  *info_dev (device int)
+ * receives the LAPACK info: 0, or k>0 if the leading minor of order k is not
+ * positive definite.  potrf_checked! (src/array/cholesky.jl:2-18) reads it. */
+int dm_dpotrf(char uplo, int n, double* A, int lda, int* info_dev, dm_stream_t s);
+int dm_spotrf(char uplo, int n, float* A, int lda, int* info_dev, dm_stream_t s);
+
+/* BLAS trsm: B := alpha * op(A)^-1 B (side 'L') or alpha * B op(A)^-1 ('R').
+ * Called at src/array/cholesky.jl:41,86 and src/array/trsm.jl:95-183. */
+int dm_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
+             double alpha, const double* A, int lda, double* B, int ldb,
+             dm_stream_t s);
+int dm_strsm(char side, char uplo, char transa, char diag, int m, int n,
+             float alpha, const float* A, int lda, float* B, int ldb,
+             dm_stream_t s);
+
+/* BLAS syrk: C := alpha op(A) op(A)^T + beta C on the uplo triangle only.
+ * trans 'T': op(A) = A^T (A is k x n); 'N': op(A) = A (A is n x k).
+ * Called at src/array/cholesky.jl:47,92. */
+int dm_dsyrk(char uplo, char trans, int n, int k, double alpha, const double* A,
+             int lda, double beta, double* C, int ldc, dm_stream_t s);
+int dm_ssyrk(char uplo, char trans, int n, int k, float alpha, const float* A,
+             int lda, float beta, float* C, int ldc, dm_stream_t s);
+
+/* BLAS gemm: C := alpha op(A) op(B) + beta C.  beta == 0 does not read C.
+ * Called at src/array/cholesky.jl:50,95 and src/array/trsm.jl:99-183. */
+int dm_dgemm(char transa, char transb, int m, int n, int k, double alpha,
+             const double* A, int lda, const double* B, int ldb, double beta,
+             double* C, int ldc, dm_stream_t s);
+int dm_sgemm(char transa, char transb, int m, int n, int k, float alpha,
+             const float* A, int lda, const float* B, int ldb, float beta,
+             float* C, int ldc, dm_stream_t s);
+
+/* BLAS trsv / gemv (vector right-hand sides), src/array/trsm.jl:17-51. */
+int dm_dtrsv(char uplo, char trans, char diag, int n, const double* A, int lda,
+             double* x, int incx, dm_stream_t s);
+int dm_dgemv(char trans, int m, int n, double alpha, const double* A, int lda,
+             const double* x, int incx, double beta, double* y, int incy,
+             dm_stream_t s);
+
+/* ------------------------------------------------------- datadeps executor
+ * A static task graph in the datadeps model (src/datadeps/queue.jl:44-54):
+ * tasks are added in submission order with In/Out/InOut access modes on
+ * their array arguments; overlapping accesses are ordered exactly as
+ * sequential execution would order them (In after earlier Out/InOut;
+ * Out/InOut after every earlier overlapping access), In/In never
+ * synchronize.  dm_dag_plan groups independent ready tasks of one kernel
+ * shape into batched launches, maps the groups onto HIP streams with event
+ * dependencies, and can capture the whole schedule into a hipGraph that
+ * dm_dag_run replays.  Replaces the Julia-side planner + Sch execution of a
+ * device datadeps region (src/datadeps/queue.jl:120-471,
+ * src/sch/Sch.jl:1788-2067) and the per-region tile copies
+ * (src/datadeps/aliasing.jl:920-1060): tiles stay resident. */
+
+enum { DM_OP_POTRF = 1, DM_OP_TRSM = 2, DM_OP_SYRK = 3, DM_OP_GEMM = 4,
+       DM_OP_TRSV = 5, DM_OP_GEMV = 6 };
+enum { DM_IN = 1, DM_OUT = 2, DM_INOUT = 3 };
+
+/* One tile task.  Operand slots by op:
+ *   POTRF: c = A (InOut), info = per-task device info int (Out)
+ *   TRSM : a = A, c = B          GEMM: a = A, b = B, c = C
+ *   SYRK : a = A, c = C          GEMV: a = A, b = x, c = y
+ *   TRSV : a = A, c = x
+ * m,n,k follow the BLAS call (TRSM: m,n of B; SYRK: n,k; POTRF: n in n).
+ * access[i] is DM_IN/DM_OUT/DM_INOUT for slots a,b,c (0 = unused). */
+typedef struct dm_task {
+    int op;
+    int dtype;
+    char side, uplo, transa, transb, diag, pad_[3];
+    int m, n, k;
+    double alpha, beta;
+    const void* a; int lda;
+    const void* b; int ldb;
+    void* c; int ldc;
+    int* info;
+    int access[3];
+} dm_task;
+
+typedef struct dm_plan_opts {
+    int num_streams;     /* streams to spread groups over (>=1) */
+    int batch;           /* 1: batch independent same-shape tasks per launch */
+    int lookahead;       /* slack (in levels) below which tasks go on the
+                            critical stream; 0 disables the split */
+    int use_graph;       /* 1: capture the schedule into a hipGraph */
+} dm_plan_opts;
+
+typedef struct dm_dag_stats {
+    int num_tasks, num_edges, num_groups, num_levels, num_launches;
+    int num_streams, graph;
+} dm_dag_stats;
+
+int dm_dag_create(dm_dag_t* dag);
+int dm_dag_destroy(dm_dag_t dag);
+int dm_dag_add_task(dm_dag_t dag, const dm_task* t);
+/* Dependency edges after planning, as (from, to) task-index pairs;
+ * *count is the capacity in pairs on input, the number of edges on output. */
+int dm_dag_edges(dm_dag_t dag, int* pairs, int* count);
+int dm_dag_plan(dm_dag_t dag, const dm_plan_opts* opts);
+/* Runs the planned DAG ordered after prior work on `s`; completes on `s`.
+ * abort_dev (may be NULL): device int; every launch becomes a no-op once it
+ * is non-zero (set by a failing POTRF), mirroring dependent tasks failing
+ * after PosDefException in the reference. */
+int dm_dag_run(dm_dag_t dag, dm_stream_t s);
+int dm_dag_get_stats(dm_dag_t dag, dm_dag_stats* st);
+
+/* ------------------------------------------------- whole-matrix forms
+ * A tiled matrix resident in HBM: Blocks(mb, nb) partition of an m x n
+ * matrix (src/array/alloc.jl:23-33), tile (i,j) is a dense column-major
+ * array with ld = its row count (Dagger's per-chunk Array), tiles packed in
+ * column-major tile order in one allocation starting at base:
+ *   offset(i,j) = j*nb*m + i*mb*cols(j)   (elements). */
+typedef struct dm_tiled {
+    void* base;
+    int64_t m, n;
+    int mb, nb;
+    int dtype;
+} dm_tiled;
+
+/* Tiled Cholesky, the exact task DAG of LinearAlgebra._chol!
+ * (src/array/cholesky.jl:19-108) for uplo 'U' or 'L', run through the
+ * datadeps executor with a cached plan/graph per (matrix, options).
+ * info_dev: device int receiving the reference's info: the LAPACK info of
+ * the first failing diagonal tile (tile-local, as potrf_checked! fills it),
+ * 0 on success.  global_info_dev (may be NULL): the global order of the
+ * failing leading minor. */
+int dm_cholesky_tiled(char uplo, const dm_tiled* A, int* info_dev,
+                      int* global_info_dev, const dm_plan_opts* opts,
+                      dm_stream_t s);
+
+/* ldiv!(C::Cholesky, B) for a matrix right-hand side (src/array/linalg.jl:
+ * 385-401): two tiled trsm! passes (src/array/trsm.jl:59-192) over the
+ * resident factor.  B is tiled with Blocks(nb, nb) rows matching A. */
+int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B,
+                   const dm_plan_opts* opts, dm_stream_t s);
+
+/* Tiled trsm!, one datadeps region (src/array/trsm.jl:59-192). */
+int dm_trsm_tiled(char side, char uplo, char trans, char diag, double alpha,
+                  const dm_tiled* A, const dm_tiled* B,
+                  const dm_plan_opts* opts, dm_stream_t s);
+
+/* Full column-major matrix (leading dimension ld) <-> tile-packed layout.
+ * pack also adds diag_add to the diagonal (0 for a plain copy).  Device
+ * side of copyto!(DA, A)/collect and of the re-blocking copy in
+ * maybe_copy_buffered (src/array/copy.jl:3-151). */
+int dm_tiled_pack(const dm_tiled* dst, const void* src, int64_t ld, double diag_add,
+                  dm_stream_t s);
+int dm_tiled_unpack(void* dst, int64_t ld, const dm_tiled* src, dm_stream_t s);
+
+/* Drop every cached plan/graph (call before freeing matrices whose plans
+ * were cached, or rely on dm_forget_matrix for one base pointer). */
+int dm_plan_cache_clear(void);
+int dm_forget_matrix(const void* base);
+
+/* ---------------------------------------------------------- generators
+ * Seeded counter-based normal variates: Philox4x32-10 keyed by `seed`,
+ * counter = element index, Box-Muller on the first two 32-bit outputs.
+ * Element i of the result depends only on (seed, offset + i). */
+int dm_fill_randn_d(double* x, size_t count, uint64_t seed, uint64_t offset,
+                    dm_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DAGGER_MI355X_H */

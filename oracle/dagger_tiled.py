@@ -1,0 +1,502 @@
+"""CPU oracle: a restatement of Dagger.jl's tiled Cholesky / ldiv! path.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module, and only as the
+checker (or the timed CPU baseline), never as the product path.
+
+What it restates (paths inside JuliaParallel/Dagger.jl):
+  * partition / _cumlength              src/array/alloc.jl:23-33
+  * each DArray chunk is its own dense column-major array
+                                        src/array/darray.jl:568-581
+  * _chol!(A, UpperTriangular)          src/array/cholesky.jl:19-63 (loop :38-53)
+  * _chol!(A, LowerTriangular)          src/array/cholesky.jl:64-108 (loop :83-98)
+  * potrf_checked! info rule            src/array/cholesky.jl:2-18
+  * trsm! (all side/uplo/trans branches) src/array/trsm.jl:59-192
+  * trsv! (vector right-hand side)      src/array/trsm.jl:1-57
+  * ldiv!(::Cholesky{T,<:DMatrix}, B)   src/array/linalg.jl:352-405
+  * datadeps dependency rules           src/datadeps/queue.jl:44-54,
+                                        src/datadeps/aliasing.jl:740-773
+The tile kernels are the same LAPACK/BLAS routines the reference calls on
+its CPU path (LAPACK.potrf!, BLAS.trsm!/syrk!/gemm!/gemv!/trsv!), here
+through scipy's bundled OpenBLAS.
+
+Execution: `run_sequential` runs the recorded tasks in submission order
+(the datadeps semantics: the result equals sequential execution);
+`run_threaded` runs the same task list on a thread pool with one
+single-threaded BLAS call per worker, honoring the datadeps dependencies
+-- the shape of Dagger's CPU datadeps execution (ThreadProc workers,
+src/threadproc.jl:13-41), used as the timed CPU baseline.
+"""
+import concurrent.futures as cf
+import os
+import threading
+
+import numpy as np
+import scipy.linalg.blas as blas
+import scipy.linalg.lapack as lapack
+
+IN, OUT, INOUT = 1, 2, 3
+
+
+class PosDefException(Exception):
+    def __init__(self, info):
+        super().__init__(f"PosDefException({info})")
+        self.info = info
+
+
+def cumlength(length, step):
+    """src/array/alloc.jl:23-28."""
+    nice, extra = divmod(length, step)
+    return [step] * nice + ([extra] if extra else [])
+
+
+class TiledMatrix:
+    """A Blocks(mb, nb) partitioned matrix whose chunks are separate
+    Fortran-ordered arrays (Dagger's DArray chunks)."""
+
+    def __init__(self, A, mb, nb=None):
+        A = np.asarray(A)
+        if A.ndim == 1:
+            A = A.reshape(-1, 1)
+            nb = 1
+        nb = mb if nb is None else nb
+        self.m, self.n = A.shape
+        self.mb, self.nb = mb, nb
+        self.rows = cumlength(self.m, mb)
+        self.cols = cumlength(self.n, nb)
+        self.chunks = np.empty((len(self.rows), len(self.cols)), dtype=object)
+        r0 = 0
+        for i, r in enumerate(self.rows):
+            c0 = 0
+            for j, c in enumerate(self.cols):
+                self.chunks[i, j] = np.array(A[r0:r0 + r, c0:c0 + c], order="F", copy=True)
+                c0 += c
+            r0 += r
+
+    @property
+    def mt(self):
+        return len(self.rows)
+
+    @property
+    def nt(self):
+        return len(self.cols)
+
+    def collect(self):
+        return np.block([[self.chunks[i, j] for j in range(self.nt)] for i in range(self.mt)])
+
+
+# ---------------------------------------------------------------- tile ops
+# Each op mutates its output tile in place, like BLAS.xxx!/LAPACK.potrf!.
+
+def _tr(t):
+    return 1 if t in ("T", "C") else 0
+
+
+def tile_potrf(uplo, A):
+    """LAPACK.potrf!(uplo, A) -> info (LAPACK convention)."""
+    f = lapack.dpotrf if A.dtype == np.float64 else lapack.spotrf
+    c, info = f(A, lower=1 if uplo == "L" else 0, clean=0, overwrite_a=0)
+    A[...] = c
+    return int(info)
+
+
+def tile_trsm(side, uplo, trans, diag, alpha, A, B):
+    f = blas.dtrsm if B.dtype == np.float64 else blas.strsm
+    B[...] = f(alpha, A, B, side=0 if side == "L" else 1, lower=1 if uplo == "L" else 0,
+               trans_a=_tr(trans), diag=1 if diag == "U" else 0)
+
+
+def tile_syrk(uplo, trans, alpha, A, beta, C):
+    f = blas.dsyrk if C.dtype == np.float64 else blas.ssyrk
+    C[...] = f(alpha, A, beta=beta, c=C, trans=_tr(trans), lower=1 if uplo == "L" else 0)
+
+
+def tile_gemm(ta, tb, alpha, A, B, beta, C):
+    f = blas.dgemm if C.dtype == np.float64 else blas.sgemm
+    C[...] = f(alpha, A, B, beta=beta, c=C, trans_a=_tr(ta), trans_b=_tr(tb))
+
+
+def tile_gemv(trans, alpha, A, x, beta, y):
+    f = blas.dgemv if y.dtype == np.float64 else blas.sgemv
+    y[...] = f(alpha, A, x.ravel(), beta=beta, y=y.ravel(), trans=_tr(trans)).reshape(y.shape)
+
+
+def tile_trsv(uplo, trans, diag, A, x):
+    f = blas.dtrsv if x.dtype == np.float64 else blas.strsv
+    x[...] = f(A, x.ravel(), lower=1 if uplo == "L" else 0, trans=_tr(trans),
+               diag=1 if diag == "U" else 0).reshape(x.shape)
+
+
+# ---------------------------------------------------------------- tasks
+class Task:
+    """One recorded tile task: fn(*args) with (array, mode) accesses."""
+
+    __slots__ = ("name", "fn", "args", "acc")
+
+    def __init__(self, name, fn, args, acc):
+        self.name, self.fn, self.args, self.acc = name, fn, args, acc
+
+    def run(self):
+        return self.fn(*self.args)
+
+
+class Region:
+    """A datadeps region: tasks recorded in submission order."""
+
+    def __init__(self):
+        self.tasks = []
+
+    def spawn(self, name, fn, args, acc):
+        self.tasks.append(Task(name, fn, args, acc))
+
+
+def datadeps_edges(tasks):
+    """Dependencies by the datadeps rules: In waits for the last writer of
+    the same array; Out/InOut waits for the last writer and every reader
+    since (src/datadeps/aliasing.jl:740-773).  Arrays are whole tiles here
+    (ContiguousAliasing of a chunk), identified by object identity."""
+    last_writer, readers = {}, {}
+    edges = set()
+    for i, t in enumerate(tasks):
+        deps = set()
+        for arr, mode in t.acc:
+            key = id(arr)
+            if key in last_writer:
+                deps.add(last_writer[key])
+            if mode & OUT:
+                deps.update(readers.get(key, ()))
+        deps.discard(i)
+        for d in deps:
+            edges.add((d, i))
+        for arr, mode in t.acc:
+            key = id(arr)
+            if mode & OUT:
+                last_writer[key] = i
+                readers[key] = []
+            else:
+                readers.setdefault(key, []).append(i)
+    return edges
+
+
+def run_sequential(region):
+    for t in region.tasks:
+        t.run()
+
+
+def run_threaded(region, workers=None):
+    """Execute the region on a thread pool honoring the datadeps edges, one
+    tile BLAS call per worker (BLAS itself single-threaded via threadpoolctl
+    when available)."""
+    tasks = region.tasks
+    n = len(tasks)
+    workers = workers or os.cpu_count() or 1
+    preds = [0] * n
+    succs = [[] for _ in range(n)]
+    for a, b in datadeps_edges(tasks):
+        preds[b] += 1
+        succs[a].append(b)
+    lock = threading.Lock()
+    done = threading.Event()
+    remaining = [n]
+    errors = []
+    ex = cf.ThreadPoolExecutor(max_workers=workers)
+
+    def submit(i):
+        ex.submit(work, i)
+
+    def work(i):
+        try:
+            tasks[i].run()
+        except Exception as e:  # surfaces after the region, like wait_all
+            errors.append(e)
+        ready = []
+        with lock:
+            for s in succs[i]:
+                preds[s] -= 1
+                if preds[s] == 0:
+                    ready.append(s)
+            remaining[0] -= 1
+            if remaining[0] == 0:
+                done.set()
+        for s in ready:
+            submit(s)
+
+    if n == 0:
+        return
+    for i in range(n):
+        if preds[i] == 0:
+            submit(i)
+    done.wait()
+    ex.shutdown(wait=True)
+    if errors:
+        raise errors[0]
+
+
+def blas_single_thread():
+    """Context limiting BLAS to one thread per call (threadpoolctl)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        return threadpool_limits(limits=1, user_api="blas")
+    except Exception:  # pragma: no cover
+        import contextlib
+        return contextlib.nullcontext()
+
+
+# ---------------------------------------------------------------- Cholesky
+def chol_region(T, uplo, info):
+    """Record the task loop of src/array/cholesky.jl:38-53 ('U') / :83-98 ('L').
+
+    info is the 1-element array filled by potrf_checked! on failure."""
+    Ac = T.chunks
+    mt, nt = Ac.shape
+    R = Region()
+
+    def potrf_checked(uplo_, A, info_arr):
+        # src/array/cholesky.jl:2-18: tile-local LAPACK info; throws
+        if info_arr[0] != 0:
+            return  # a dependent of the failed task does not run
+        i = tile_potrf(uplo_, A)
+        if i != 0:
+            info_arr[0] = i
+            raise PosDefException(i)
+
+    def guarded(fn):
+        def g(*a):
+            if info[0] != 0:
+                return  # tasks after a failed POTRF fail with it
+            fn(*a)
+        return g
+
+    trsm, syrk, gemm = guarded(tile_trsm), guarded(tile_syrk), guarded(tile_gemm)
+    if uplo == "U":
+        for k in range(mt):
+            R.spawn("potrf", potrf_checked, ("U", Ac[k, k], info), [(Ac[k, k], INOUT), (info, OUT)])
+            for n in range(k + 1, nt):
+                R.spawn("trsm", trsm, ("L", "U", "T", "N", 1.0, Ac[k, k], Ac[k, n]),
+                        [(Ac[k, k], IN), (Ac[k, n], INOUT)])
+            for m in range(k + 1, mt):
+                R.spawn("syrk", syrk, ("U", "T", -1.0, Ac[k, m], 1.0, Ac[m, m]),
+                        [(Ac[k, m], IN), (Ac[m, m], INOUT)])
+                for n in range(m + 1, nt):
+                    R.spawn("gemm", gemm, ("T", "N", -1.0, Ac[k, m], Ac[k, n], 1.0, Ac[m, n]),
+                            [(Ac[k, m], IN), (Ac[k, n], IN), (Ac[m, n], INOUT)])
+    else:
+        for k in range(mt):
+            R.spawn("potrf", potrf_checked, ("L", Ac[k, k], info), [(Ac[k, k], INOUT), (info, OUT)])
+            for m in range(k + 1, mt):
+                R.spawn("trsm", trsm, ("R", "L", "T", "N", 1.0, Ac[k, k], Ac[m, k]),
+                        [(Ac[k, k], IN), (Ac[m, k], INOUT)])
+            for n in range(k + 1, nt):
+                R.spawn("syrk", syrk, ("L", "N", -1.0, Ac[n, k], 1.0, Ac[n, n]),
+                        [(Ac[n, k], IN), (Ac[n, n], INOUT)])
+                for m in range(n + 1, mt):
+                    R.spawn("gemm", gemm, ("N", "T", -1.0, Ac[m, k], Ac[n, k], 1.0, Ac[m, n]),
+                            [(Ac[m, k], IN), (Ac[n, k], IN), (Ac[m, n], INOUT)])
+    return R
+
+
+def cholesky(A, nb, uplo="U", threaded=False, workers=None):
+    """_chol!(DArray(A, Blocks(nb, nb)), uplo): returns (factor matrix, info).
+
+    The factor matrix is the collected DArray: the uplo triangle holds the
+    factor, the other triangle keeps the input (as the reference leaves it)."""
+    T = TiledMatrix(np.array(A, copy=True), nb, nb)
+    info = np.zeros(1, dtype=np.int64)
+    R = chol_region(T, uplo, info)
+    try:
+        if threaded:
+            run_threaded(R, workers)
+        else:
+            for t in R.tasks:
+                try:
+                    t.run()
+                except PosDefException:
+                    pass
+    except PosDefException:
+        pass
+    return T.collect(), int(info[0])
+
+
+# ---------------------------------------------------------------- trsm!
+def trsm_region(side, uplo, trans, diag, alpha, TA, TB):
+    """Record the task loop of src/array/trsm.jl:88-190."""
+    Ac, Bc = TA.chunks, TB.chunks
+    Bmt, Bnt = Bc.shape
+    R = Region()
+    tr = "T" if trans in ("T", "C") else "N"
+
+    def trsm(al, a, b):
+        R.spawn("trsm", tile_trsm, (side, uplo, tr, diag, al, a, b), [(a, IN), (b, INOUT)])
+
+    def gemm(ta, tb, x, y, beta, c):
+        R.spawn("gemm", tile_gemm, (ta, tb, -1.0, x, y, beta, c), [(x, IN), (y, IN), (c, INOUT)])
+
+    if side == "L":
+        if uplo == "U":
+            if tr == "N":
+                for k in range(1, Bmt + 1):
+                    la = alpha if k == 1 else 1.0
+                    kk = Bmt - k
+                    for n in range(Bnt):
+                        trsm(la, Ac[kk, kk], Bc[kk, n])
+                    for m in range(k + 1, Bmt + 1):
+                        for n in range(Bnt):
+                            gemm("N", "N", Ac[Bmt - m, kk], Bc[kk, n], la, Bc[Bmt - m, n])
+            else:
+                for k in range(Bmt):
+                    la = alpha if k == 0 else 1.0
+                    for n in range(Bnt):
+                        trsm(la, Ac[k, k], Bc[k, n])
+                    for m in range(k + 1, Bmt):
+                        for n in range(Bnt):
+                            gemm(tr, "N", Ac[k, m], Bc[k, n], la, Bc[m, n])
+        else:
+            if tr == "N":
+                for k in range(Bmt):
+                    la = alpha if k == 0 else 1.0
+                    for n in range(Bnt):
+                        trsm(la, Ac[k, k], Bc[k, n])
+                    for m in range(k + 1, Bmt):
+                        for n in range(Bnt):
+                            gemm("N", "N", Ac[m, k], Bc[k, n], la, Bc[m, n])
+            else:
+                for k in range(1, Bmt + 1):
+                    la = alpha if k == 1 else 1.0
+                    kk = Bmt - k
+                    for n in range(Bnt):
+                        trsm(la, Ac[kk, kk], Bc[kk, n])
+                    for m in range(k + 1, Bmt + 1):
+                        for n in range(Bnt):
+                            gemm(tr, "N", Ac[kk, Bmt - m], Bc[kk, n], la, Bc[Bmt - m, n])
+    else:
+        if uplo == "U":
+            if tr == "N":
+                for k in range(Bnt):
+                    la = alpha if k == 0 else 1.0
+                    for m in range(Bmt):
+                        trsm(la, Ac[k, k], Bc[m, k])
+                    for m in range(Bmt):
+                        for n in range(k + 1, Bnt):
+                            gemm("N", "N", Bc[m, k], Ac[k, n], la, Bc[m, n])
+            else:
+                for k in range(1, Bnt + 1):
+                    kk = Bnt - k
+                    for m in range(Bmt):
+                        trsm(alpha, Ac[kk, kk], Bc[m, kk])
+                        for n in range(k + 1, Bnt + 1):
+                            gemm("N", tr, Bc[m, kk], Ac[Bnt - n, kk], 1.0, Bc[m, Bnt - n])
+        else:
+            if tr == "N":
+                for k in range(1, Bnt + 1):
+                    la = alpha if k == 1 else 1.0
+                    kk = Bnt - k
+                    for m in range(Bmt):
+                        trsm(la, Ac[kk, kk], Bc[m, kk])
+                        for n in range(k + 1, Bnt + 1):
+                            gemm("N", "N", Bc[m, kk], Ac[kk, Bnt - n], la, Bc[m, Bnt - n])
+            else:
+                for k in range(Bnt):
+                    for m in range(Bmt):
+                        trsm(alpha, Ac[k, k], Bc[m, k])
+                        for n in range(k + 1, Bnt):
+                            gemm("N", tr, Bc[m, k], Ac[n, k], 1.0, Bc[m, n])
+    return R
+
+
+def trsm(side, uplo, trans, diag, alpha, A, B, nb):
+    TA = TiledMatrix(np.array(A, copy=True), nb, nb)
+    TB = TiledMatrix(np.array(B, copy=True), nb, nb)
+    run_sequential(trsm_region(side, uplo, trans, diag, alpha, TA, TB))
+    return TB.collect()
+
+
+def trsv_region(uplo, trans, diag, TA, TB):
+    """src/array/trsm.jl:16-54 (alpha = 1, the ldiv! use)."""
+    Ac = TA.chunks
+    Bc = [TB.chunks[i, 0] for i in range(TB.mt)]
+    Bnt = len(Bc)
+    R = Region()
+
+    def trsv(u, t, a, x):
+        R.spawn("trsv", tile_trsv, (u, t, diag, a, x), [(a, IN), (x, INOUT)])
+
+    def gemv(t, a, x, y):
+        R.spawn("gemv", tile_gemv, (t, -1.0, a, x, 1.0, y), [(a, IN), (x, IN), (y, INOUT)])
+
+    if uplo == "U":
+        if trans == "N":
+            for k in reversed(range(Bnt)):
+                trsv("U", "N", Ac[k, k], Bc[k])
+                for i in range(k):
+                    gemv("N", Ac[i, k], Bc[k], Bc[i])
+        else:
+            for k in range(Bnt):
+                trsv("U", trans, Ac[k, k], Bc[k])
+                for i in range(k + 1, Bnt):
+                    gemv(trans, Ac[k, i], Bc[k], Bc[i])
+    else:
+        if trans == "N":
+            for k in range(Bnt):
+                trsv("L", "N", Ac[k, k], Bc[k])
+                for i in range(k + 1, Bnt):
+                    gemv("N", Ac[i, k], Bc[k], Bc[i])
+        else:
+            for k in reversed(range(Bnt)):
+                trsv("L", trans, Ac[k, k], Bc[k])
+                for i in range(k):
+                    gemv(trans, Ac[k, i], Bc[k], Bc[i])
+    return R
+
+
+def ldiv(F, uplo, B, nb, threaded=False, workers=None):
+    """ldiv!(Cholesky(F, uplo), B) (src/array/linalg.jl:352-405): the two
+    triangular-solve regions over the tiled factor; returns X."""
+    B = np.asarray(B)
+    TA = TiledMatrix(np.array(F, copy=True), nb, nb)
+    run = (lambda R: run_threaded(R, workers)) if threaded else run_sequential
+    if B.ndim == 1:
+        TB = TiledMatrix(np.array(B, copy=True), nb)
+        if uplo == "U":
+            run(trsv_region("U", "T", "N", TA, TB))
+            run(trsv_region("U", "N", "N", TA, TB))
+        else:
+            run(trsv_region("L", "N", "N", TA, TB))
+            run(trsv_region("L", "T", "N", TA, TB))
+        return TB.collect().ravel()
+    TB = TiledMatrix(np.array(B, copy=True), nb, nb)
+    if uplo == "U":
+        run(trsm_region("L", "U", "T", "N", 1.0, TA, TB))
+        run(trsm_region("L", "U", "N", "N", 1.0, TA, TB))
+    else:
+        run(trsm_region("L", "L", "N", "N", 1.0, TA, TB))
+        run(trsm_region("L", "L", "T", "N", 1.0, TA, TB))
+    return TB.collect()
+
+
+# ---------------------------------------------------------------- metrics
+def rel_factor_diff(F1, F2, uplo="U"):
+    """Frobenius relative difference of the uplo triangles."""
+    tri = np.triu if uplo == "U" else np.tril
+    a, b = tri(F1), tri(F2)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def reconstruction_error(F, A, uplo="U"):
+    """||A - U^T U||_F / ||A||_F (or L L^T)."""
+    if uplo == "U":
+        U = np.triu(F)
+        R = U.T @ U
+    else:
+        L = np.tril(F)
+        R = L @ L.T
+    return float(np.linalg.norm(A - R) / np.linalg.norm(A))
+
+
+def backward_error(A, X, B):
+    """Normwise ||B - A X||_inf / (||A||_inf ||X||_inf + ||B||_inf)."""
+    X2 = X.reshape(X.shape[0], -1)
+    B2 = B.reshape(B.shape[0], -1)
+    R = B2 - A @ X2
+    return float(np.abs(R).sum(axis=1).max() /
+                 (np.abs(A).sum(axis=1).max() * np.abs(X2).sum(axis=1).max() +
+                  np.abs(B2).sum(axis=1).max()))

@@ -1,0 +1,207 @@
+"""Pins the CPU oracle (oracle/dagger_tiled.py, oracle/gen.py) before it is
+trusted as the checker of the GPU path.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import dagger_tiled as O
+from oracle import gen
+
+GOLD = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                          "cholesky_golden.npz"))
+
+
+def dense_potrf(A, uplo):
+    F, info = sla.lapack.dpotrf(A, lower=1 if uplo == "L" else 0, clean=0)
+    return F, info
+
+
+# ------------------------------------------------------------ Cholesky
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("n,nb", [(1, 1), (7, 3), (64, 16), (100, 32), (128, 32), (130, 64),
+                                  (96, 96), (50, 64)])
+def test_cholesky_matches_dense_lapack(n, nb, uplo):
+    A = gen.spd(n, 100 + n)
+    F, info = O.cholesky(A, nb, uplo)
+    Fd, infod = dense_potrf(A, uplo)
+    assert info == 0 and infod == 0
+    assert O.rel_factor_diff(F, Fd, uplo) < 1e-13
+    # the other triangle keeps the input, as the reference leaves it
+    other = np.tril(np.ones((n, n)), -1) if uplo == "U" else np.triu(np.ones((n, n)), 1)
+    assert np.array_equal(F * other, A * other)
+    assert O.reconstruction_error(F, A, uplo) < 1e-14
+
+
+def _py_cholesky_upper(A):
+    """Textbook Cholesky-Banachiewicz (pure Python), U with A = U^T U."""
+    n = len(A)
+    U = [[0.0] * n for _ in range(n)]
+    for j in range(n):
+        s = A[j][j] - sum(U[k][j] ** 2 for k in range(j))
+        U[j][j] = math.sqrt(s)
+        for i in range(j + 1, n):
+            U[j][i] = (A[j][i] - sum(U[k][j] * U[k][i] for k in range(j))) / U[j][j]
+    return U
+
+
+@pytest.mark.parametrize("n,nb", [(5, 2), (6, 4), (9, 3)])
+def test_cholesky_brute_force_tiny(n, nb):
+    A = gen.spd(n, 7 * n)
+    U = np.array(_py_cholesky_upper(A.tolist()))
+    F, info = O.cholesky(A, nb, "U")
+    assert info == 0
+    assert np.allclose(np.triu(F), U, rtol=0, atol=1e-14)
+
+
+def test_cholesky_closed_forms():
+    n = 96
+    F, info = O.cholesky(np.eye(n), 32, "U")
+    assert info == 0 and np.array_equal(np.triu(F), np.eye(n))
+    d = np.linspace(1.0, 50.0, n)
+    F, info = O.cholesky(np.diag(d), 32, "L")
+    assert np.allclose(np.tril(F), np.diag(np.sqrt(d)), rtol=1e-15, atol=0)
+    # constructed factor: A = U^T U with U upper, positive diagonal
+    rng = np.random.default_rng(5)
+    U = np.triu(rng.standard_normal((n, n))) + np.diag(np.full(n, 4.0))
+    U[np.diag_indices(n)] = np.abs(U[np.diag_indices(n)])
+    A = U.T @ U
+    F, info = O.cholesky(A, 32, "U")
+    assert info == 0
+    assert np.allclose(np.triu(F), U, rtol=1e-12, atol=1e-12)
+
+
+def test_cholesky_non_pd_info_is_tile_local():
+    # reference construction: A[1,1] = -100 fails the first pivot
+    A = GOLD["npd_first/A"]
+    F, info = O.cholesky(A, 32, "U")
+    assert info == 1
+    # failure inside the third diagonal tile: tile-local LAPACK info
+    A = GOLD["npd_tile3/A"]
+    dense = int(GOLD["npd_tile3/dense_info"][0])
+    F, info = O.cholesky(A, 32, "U")
+    assert dense > 64 and dense <= 96
+    assert info == dense - 64  # potrf_checked! reports the tile's own info
+
+
+@pytest.mark.parametrize("key", ["n64_b16_U", "n64_b16_L", "n128_b32_U", "n128_b32_L",
+                                 "n100_b32_U", "n1_b1_U", "n7_b3_L"])
+def test_cholesky_and_ldiv_golden(key):
+    n, nb, lower, seed, nrhs = (int(v) for v in GOLD[f"{key}/meta"])
+    uplo = "L" if lower else "U"
+    A, Fg, B, Xg = (GOLD[f"{key}/{s}"] for s in ("A", "F", "B", "X"))
+    F, info = O.cholesky(A, nb, uplo)
+    assert info == 0
+    assert O.rel_factor_diff(F, Fg, uplo) < 1e-13
+    X = O.ldiv(F, uplo, B, nb)
+    assert np.allclose(X, Xg, rtol=1e-11, atol=1e-12)
+    if nrhs == 1:
+        x = O.ldiv(F, uplo, B[:, 0], nb)
+        assert np.allclose(x, Xg[:, 0], rtol=1e-11, atol=1e-12)
+
+
+def test_planted_mistake_is_caught(monkeypatch):
+    """A plausible oracle bug (gemm transpose flags swapped in the trailing
+    update) must fail the dense-LAPACK comparison used above."""
+    A = gen.spd(96, 3)
+    good, _ = O.cholesky(A, 32, "U")
+    Fd, _ = dense_potrf(A, "U")
+    assert O.rel_factor_diff(good, Fd) < 1e-13
+    real = O.tile_gemm
+
+    def wrong(ta, tb, alpha, X, Y, beta, Cm):
+        return real("N" if ta == "T" else "T", tb, alpha, X, Y, beta, Cm)
+
+    monkeypatch.setattr(O, "tile_gemm", wrong)
+    bad, _ = O.cholesky(A, 32, "U")
+    assert O.rel_factor_diff(bad, Fd) > 1e-3
+
+
+# ------------------------------------------------------------ trsm! / ldiv!
+def _tri(A, uplo, diag):
+    T = np.triu(A) if uplo == "U" else np.tril(A)
+    if diag == "U":
+        T = T.copy()
+        T[np.diag_indices(len(T))] = 1.0
+    return T
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trsm_all_branches(side, uplo, trans, diag):
+    n, nb = 96, 32
+    rng = np.random.default_rng(sum(map(ord, side + uplo + trans + diag)))
+    A = 0.05 * rng.standard_normal((n, n)) + np.diag(np.full(n, 2.0))
+    B = rng.standard_normal((n, 64) if side == "L" else (64, n))
+    alpha = 1.0 if side == "R" and trans == "T" else 0.5
+    X = O.trsm(side, uplo, trans, diag, alpha, A, B, nb)
+    T = _tri(A, uplo, diag)
+    opT = T.T if trans == "T" else T
+    if side == "L":
+        ref = np.linalg.solve(opT, alpha * B)
+    else:
+        ref = np.linalg.solve(opT.T, alpha * B.T).T
+    assert np.allclose(X, ref, rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("vector", [False, True])
+def test_ldiv_solves_system(uplo, vector):
+    n, nb = 160, 64
+    A = gen.spd(n, 77)
+    F, info = O.cholesky(A, nb, uplo)
+    B = gen.randn(n if vector else n * 3, 78).reshape((n,) if vector else (3, n)).T.copy()
+    X = O.ldiv(F, uplo, B, nb)
+    assert np.allclose(X, np.linalg.solve(A, B), rtol=1e-11, atol=1e-12)
+    assert O.backward_error(A, X, B) < 10 * n * np.finfo(float).eps
+
+
+def test_threaded_executor_equals_sequential():
+    A = gen.spd(256, 8)
+    F1, _ = O.cholesky(A, 64, "U")
+    with O.blas_single_thread():
+        F2, _ = O.cholesky(A, 64, "U", threaded=True, workers=4)
+    assert np.array_equal(F1, F2)
+
+
+# ------------------------------------------------------------ datadeps rules
+def test_datadeps_edge_rules():
+    a, b, c = np.zeros(1), np.zeros(1), np.zeros(1)
+    R = O.Region()
+    nop = lambda *x: None  # noqa: E731
+    R.spawn("r1", nop, (), [(a, O.IN)])          # 0
+    R.spawn("r2", nop, (), [(a, O.IN)])          # 1: R after R -> no edge
+    R.spawn("w1", nop, (), [(a, O.INOUT)])       # 2: W after R,R -> 0->2, 1->2
+    R.spawn("r3", nop, (), [(a, O.IN), (b, O.IN)])  # 3: R after W -> 2->3
+    R.spawn("w2", nop, (), [(b, O.OUT)])         # 4: W after R(b) -> 3->4
+    R.spawn("w3", nop, (), [(c, O.OUT)])         # 5: independent
+    R.spawn("w4", nop, (), [(c, O.OUT)])         # 6: W after W -> 5->6
+    assert O.datadeps_edges(R.tasks) == {(0, 2), (1, 2), (2, 3), (3, 4), (5, 6)}
+
+
+# ------------------------------------------------------------ generator
+def test_philox_known_answers():
+    """Random123 published known-answer vectors for philox4x32_10."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((0xffffffff,) * 4, (0xffffffff, 0xffffffff),
+         (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in kat:
+        got = gen.philox4x32_10_block(*[np.array([c]) for c in ctr], *key)
+        assert tuple(int(g[0]) for g in got) == want
+
+
+def test_randn_moments_and_determinism():
+    z = gen.randn(400000, 42)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+    assert np.array_equal(z[1000:2000], gen.randn(1000, 42, offset=1000))
+    assert not np.array_equal(z[:1000], gen.randn(1000, 43))
+    A = gen.spd(64, 1)
+    assert np.array_equal(A, A.T)
+    assert np.linalg.eigvalsh(A).min() > 0.9
