@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Benchmark: plain FP64 tiled Cholesky (cholesky! + ldiv!) on one MI355X.
+
+Workload (the single-GPU configuration of the headline metric): A = G G^T/N + I
+with G an N x N matrix of seeded Philox/Box-Muller N(0,1) variates, N = 32768,
+Blocks(1024, 1024) (32 x 32 tiles, 8 GiB resident in HBM), factorized in
+place by `cholesky!` (the task DAG of Dagger's _chol!, U variant) and solved
+by `ldiv!` with 64 right-hand sides.  Inputs are generated on the device
+before timing.  One step = restore A and B from pristine device copies
+(device-to-device), cholesky!, ldiv!.
+
+Reported value: Cholesky TFLOP/s = (N^3/3) / (hipEvent time of cholesky!),
+averaged over the timed steps.  Also reported: the dominant kernel's
+roofline (the batched FP64 MFMA GEMM/SYRK update, timed per launch group
+with HIP events on its own stream in a profiled eager run of the same
+schedule) and a CPU baseline: the oracle's restatement of Dagger's CPU
+datadeps Cholesky (same tile DAG on a thread pool, single-threaded OpenBLAS
+per tile) on a bounded sample of the same workload, timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dagger.jl_amd"))
+
+METRIC = "Cholesky TFLOP/s N=32k FP64 vs mixed+IR @1e-13 bwd err; served solves/s + p99"
+FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix (vendor figure)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+class Timer:
+    def __init__(self, call):
+        self.call = call
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        call("dm_event_create", C.byref(self.e0))
+        call("dm_event_create", C.byref(self.e1))
+
+    def start(self):
+        self.call("dm_event_record", self.e0, None)
+
+    def stop_ms(self):
+        self.call("dm_event_record", self.e1, None)
+        self.call("dm_event_sync", self.e1)
+        ms = C.c_float()
+        self.call("dm_event_elapsed_ms", C.byref(ms), self.e0, self.e1)
+        return ms.value
+
+
+def restore(call, dst, src):
+    call("dm_memcpy_d2d", C.c_void_p(dst.buffer.ptr), C.c_void_p(src.buffer.ptr),
+         C.c_size_t(src.m * src.n * 8), None)
+
+
+def gemm_roofline(dm, A0, DA, nb):
+    """Profiled eager run of the same Cholesky schedule: per-group HIP-event
+    times; the GEMM/SYRK update groups are the dominant kernel."""
+    call = dm._lib.call
+    restore(call, DA, A0)
+    call("dm_stream_sync", None)
+    info = dm.DeviceInt(1)
+    dag = dm.Dag.cholesky(DA, "U", info)
+    dag.plan(dm.PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=False,
+                            profile=True))
+    dag.run()
+    prof = dag.profile()
+    g = [p for p in prof if p["op"] == 4 and p["ms"] > 0]  # DM_OP_GEMM groups
+    flops = sum(p["flops"] for p in g)
+    ms = sum(p["ms"] for p in g)
+    launches = len(g)
+    achieved = flops / (ms * 1e-3) / 1e12
+    by_op = {}
+    for p in prof:
+        d = by_op.setdefault(p["op"], [0, 0.0, 0.0])
+        d[0] += 1
+        d[1] += p["ms"]
+        d[2] += p["flops"]
+    return {
+        "bound": "mfma",
+        "achieved": round(achieved, 2),
+        "peak": FP64_MFMA_PEAK,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_MFMA_PEAK, 4),
+        "traffic": None,
+        "kernel": "gemm_kernel (batched FP64 MFMA GEMM/SYRK trailing update)",
+        "launches": launches,
+        "avg_launch_ms": round(ms / max(launches, 1), 4),
+        "avg_launch_flops": flops / max(launches, 1),
+    }, {str(k): {"groups": v[0], "ms": round(v[1], 3), "flops": v[2]} for k, v in by_op.items()}
+
+
+def cpu_baseline(dm, n, nb, seed):
+    """Oracle restatement of Dagger's CPU datadeps Cholesky, threaded."""
+    from oracle import dagger_tiled as O
+    DA = dm.spd_generate(n, dm.Blocks(nb, nb), seed=seed)
+    A = DA.collect()
+    DA.unsafe_free()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    with O.blas_single_thread():
+        t0 = time.perf_counter()
+        F, info = O.cholesky(A, nb, "U", threaded=True, workers=cores)
+        dt = time.perf_counter() - t0
+    return {
+        "value": round(n ** 3 / 3 / dt / 1e12, 4),
+        "unit": "TFLOP/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"full tiled Cholesky (U) at N={n}, Blocks({nb},{nb}) on the host: the "
+                   f"oracle's datadeps tile DAG on {cores} threads, one single-threaded "
+                   f"OpenBLAS call per tile task; {dt:.2f} s"),
+        "seconds": round(dt, 3),
+        "info": info,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--nb", type=int, default=1024)
+    ap.add_argument("--nrhs", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-n", type=int, default=12288,
+                    help="N of the CPU-baseline sample (0 disables it)")
+    ap.add_argument("--no-roofline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import dagger_mi355x as dm
+    call = dm._lib.call
+    dm.init(0)
+    n, nb, nrhs = a.n, a.nb, a.nrhs
+    blocks = dm.Blocks(nb, nb)
+
+    t0 = time.perf_counter()
+    A0 = dm.spd_generate(n, blocks, seed=a.seed)
+    B0 = dm.randn(blocks, n, nrhs, seed=a.seed + 1)
+    DA = A0.copy()
+    DB = B0.copy()
+    call("dm_device_sync")
+    log(f"generated N={n} nb={nb} nrhs={nrhs} in {time.perf_counter() - t0:.2f}s")
+
+    timer = Timer(call)
+    # first call: plan + hipGraph capture + run (reported, not timed as a step)
+    t0 = time.perf_counter()
+    F = dm.cholesky_(DA)
+    dm.ldiv_(F, DB)
+    first_s = time.perf_counter() - t0
+    log(f"first cholesky!+ldiv! (plan+capture+run): {first_s:.3f}s info={F.info}")
+    if F.info != 0:
+        raise SystemExit(f"factorization failed: info={F.info}")
+
+    def step():
+        restore(call, DA, A0)
+        restore(call, DB, B0)
+        timer.start()
+        Fs = dm.cholesky_(DA)
+        c_ms = timer.stop_ms()
+        timer.start()
+        dm.ldiv_(Fs, DB)
+        s_ms = timer.stop_ms()
+        return c_ms, s_ms
+
+    for _ in range(a.warmup):
+        step()
+    call("dm_device_sync")
+    chol, solve = [], []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        c_ms, s_ms = step()
+        chol.append(c_ms)
+        solve.append(s_ms)
+        log(f"step {i}: cholesky {c_ms:.2f} ms ({n ** 3 / 3 / c_ms / 1e9:.2f} TFLOP/s), "
+            f"ldiv {s_ms:.2f} ms")
+    call("dm_device_sync")
+    wall = time.perf_counter() - t0
+    chol_ms = sum(chol) / len(chol)
+    solve_ms = sum(solve) / len(solve)
+    value = n ** 3 / 3 / (chol_ms * 1e-3) / 1e12
+
+    roof, by_op = (None, None)
+    if not a.no_roofline:
+        roof, by_op = gemm_roofline(dm, A0, DA, nb)
+        log(f"roofline: {roof}")
+    cpu = None
+    if a.cpu_n > 0:
+        cpu = cpu_baseline(dm, a.cpu_n, nb, a.seed)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"plain FP64 tiled Cholesky (cholesky!, U) + ldiv! with {nrhs} RHS, "
+                         f"N={n}, Blocks({nb},{nb}), A=G*G^T/N+I (Philox seed {a.seed})"),
+            "n": n, "nb": nb, "nrhs": nrhs,
+            "parallelism": "1 GPU (2 HIP streams, look-ahead 1, hipGraph replay)",
+        },
+        "cholesky_ms": round(chol_ms, 3),
+        "ldiv_ms": round(solve_ms, 3),
+        "ldiv_tflops": round(2.0 * n * n * nrhs / (solve_ms * 1e-3) / 1e12, 3),
+        "first_call_s": round(first_s, 3),
+        "roofline": roof,
+        "groups_by_op": by_op,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

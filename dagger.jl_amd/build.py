@@ -25,6 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CFLAGS = [
     "--offload-arch=gfx950",
     "-O3",
+    "-g",
     "-std=c++17",
     "-fPIC",
     "-Wall",

@@ -3,6 +3,7 @@
 
 #include <functional>
 #include <map>
+#include <memory>
 #include <vector>
 #include "dm_internal.h"
 
@@ -47,6 +48,7 @@ struct Group {
   int maxL = 0;
   std::vector<InvSrc> inv_src;
   // common
+  double flops = 0;  // algorithmic flops of the group's tasks
   int njobs = 0;
   long total_wg = 0;
   size_t table_off = 0;
@@ -69,12 +71,17 @@ struct Dag {
   void* dev_mem = nullptr;
   int* abort_dev = nullptr;  // 0, or the global order of the first failed pivot
   size_t inv_bytes = 0;
+  std::vector<size_t> task_ws_off;  // byte offset in dev_mem, or SIZE_MAX
   std::vector<hipStream_t> streams;
   hipStream_t origin = nullptr;
   std::vector<hipEvent_t> events, fork_ev, join_ev;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_nodes = 0;
+  bool capturing = false;
+  // profiling (opts.profile, eager runs): events around each group
+  std::vector<hipEvent_t> prof_start, prof_stop;
+  bool profiled = false;
   // Extra work appended to every run on the origin stream (captured too).
   std::function<int(Dag*, hipStream_t)> post_hook;
 
@@ -94,3 +101,7 @@ struct Dag {
 };
 
 }  // namespace dm
+
+struct dm_dag_s {
+  std::unique_ptr<dm::Dag> d;
+};

@@ -211,6 +211,7 @@ int Dag::plan(const dm_plan_opts& o) {
       g = it->second;
     }
     groups[g].tasks.push_back(i);
+    groups[g].flops += r.cost;
     r.group = g;
   }
   // issue order: by level, critical first (std::map order already is)
@@ -261,6 +262,12 @@ int Dag::create_streams() {
   for (auto& e : events) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   fork_ev.resize(1);
   DM_HIP(hipEventCreateWithFlags(&fork_ev[0], hipEventDisableTiming));
+  if (opts.profile) {
+    prof_start.resize(groups.size());
+    prof_stop.resize(groups.size());
+    for (auto& e : prof_start) DM_HIP(hipEventCreate(&e));
+    for (auto& e : prof_stop) DM_HIP(hipEventCreate(&e));
+  }
   join_ev.resize(opts.num_streams);
   for (auto& e : join_ev) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return 0;
@@ -382,8 +389,24 @@ int Dag::build_device_tables() {
       if (rc) return rc;
     }
   }
-  // device memory: [abort int (256 B)] [inverse blocks] [job tables]
-  const size_t tab_base = 256 + inv_bytes;
+  // per-task workspace for launches that solve through trsm_full
+  // (POTRF recursion, unbatched TRSM/TRSV): allocated here so nothing is
+  // allocated while a run is issued or captured.
+  task_ws_off.assign(tasks.size(), (size_t)-1);
+  size_t ws_bytes = 0;
+  for (Group& G : groups) {
+    const bool needs = G.op == DM_OP_POTRF || G.op == DM_OP_TRSV ||
+                       (G.op == DM_OP_TRSM && !G.batched_trsm);
+    if (!needs) continue;
+    const size_t per = G.dtype == DM_F64 ? trsm_ws_elems<double>() * sizeof(double)
+                                         : trsm_ws_elems<float>() * sizeof(float);
+    for (int ti : G.tasks) {
+      task_ws_off[ti] = 256 + inv_bytes + ws_bytes;
+      ws_bytes += per;
+    }
+  }
+  // device memory: [abort int (256 B)] [inverse blocks] [workspaces] [job tables]
+  const size_t tab_base = 256 + inv_bytes + ws_bytes;
   const size_t total = tab_base + blob.size();
   DM_HIP(hipMalloc(&dev_mem, total));
   abort_dev = (int*)dev_mem;
@@ -433,16 +456,17 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
   // Unbatched ops: one call per task in submission order.
   for (int ti : G.tasks) {
     const dm_task& t = tasks[ti].t;
+    T* ws = task_ws_off[ti] == (size_t)-1 ? nullptr : (T*)((char*)dev_mem + task_ws_off[ti]);
     int rc = 0;
     switch (t.op) {
       case DM_OP_POTRF: {
         const int gofs = t.k;  // global row offset of the tile (set by the builders; 0 otherwise)
-        rc = potrf_full<T>(is_upper(t.uplo), t.n, (T*)t.c, t.ldc, t.info, abort_dev, gofs, s);
+        rc = potrf_full<T>(is_upper(t.uplo), t.n, (T*)t.c, t.ldc, t.info, abort_dev, gofs, ws, s);
         break;
       }
       case DM_OP_TRSM:
         rc = trsm_full<T>(t.side, t.uplo, t.transa, t.diag, t.m, t.n, (T)t.alpha, (const T*)t.a,
-                          t.lda, (T*)t.c, t.ldc, ab, s);
+                          t.lda, (T*)t.c, t.ldc, ws, ab, s);
         break;
       case DM_OP_GEMM:
         rc = gemm_single<T>(is_trans(t.transa), is_trans(t.transb), t.m, t.n, t.k, (T)t.alpha,
@@ -464,10 +488,10 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       case DM_OP_TRSV:
         if (t.ldc == 1)
           rc = trsm_full<T>('L', t.uplo, t.transa, t.diag, t.n, 1, T(1), (const T*)t.a, t.lda,
-                            (T*)t.c, t.n, ab, s);
+                            (T*)t.c, t.n, ws, ab, s);
         else
           rc = trsm_full<T>('R', t.uplo, is_trans(t.transa) ? 'N' : 'T', t.diag, 1, t.n, T(1),
-                            (const T*)t.a, t.lda, (T*)t.c, t.ldc, ab, s);
+                            (const T*)t.a, t.lda, (T*)t.c, t.ldc, ws, ab, s);
         break;
     }
     if (rc) return rc;
@@ -485,8 +509,11 @@ int Dag::issue(hipStream_t o) {
     const Group& G = groups[g];
     hipStream_t s = streams[G.stream];
     for (int w : G.waits) DM_HIP(hipStreamWaitEvent(s, events[w], 0));
+    const bool prof = opts.profile && !capturing;
+    if (prof) DM_HIP(hipEventRecord(prof_start[g], s));
     rc = G.dtype == DM_F64 ? launch_group_t<double>(G, s) : launch_group_t<float>(G, s);
     if (rc) return rc;
+    if (prof) DM_HIP(hipEventRecord(prof_stop[g], s));
     DM_HIP(hipEventRecord(events[g], s));
   }
   for (size_t i = 0; i < streams.size(); ++i) {
@@ -497,12 +524,15 @@ int Dag::issue(hipStream_t o) {
     rc = post_hook(this, o);
     if (rc) return rc;
   }
+  profiled = opts.profile && !capturing;
   return 0;
 }
 
 int Dag::capture() {
   DM_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+  capturing = true;
   int rc = issue(origin);
+  capturing = false;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(origin, &g);
   if (rc) {
@@ -544,6 +574,8 @@ Dag::~Dag() {
   for (auto& e : events) (void)hipEventDestroy(e);
   for (auto& e : fork_ev) (void)hipEventDestroy(e);
   for (auto& e : join_ev) (void)hipEventDestroy(e);
+  for (auto& e : prof_start) (void)hipEventDestroy(e);
+  for (auto& e : prof_stop) (void)hipEventDestroy(e);
   if (!streams.empty() || origin) (void)hipDeviceSynchronize();
   for (auto& s : streams) (void)hipStreamDestroy(s);
   if (origin) (void)hipStreamDestroy(origin);
@@ -553,10 +585,6 @@ Dag::~Dag() {
 }  // namespace dm
 
 using namespace dm;
-
-struct dm_dag_s {
-  std::unique_ptr<Dag> d;
-};
 
 extern "C" {
 
@@ -597,7 +625,7 @@ int dm_dag_edges(dm_dag_t dag, int* pairs, int* count) {
 
 int dm_dag_plan(dm_dag_t dag, const dm_plan_opts* opts) {
   if (!dag) return arg_fail(1, "dag is NULL");
-  dm_plan_opts o{2, 1, 1, 1};
+  dm_plan_opts o{2, 1, 1, 1, 0};
   if (opts) o = *opts;
   int rc = ensure_init();
   if (rc) return rc;
@@ -609,6 +637,36 @@ int dm_dag_run(dm_dag_t dag, dm_stream_t s) {
   int rc = ensure_init();
   if (rc) return rc;
   return dag->d->run(stream_of(s));
+}
+
+int dm_dag_profile(dm_dag_t dag, dm_group_profile* out, int* count) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  if (!count) return arg_fail(3, "count is NULL");
+  Dag& d = *dag->d;
+  if (!d.profiled) return arg_fail(1, "no profiled run (plan with profile=1, no graph, then run)");
+  int rc = ensure_init();
+  if (rc) return rc;
+  const int cap = *count;
+  int n = 0;
+  for (int g : d.order) {
+    const Group& G = d.groups[g];
+    if (out && n < cap) {
+      float ms = 0.f;
+      DM_HIP(hipEventSynchronize(d.prof_stop[g]));
+      DM_HIP(hipEventElapsedTime(&ms, d.prof_start[g], d.prof_stop[g]));
+      dm_group_profile& p = out[n];
+      p.op = G.op;
+      p.ntasks = (int)G.tasks.size();
+      p.stream = G.stream;
+      p.level = G.level;
+      p.flops = G.flops;
+      p.ms = ms;
+      p.pad_ = 0;
+    }
+    ++n;
+  }
+  *count = n;
+  return 0;
 }
 
 int dm_dag_get_stats(dm_dag_t dag, dm_dag_stats* st) {

@@ -119,6 +119,12 @@ TrsmShape trsm_shape(char side, char uplo, char transa, char diag);
 constexpr int TRSM_BS = 32;  // diagonal block inverted up front
 template <typename T>
 constexpr int trsm_max_len() { return sizeof(T) == 8 ? 1024 : 2048; }
+// Workspace (elements) for any trsm_full / potrf_full call: the inverted
+// diagonal blocks of one LDS-resident solve (longer solves are split).
+template <typename T>
+constexpr size_t trsm_ws_elems() {
+  return (size_t)(trsm_max_len<T>() / TRSM_BS) * TRSM_BS * TRSM_BS;
+}
 
 // Inverts the 32x32 diagonal blocks of Tm into inv (ceil(L/32)*1024 elems).
 template <typename T>
@@ -135,15 +141,15 @@ inline int trsm_wgs(int NV) { return (NV + 15) / 16; }
 // Full BLAS trsm (any size; host recursion above trsm_max_len).
 template <typename T>
 int trsm_full(char side, char uplo, char transa, char diag, int m, int n,
-              T alpha, const T* A, int lda, T* B, int ldb, const int* abort,
-              hipStream_t s);
+              T alpha, const T* A, int lda, T* B, int ldb, T* ws,
+              const int* abort, hipStream_t s);
 
 // ------------------------------------------------------------ POTRF
 // info: LAPACK info of this call (local order); abort (may be null): set to
 // global_offset + local order on failure and checked by every launch.
 template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
-               int global_offset, hipStream_t s);
+               int global_offset, T* ws, hipStream_t s);
 
 // ------------------------------------------------------------ vectors
 int dgemv_launch(bool trans, int m, int n, double alpha, const double* A,

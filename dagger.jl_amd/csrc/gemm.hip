@@ -26,14 +26,19 @@
 namespace dm {
 
 template <typename T> struct Mfma;
+// Output layouts (lane l, accumulator item r; 16x16 block D[i][j], j = l%16):
+//   f64 16x16x4: i = 4*r + l/16        f32 16x16x4: i = 4*(l/16) + r
+// (the f64 layout was measured on gfx950 with a one-hot lane probe).
 template <> struct Mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
   static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
 };
 template <> struct Mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
   static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -243,7 +248,7 @@ struct GemmKernel {
       __syncthreads();
     }
 
-    // Epilogue: lane holds C[m][n] with m = .. + lr, n = .. + 4*lk + r.
+    // Epilogue: lane holds C[m][n] with m = .. + lr, n = .. + row(lk, r).
     const T alpha = job.alpha, beta = job.beta;
     const int tri = job.tri;
 #pragma unroll
@@ -253,7 +258,7 @@ struct GemmKernel {
       for (int j = 0; j < FN; ++j) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn0 + j * 16 + 4 * lk + r;
+          const int n = n0 + wn0 + j * 16 + Mfma<T>::row(lk, r);
           bool ok = m < M && n < N;
           if (tri == 1) ok = ok && m <= n;
           else if (tri == 2) ok = ok && m >= n;

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void potrf_leaf_kernel(T* __restrict__ A, int n
 
 template <typename T>
 static int potrf_rec(bool upper, int n, T* A, int lda, int* info, int info_base, int* abort,
-                     int global_base, const int* chk, hipStream_t s) {
+                     int global_base, const int* chk, T* ws, hipStream_t s) {
   if (n <= 0) return 0;
   if (n <= POTRF_LEAF) {
     if (upper)
@@ -96,36 +96,37 @@ static int potrf_rec(bool upper, int n, T* A, int lda, int* info, int info_base,
   const int n2 = n - n1;
   T* A11 = A;
   T* A22 = A + (size_t)n1 * lda + n1;
-  int rc = potrf_rec<T>(upper, n1, A11, lda, info, info_base, abort, global_base, chk, s);
+  int rc = potrf_rec<T>(upper, n1, A11, lda, info, info_base, abort, global_base, chk, ws, s);
   if (rc) return rc;
   if (upper) {
     T* A12 = A + (size_t)n1 * lda;
-    rc = trsm_full<T>('L', 'U', 'T', 'N', n1, n2, T(1), A11, lda, A12, lda, chk, s);
+    rc = trsm_full<T>('L', 'U', 'T', 'N', n1, n2, T(1), A11, lda, A12, lda, ws, chk, s);
     if (rc) return rc;
     rc = gemm_single<T>(true, false, n2, n2, n1, T(-1), A12, lda, A12, lda, T(1), A22, lda, 1, chk,
                         s);
   } else {
     T* A21 = A + n1;
-    rc = trsm_full<T>('R', 'L', 'T', 'N', n2, n1, T(1), A11, lda, A21, lda, chk, s);
+    rc = trsm_full<T>('R', 'L', 'T', 'N', n2, n1, T(1), A11, lda, A21, lda, ws, chk, s);
     if (rc) return rc;
     rc = gemm_single<T>(false, true, n2, n2, n1, T(-1), A21, lda, A21, lda, T(1), A22, lda, 2, chk,
                         s);
   }
   if (rc) return rc;
-  return potrf_rec<T>(upper, n2, A22, lda, info, info_base + n1, abort, global_base + n1, chk, s);
+  return potrf_rec<T>(upper, n2, A22, lda, info, info_base + n1, abort, global_base + n1, chk, ws,
+                      s);
 }
 
 template <typename T>
-int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset,
+int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset, T* ws,
                hipStream_t s) {
   // The trsm/syrk launches inside the recursion stop once a pivot failed:
   // they watch the plan-level abort flag when there is one, else info.
   const int* chk = abort ? abort : info;
-  return potrf_rec<T>(upper, n, A, lda, info, 0, abort, global_offset, chk, s);
+  return potrf_rec<T>(upper, n, A, lda, info, 0, abort, global_offset, chk, ws, s);
 }
 
-template int potrf_full<double>(bool, int, double*, int, int*, int*, int, hipStream_t);
-template int potrf_full<float>(bool, int, float*, int, int*, int*, int, hipStream_t);
+template int potrf_full<double>(bool, int, double*, int, int*, int*, int, double*, hipStream_t);
+template int potrf_full<float>(bool, int, float*, int, int*, int*, int, float*, hipStream_t);
 
 // ------------------------------------------------------------ gemv
 // y := alpha op(A) x + beta y.  'N': a workgroup owns 64 rows, its 4 waves

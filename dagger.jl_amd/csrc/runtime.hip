@@ -1,6 +1,10 @@
 // C ABI: runtime (errors, device, memory, streams, events, graphs) and the
 // BLAS/LAPACK-shaped tile entry points.
+#include <execinfo.h>
+#include <unistd.h>
+#include <csignal>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <unordered_map>
 #include "dm_internal.h"
@@ -15,6 +19,26 @@ static int g_device = -1;
 static hipStream_t g_default_stream = nullptr;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Opt-in host crash report: with DM_SEGV_BACKTRACE=1 in the environment a
+// SIGSEGV in host code prints the native stack (library offsets) to stderr.
+namespace {
+void segv_report(int sig) {
+  void* frames[64];
+  const int n = backtrace(frames, 64);
+  const char msg[] = "dagger_mi355x: host SIGSEGV, native backtrace:\n";
+  (void)!write(2, msg, sizeof msg - 1);
+  backtrace_symbols_fd(frames, n, 2);
+  signal(sig, SIG_DFL);
+  raise(sig);
+}
+struct SegvInstaller {
+  SegvInstaller() {
+    const char* e = getenv("DM_SEGV_BACKTRACE");
+    if (e && e[0] == '1') signal(SIGSEGV, segv_report);
+  }
+} g_segv_installer;
+}  // namespace
 
 int hip_fail(hipError_t e, const char* what) {
   char buf[512];
@@ -34,6 +58,9 @@ int ensure_init() {
   std::lock_guard<std::mutex> lk(g_init_mu);
   if (g_inited) {
     DM_HIP(hipSetDevice(g_device));
+    // Launch errors are read with hipGetLastError after each launch; drop
+    // anything left over from an earlier, unrelated call.
+    (void)hipGetLastError();
     return 0;
   }
   int n = 0;
@@ -329,7 +356,9 @@ static int potrf_api(char uplo, int n, T* A, int lda, int* info_dev, dm_stream_t
   int rc = fill_i32_launch(info_dev, 0, 1, st);
   if (rc) return rc;
   if (n == 0) return 0;
-  return potrf_full<T>(is_upper(uplo), n, A, lda, info_dev, nullptr, 0, st);
+  T* ws = (T*)stream_workspace(st, trsm_ws_elems<T>() * sizeof(T));
+  if (!ws) return arg_fail(0, "potrf: workspace allocation failed");
+  return potrf_full<T>(is_upper(uplo), n, A, lda, info_dev, nullptr, 0, ws, st);
 }
 
 extern "C" int dm_dpotrf(char uplo, int n, double* A, int lda, int* info_dev, dm_stream_t s) {
@@ -352,7 +381,9 @@ static int trsm_api(char side, char uplo, char transa, char diag, int m, int n, 
   if (lda < (ka > 1 ? ka : 1)) return arg_fail(9, "lda too small");
   if (ldb < (m > 1 ? m : 1)) return arg_fail(11, "ldb < max(1,m)");
   DM_INIT();
-  return trsm_full<T>(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, nullptr,
+  T* ws = (T*)stream_workspace(stream_of(s), trsm_ws_elems<T>() * sizeof(T));
+  if (!ws) return arg_fail(0, "trsm: workspace allocation failed");
+  return trsm_full<T>(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, ws, nullptr,
                       stream_of(s));
 }
 
@@ -428,13 +459,15 @@ extern "C" int dm_dtrsv(char uplo, char trans, char diag, int n, const double* A
   if (incx <= 0) return arg_fail(8, "incx must be positive");
   DM_INIT();
   if (n == 0) return 0;
+  double* ws = (double*)stream_workspace(stream_of(s), trsm_ws_elems<double>() * sizeof(double));
+  if (!ws) return arg_fail(0, "trsv: workspace allocation failed");
   if (incx == 1)
-    return trsm_full<double>('L', uplo, trans, diag, n, 1, 1.0, A, lda, x, n, nullptr,
+    return trsm_full<double>('L', uplo, trans, diag, n, 1, 1.0, A, lda, x, n, ws, nullptr,
                              stream_of(s));
   // x with stride incx is the single row of a 1 x n matrix with ld = incx:
   // solve x^T op'(A) = b^T with op' the opposite transpose.
   return trsm_full<double>('R', uplo, is_trans(trans) ? 'N' : 'T', diag, 1, n, 1.0, A, lda, x,
-                           incx, nullptr, stream_of(s));
+                           incx, ws, nullptr, stream_of(s));
 }
 
 extern "C" int dm_dgemv(char trans, int m, int n, double alpha, const double* A, int lda, const double* x,

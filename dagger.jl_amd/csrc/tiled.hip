@@ -265,13 +265,14 @@ std::string key_of(const char* kind, std::initializer_list<const void*> ptrs,
   os << kind;
   for (const void* p : ptrs) os << ':' << p;
   for (int64_t v : nums) os << ':' << v;
-  os << ':' << o.num_streams << ',' << o.batch << ',' << o.lookahead << ',' << o.use_graph;
+  os << ':' << o.num_streams << ',' << o.batch << ',' << o.lookahead << ',' << o.use_graph << ','
+     << o.profile;
   return os.str();
 }
 
 dm_plan_opts opts_or_default(const dm_plan_opts* o) {
   if (o) return *o;
-  return dm_plan_opts{2, 1, 1, 1};
+  return dm_plan_opts{2, 1, 1, 1, 0};
 }
 
 // Looks up or builds (build(d) fills tasks) and plans a DAG.
@@ -417,6 +418,42 @@ int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B, const dm_pla
                 &d);
   if (rc) return rc;
   return d->run(stream_of(s));
+}
+
+int dm_dag_build_cholesky(dm_dag_t* dag, char uplo, const dm_tiled* A, int* info_dev) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(2, "uplo must be 'U' or 'L'");
+  int rc = check_tiled(A, 3);
+  if (rc) return rc;
+  if (A->m != A->n || A->mb != A->nb) return arg_fail(3, "A must be square with square tiles");
+  std::unique_ptr<Dag> d(new Dag());
+  TileView tv(*A);
+  build_cholesky(*d, tv, is_upper(uplo) ? 'U' : 'L', info_dev);
+  *dag = new dm_dag_s{std::move(d)};
+  return 0;
+}
+
+int dm_dag_build_potrs(dm_dag_t* dag, char uplo, const dm_tiled* A, const dm_tiled* B) {
+  if (!dag) return arg_fail(1, "dag is NULL");
+  if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(2, "uplo must be 'U' or 'L'");
+  int rc = check_tiled(A, 3);
+  if (rc) return rc;
+  rc = check_tiled(B, 4);
+  if (rc) return rc;
+  if (A->m != A->n || A->mb != A->nb) return arg_fail(3, "A must be square with square tiles");
+  if (B->mb != A->mb || B->nb != A->nb || B->m != A->m || B->dtype != A->dtype)
+    return arg_fail(4, "B does not conform to A");
+  std::unique_ptr<Dag> d(new Dag());
+  TileView ta(*A), tb(*B);
+  if (is_upper(uplo)) {
+    build_trsm(*d, 'L', 'U', 'T', 'N', 1.0, ta, tb);
+    build_trsm(*d, 'L', 'U', 'N', 'N', 1.0, ta, tb);
+  } else {
+    build_trsm(*d, 'L', 'L', 'N', 'N', 1.0, ta, tb);
+    build_trsm(*d, 'L', 'L', 'T', 'N', 1.0, ta, tb);
+  }
+  *dag = new dm_dag_s{std::move(d)};
+  return 0;
 }
 
 int dm_plan_cache_clear(void) {

@@ -23,14 +23,19 @@
 namespace dm {
 
 template <typename T> struct MfmaT;
+// Output layouts (lane l, accumulator item r; 16x16 block D[i][j], j = l%16):
+//   f64 16x16x4: i = 4*r + l/16        f32 16x16x4: i = 4*(l/16) + r
+// (the f64 layout was measured on gfx950 with a one-hot lane probe).
 template <> struct MfmaT<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
   static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
 };
 template <> struct MfmaT<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
   static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        red[(wave * TRSM_BS + h * 16 + 4 * lk + r) * 16 + lr] = acc[h][0][r] + acc[h][1][r];
+        red[(wave * TRSM_BS + h * 16 + MfmaT<T>::row(lk, r)) * 16 + lr] = acc[h][0][r] + acc[h][1][r];
     __syncthreads();
     // ---- phase 2: bt = alpha*V_I - sum of partials
     for (int e = tid; e < TRSM_BS * 16; e += 256) {
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = wave * 16 + 4 * lk + r;
+        const int i = wave * 16 + MfmaT<T>::row(lk, r);
         const int gi = r0 + i, gj = j0 + lr;
         const T v = xa[r] + xb[r];
         X[gi * 16 + lr] = v;
@@ -303,12 +308,11 @@ static inline const T* tm_sub(const TrsmShape& sh, const T* A, int lda, int r, i
 // Effective solve of L-long vectors with host recursion over trsm_max_len.
 template <typename T>
 static int trsm_eff(const TrsmShape& sh, int L, int NV, T alpha, const T* A, int lda, T* B,
-                    int ldb, const int* abort, hipStream_t s) {
+                    int ldb, T* ws, const int* abort, hipStream_t s) {
   if (L <= 0 || NV <= 0) return 0;
   if (L <= trsm_max_len<T>()) {
-    const int nblk = (L + TRSM_BS - 1) / TRSM_BS;
-    T* inv = (T*)stream_workspace(s, sizeof(T) * (size_t)nblk * TRSM_BS * TRSM_BS);
-    if (!inv) return arg_fail(0, "trsm: workspace allocation failed");
+    if (!ws) return arg_fail(0, "trsm: no workspace");
+    T* inv = ws;
     int rc = trinv_blocks_launch<T>(sh, A, lda, L, inv, abort, s);
     if (rc) return rc;
     TrsmJob<T> j{};
@@ -326,7 +330,7 @@ static int trsm_eff(const TrsmShape& sh, int L, int NV, T alpha, const T* A, int
   const int first_r = sh.lower ? 0 : L2;
   const int second_r = sh.lower ? L1 : 0;
   int rc = trsm_eff<T>(sh, first_n, NV, alpha, tm_sub(sh, A, lda, first_r, first_r), lda,
-                       vsub(first_r), ldb, abort, s);
+                       vsub(first_r), ldb, ws, abort, s);
   if (rc) return rc;
   // V2 := alpha*V2 - Tm[second, first] X1
   const T* t21 = tm_sub(sh, A, lda, second_r, first_r);
@@ -341,12 +345,12 @@ static int trsm_eff(const TrsmShape& sh, int L, int NV, T alpha, const T* A, int
   }
   if (rc) return rc;
   return trsm_eff<T>(sh, second_n, NV, T(1), tm_sub(sh, A, lda, second_r, second_r), lda,
-                     vsub(second_r), ldb, abort, s);
+                     vsub(second_r), ldb, ws, abort, s);
 }
 
 template <typename T>
 int trsm_full(char side, char uplo, char transa, char diag, int m, int n, T alpha, const T* A,
-              int lda, T* B, int ldb, const int* abort, hipStream_t s) {
+              int lda, T* B, int ldb, T* ws, const int* abort, hipStream_t s) {
   const TrsmShape sh = trsm_shape(side, uplo, transa, diag);
   const int L = sh.side_l ? m : n;
   const int NV = sh.side_l ? n : m;
@@ -355,7 +359,7 @@ int trsm_full(char side, char uplo, char transa, char diag, int m, int n, T alph
     // B := 0 (BLAS: A not referenced)
     return gemm_single<T>(false, false, m, n, 0, T(0), B, ldb, B, ldb, T(0), B, ldb, 0, abort, s);
   }
-  return trsm_eff<T>(sh, L, NV, alpha, A, lda, B, ldb, abort, s);
+  return trsm_eff<T>(sh, L, NV, alpha, A, lda, B, ldb, ws, abort, s);
 }
 
 template int trinv_blocks_launch<double>(const TrsmShape&, const double*, int, int, double*,
@@ -367,8 +371,8 @@ template int trsm_launch<double>(const TrsmShape&, const TrsmJob<double>*, int,
 template int trsm_launch<float>(const TrsmShape&, const TrsmJob<float>*, int,
                                 const TrsmJob<float>&, long, const int*, hipStream_t);
 template int trsm_full<double>(char, char, char, char, int, int, double, const double*, int,
-                               double*, int, const int*, hipStream_t);
+                               double*, int, double*, const int*, hipStream_t);
 template int trsm_full<float>(char, char, char, char, int, int, float, const float*, int, float*,
-                              int, const int*, hipStream_t);
+                              int, float*, const int*, hipStream_t);
 
 }  // namespace dm

@@ -13,14 +13,14 @@ falls back to a CPU implementation.
 """
 from . import _lib
 from ._lib import DMError, load as load_library
-from .darray import Blocks, DArray, DeviceBuffer, DeviceInt, Tile, zeros, spd_generate
+from .darray import Blocks, DArray, DeviceBuffer, DeviceInt, Tile, zeros, randn, spd_generate
 from .datadeps import (In, Out, InOut, BLAS, LAPACK, potrf_checked, spawn, spawn_datadeps,
                        PlanOptions, SEQUENTIAL, Dag)
 from .linalg import (Cholesky, PosDefException, cholesky, cholesky_, ldiv_, trsm_, trsv_,
                      DEFAULT_OPTIONS)
 
 __all__ = [
-    "Blocks", "DArray", "DeviceBuffer", "DeviceInt", "Tile", "zeros", "spd_generate",
+    "Blocks", "DArray", "DeviceBuffer", "DeviceInt", "Tile", "zeros", "randn", "spd_generate",
     "In", "Out", "InOut", "BLAS", "LAPACK", "potrf_checked", "spawn", "spawn_datadeps",
     "PlanOptions", "SEQUENTIAL", "Dag", "Cholesky", "PosDefException", "cholesky",
     "cholesky_", "ldiv_", "trsm_", "trsv_", "DEFAULT_OPTIONS", "DMError", "load_library",

@@ -41,7 +41,13 @@ class dm_task(C.Structure):
 
 
 class dm_plan_opts(C.Structure):
-    _fields_ = [("num_streams", i32), ("batch", i32), ("lookahead", i32), ("use_graph", i32)]
+    _fields_ = [("num_streams", i32), ("batch", i32), ("lookahead", i32), ("use_graph", i32),
+                ("profile", i32)]
+
+
+class dm_group_profile(C.Structure):
+    _fields_ = [("op", i32), ("ntasks", i32), ("stream", i32), ("level", i32), ("flops", f64),
+                ("ms", f32), ("pad_", i32)]
 
 
 class dm_dag_stats(C.Structure):
@@ -100,6 +106,9 @@ _PROTOS = {
     "dm_dag_plan": (i32, [vp, C.POINTER(dm_plan_opts)]),
     "dm_dag_run": (i32, [vp, vp]),
     "dm_dag_get_stats": (i32, [vp, C.POINTER(dm_dag_stats)]),
+    "dm_dag_profile": (i32, [vp, C.POINTER(dm_group_profile), C.POINTER(i32)]),
+    "dm_dag_build_cholesky": (i32, [C.POINTER(vp), c_char, C.POINTER(dm_tiled), vp]),
+    "dm_dag_build_potrs": (i32, [C.POINTER(vp), c_char, C.POINTER(dm_tiled), C.POINTER(dm_tiled)]),
     "dm_cholesky_tiled": (i32, [c_char, C.POINTER(dm_tiled), vp, vp, C.POINTER(dm_plan_opts), vp]),
     "dm_potrs_tiled": (i32, [c_char, C.POINTER(dm_tiled), C.POINTER(dm_tiled),
                              C.POINTER(dm_plan_opts), vp]),

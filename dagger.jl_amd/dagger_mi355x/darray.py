@@ -277,6 +277,22 @@ def zeros(blocks, dtype, *shape):
     return D
 
 
+def randn(blocks, m, n, seed, offset=0, stream=None):
+    """`randn(Blocks(...), m, n)` generated in HBM: element (i, j) is the
+    Philox/Box-Muller variate with index offset + i + j*m (dm_fill_randn_d)."""
+    D = DArray(np.float64, (m, n) if n is not None else m, blocks)
+    count = D.m * D.n
+    if count:
+        F = DeviceBuffer(count * 8)
+        call("dm_fill_randn_d", C.c_void_p(F.ptr), C.c_size_t(count), C.c_uint64(seed),
+             C.c_uint64(offset), stream)
+        t = D.tiled()
+        call("dm_tiled_pack", C.byref(t), C.c_void_p(F.ptr), D.m, 0.0, stream)
+        call("dm_stream_sync", stream)
+        F.free()
+    return D
+
+
 def spd_generate(n, blocks, seed, dtype=np.float64, stream=None):
     """A = G G^T / n + I with G an n x n matrix of Philox-Box-Muller N(0,1)
     variates (dm_fill_randn_d), built entirely on the device: G in a full

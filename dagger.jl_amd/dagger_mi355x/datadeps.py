@@ -162,13 +162,14 @@ def make_task(f, args):
 class PlanOptions:
     """Executor knobs: streams, batching, look-ahead depth, hipGraph."""
 
-    def __init__(self, num_streams=2, batch=True, lookahead=1, use_graph=False):
+    def __init__(self, num_streams=2, batch=True, lookahead=1, use_graph=False, profile=False):
         self.num_streams, self.batch = num_streams, batch
-        self.lookahead, self.use_graph = lookahead, use_graph
+        self.lookahead, self.use_graph, self.profile = lookahead, use_graph, profile
 
     def c(self):
         return _lib.dm_plan_opts(int(self.num_streams), int(bool(self.batch)),
-                                 int(self.lookahead), int(bool(self.use_graph)))
+                                 int(self.lookahead), int(bool(self.use_graph)),
+                                 int(bool(self.profile)))
 
 
 SEQUENTIAL = PlanOptions(num_streams=1, batch=False, lookahead=0, use_graph=False)
@@ -177,14 +178,43 @@ SEQUENTIAL = PlanOptions(num_streams=1, batch=False, lookahead=0, use_graph=Fals
 class Dag:
     """A recorded task list planned by the native executor."""
 
-    def __init__(self, tasks):
+    def __init__(self, tasks=(), handle=None):
         self.tasks = list(tasks)
-        h = C.c_void_p()
-        call("dm_dag_create", C.byref(h))
-        self.h = h
+        if handle is None:
+            handle = C.c_void_p()
+            call("dm_dag_create", C.byref(handle))
+        self.h = handle
         for t in self.tasks:
             call("dm_dag_add_task", self.h, C.byref(t))
         self.planned = False
+
+    @classmethod
+    def cholesky(cls, A, uplo="U", info=None):
+        """The tiled Cholesky DAG of a resident DArray (dm_dag_build_cholesky)."""
+        h = C.c_void_p()
+        t = A.tiled()
+        call("dm_dag_build_cholesky", C.byref(h), uplo.encode(), C.byref(t),
+             C.c_void_p(info.ptr if info is not None else 0))
+        return cls(handle=h)
+
+    @classmethod
+    def potrs(cls, A, B, uplo="U"):
+        """The two trsm! regions of ldiv! (dm_dag_build_potrs)."""
+        h = C.c_void_p()
+        ta, tb = A.tiled(), B.tiled()
+        call("dm_dag_build_potrs", C.byref(h), uplo.encode(), C.byref(ta), C.byref(tb))
+        return cls(handle=h)
+
+    def profile(self):
+        """Per-group [op, ntasks, stream, level, flops, ms] of the last
+        profiled run."""
+        cnt = C.c_int(0)
+        call("dm_dag_profile", self.h, None, C.byref(cnt))
+        buf = (_lib.dm_group_profile * max(cnt.value, 1))()
+        cnt2 = C.c_int(cnt.value)
+        call("dm_dag_profile", self.h, buf, C.byref(cnt2))
+        return [dict(op=b.op, ntasks=b.ntasks, stream=b.stream, level=b.level, flops=b.flops,
+                     ms=b.ms) for b in buf[:cnt2.value]]
 
     def edges(self):
         cnt = C.c_int(0)

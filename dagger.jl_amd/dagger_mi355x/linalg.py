@@ -109,8 +109,12 @@ def _chol(A, uplo, via, options):
     """LinearAlgebra._chol!(A, UpperTriangular/LowerTriangular): returns info."""
     if A.ndim != 2 or A.m != A.n:
         raise ValueError("DimensionMismatch: matrix is not square")
-    info = DeviceInt(1)
-    ginfo = DeviceInt(1)
+    # One info pair per matrix: the cached plan/graph is keyed on these
+    # device pointers, so reusing them keeps repeated factorizations on the
+    # captured graph.
+    if getattr(A, "_chol_info", None) is None:
+        A._chol_info = (DeviceInt(1), DeviceInt(1))
+    info, ginfo = A._chol_info
 
     def run(Ab):
         if via == "native":

@@ -190,7 +190,24 @@ typedef struct dm_plan_opts {
     int lookahead;       /* slack (in levels) below which tasks go on the
                             critical stream; 0 disables the split */
     int use_graph;       /* 1: capture the schedule into a hipGraph */
+    int profile;         /* 1 (eager runs only): time every launch group
+                            with HIP events on its own stream */
 } dm_plan_opts;
+
+/* Per-launch-group timing of the last profiled run (dm_plan_opts.profile):
+ * the hipEvent time of the group's launches on its stream and the
+ * algorithmic flops of its tasks (GEMM 2mnk, SYRK n^2 k, TRSM m n ka,
+ * POTRF n^3/3).  The event-based counterpart of Dagger's timespan logging
+ * (src/utils/logging.jl:26-143) for device work. */
+typedef struct dm_group_profile {
+    int op;        /* DM_OP_* of the group's tasks (SYRK groups report GEMM) */
+    int ntasks;
+    int stream;
+    int level;
+    double flops;
+    float ms;
+    int pad_;
+} dm_group_profile;
 
 typedef struct dm_dag_stats {
     int num_tasks, num_edges, num_groups, num_levels, num_launches;
@@ -210,6 +227,8 @@ int dm_dag_plan(dm_dag_t dag, const dm_plan_opts* opts);
  * after PosDefException in the reference. */
 int dm_dag_run(dm_dag_t dag, dm_stream_t s);
 int dm_dag_get_stats(dm_dag_t dag, dm_dag_stats* st);
+/* Group timings of the last profiled run; *count: capacity in, groups out. */
+int dm_dag_profile(dm_dag_t dag, dm_group_profile* out, int* count);
 
 /* ------------------------------------------------- whole-matrix forms
  * A tiled matrix resident in HBM: Blocks(mb, nb) partition of an m x n
@@ -240,6 +259,13 @@ int dm_cholesky_tiled(char uplo, const dm_tiled* A, int* info_dev,
  * resident factor.  B is tiled with Blocks(nb, nb) rows matching A. */
 int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B,
                    const dm_plan_opts* opts, dm_stream_t s);
+
+/* Record (unplanned) the task DAG of the tiled Cholesky / ldiv! on resident
+ * tiles into a new dag handle -- the same tasks dm_cholesky_tiled and
+ * dm_potrs_tiled run -- for callers that plan, profile or replay it
+ * themselves. */
+int dm_dag_build_cholesky(dm_dag_t* dag, char uplo, const dm_tiled* A, int* info_dev);
+int dm_dag_build_potrs(dm_dag_t* dag, char uplo, const dm_tiled* A, const dm_tiled* B);
 
 /* Tiled trsm!, one datadeps region (src/array/trsm.jl:59-192). */
 int dm_trsm_tiled(char side, char uplo, char trans, char diag, double alpha,

@@ -223,9 +223,11 @@ def run_threaded(region, workers=None):
 
     if n == 0:
         return
-    for i in range(n):
-        if preds[i] == 0:
-            submit(i)
+    # snapshot the sources first: once submitted, completing tasks release
+    # their successors themselves (a live scan could submit a task twice)
+    sources = [i for i in range(n) if preds[i] == 0]
+    for i in sources:
+        submit(i)
     done.wait()
     ex.shutdown(wait=True)
     if errors:
