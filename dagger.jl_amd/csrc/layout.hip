@@ -64,3 +64,120 @@ extern "C" int dm_tiled_unpack(void* dst, int64_t ld, const dm_tiled* src, dm_st
   if (src->dtype == DM_F32) return tile_copy<float>(false, src, dst, ld, 0.0, stream_of(s));
   return arg_fail(3, "unknown dtype");
 }
+
+namespace dm {
+__global__ void add_diag_kernel(double* __restrict__ A, int64_t ld, int64_t n,
+                                const double* __restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[i * ld + i] += d[i];
+}
+}  // namespace dm
+
+// A[i,i] += d[i] on a full column-major FP64 matrix (generator helper).
+extern "C" int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_dev,
+                             dm_stream_t s) {
+  if (!A && n > 0) return arg_fail(1, "A is NULL");
+  if (ld < (n > 1 ? n : 1)) return arg_fail(2, "ld < max(1,n)");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     stream_of(s), A, ld, n, d_dev);
+  DM_LAUNCH_CHECK("add_diag_kernel");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ishermitian(::DArray) (src/array/linalg.jl:90-124): diagonal tiles must be
+// exactly symmetric (Julia's ishermitian on a dense block); an off-diagonal
+// pair (i > j) passes when ||A_ij - A_ji^T||_F <= sqrt(eps) *
+// max(||A_ij||_F, ||A_ji||_F) (is_cross_hermitian, src/array/linalg.jl:38-48).
+// One workgroup per tile pair; fixed-order reductions.
+namespace dm {
+template <typename T>
+__global__ __launch_bounds__(256) void herm_pair_kernel(const T* __restrict__ A, int64_t m,
+                                                        int mb, int mt,
+                                                        int* __restrict__ ok_pairs) {
+  __shared__ double s0[256], s1[256], s2[256];
+  // decode pair index -> (i, j) with i >= j, column by column of the lower triangle
+  int p = blockIdx.x, j = 0;
+  while (p >= mt - j) { p -= mt - j; ++j; }
+  const int i = j + p;
+  const int64_t ri = (i * (int64_t)mb + mb <= m) ? mb : m - i * (int64_t)mb;
+  const int64_t rj = (j * (int64_t)mb + mb <= m) ? mb : m - j * (int64_t)mb;
+  // tile (r, c) origin for a square m x m Blocks(mb, mb) matrix
+  auto tile = [&](int r, int c, int64_t cols) { return A + c * (int64_t)mb * m + r * (int64_t)mb * cols; };
+  const T* Aij = tile(i, j, rj);  // ri x rj, ld = ri
+  const T* Aji = tile(j, i, ri);  // rj x ri, ld = rj
+  double d2 = 0, n1 = 0, n2 = 0;
+  for (int64_t e = threadIdx.x; e < ri * rj; e += 256) {
+    const int64_t r = e % ri, c = e / ri;
+    const double a = (double)Aij[c * ri + r];
+    const double b = (double)Aji[r * rj + c];  // (A_ji^T)(r, c) = A_ji(c, r)
+    if (i == j) {
+      d2 += (a != b) ? 1.0 : 0.0;
+    } else {
+      d2 += (a - b) * (a - b);
+      n1 += a * a;
+      n2 += b * b;
+    }
+  }
+  s0[threadIdx.x] = d2; s1[threadIdx.x] = n1; s2[threadIdx.x] = n2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int ok;
+    if (i == j) ok = s0[0] == 0.0;
+    else {
+      const double rtol = sizeof(T) == 8 ? 1.4901161193847656e-08 : 3.4526698300124393e-04;
+      ok = sqrt(s0[0]) <= rtol * fmax(sqrt(s1[0]), sqrt(s2[0]));
+    }
+    ok_pairs[blockIdx.x] = ok;
+  }
+}
+
+__global__ void all_ok_kernel(const int* __restrict__ ok, int count, int* __restrict__ out) {
+  __shared__ int s[256];
+  int v = 1;
+  for (int i = threadIdx.x; i < count; i += 256) v &= ok[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] &= s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+}  // namespace dm
+
+// *result_dev = 1 if A (square, Blocks(nb, nb)) is Hermitian by the rule
+// above, else 0.  work_dev: mt*(mt+1)/2 ints.
+extern "C" int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* work_dev,
+                                    dm_stream_t s) {
+  if (!A) return arg_fail(1, "A is NULL");
+  if (A->m != A->n || A->mb != A->nb) return arg_fail(1, "A must be square with square tiles");
+  if (!result_dev || !work_dev) return arg_fail(2, "NULL result/work");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = stream_of(s);
+  const int mt = (int)((A->m + A->mb - 1) / A->mb);
+  const int pairs = mt * (mt + 1) / 2;
+  if (pairs == 0) return fill_i32_launch(result_dev, 1, 1, st);
+  if (A->dtype == DM_F64)
+    hipLaunchKernelGGL((herm_pair_kernel<double>), dim3(pairs), dim3(256), 0, st,
+                       (const double*)A->base, A->m, A->mb, mt, work_dev);
+  else
+    hipLaunchKernelGGL((herm_pair_kernel<float>), dim3(pairs), dim3(256), 0, st,
+                       (const float*)A->base, A->m, A->mb, mt, work_dev);
+  DM_LAUNCH_CHECK("herm_pair_kernel");
+  hipLaunchKernelGGL(all_ok_kernel, dim3(1), dim3(256), 0, st, work_dev, pairs, result_dev);
+  DM_LAUNCH_CHECK("all_ok_kernel");
+  return 0;
+}

@@ -1,0 +1,182 @@
+// Mixed precision: low-precision tiled Cholesky + FP64 iterative refinement.
+//
+// The "precision" form beside the plain FP64 drop-in: the factorization runs
+// the same tile DAG (src/array/cholesky.jl:38-53) on an FP32 copy of A
+// (TRSM/SYRK/GEMM on FP32 MFMA), while the residual r = b - A x, its norms and
+// the solution update stay in FP64 on the original A -- the structure of
+// LAPACK's dsposv, on resident tiles:
+//   x0 = A32^-1 b;  repeat  r = b - A x (FP64);  stop if
+//   max_j ||r_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol;
+//   d = A32^-1 r (FP32 solve);  x += d (FP64).
+// Kernels here: precision conversion of tile slabs, column infinity norms,
+// the infinity norm of A, and x += d.  Deterministic: one owner per output,
+// fixed reduction order.
+#include <cmath>
+#include <memory>
+#include <vector>
+#include "dag.h"
+
+namespace dm {
+
+__global__ void convert_d2s_kernel(float* __restrict__ dst, const double* __restrict__ src,
+                                   size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = (float)src[i];
+}
+
+__global__ void convert_s2d_kernel(double* __restrict__ dst, const float* __restrict__ src,
+                                   size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = (double)src[i];
+}
+
+// x += (double)d over a packed slab.
+__global__ void axpy_s2d_kernel(double* __restrict__ x, const float* __restrict__ d,
+                                size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[i] += (double)d[i];
+}
+
+// Row sums of |A| for a tile-packed m x n matrix: one thread per row, loop
+// over the tiles of that row (consecutive threads read consecutive rows of a
+// tile, so loads are coalesced).  rowsum[i] = sum_j |A_ij|.
+__global__ void row_abs_sum_kernel(const double* __restrict__ A, int64_t m, int64_t n, int mb,
+                                   int nb, double* __restrict__ rowsum) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t ti = i / mb, ii = i - ti * mb;
+  const int64_t rows = (ti * mb + mb <= m) ? mb : m - ti * mb;
+  double s = 0.0;
+  const int64_t nt = (n + nb - 1) / nb;
+  for (int64_t tj = 0; tj < nt; ++tj) {
+    const int64_t cols = (tj * nb + nb <= n) ? nb : n - tj * nb;
+    const double* t = A + tj * nb * m + ti * mb * cols;
+    for (int64_t jj = 0; jj < cols; ++jj) s += fabs(t[jj * rows + ii]);
+  }
+  rowsum[i] = s;
+}
+
+// Max over a vector (single workgroup, fixed order): out = max_i v[i].
+__global__ __launch_bounds__(256) void max_reduce_kernel(const double* __restrict__ v,
+                                                         int64_t count,
+                                                         double* __restrict__ out) {
+  __shared__ double part[256];
+  double mx = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += 256) mx = fmax(mx, v[i]);
+  part[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0];
+}
+
+// Column infinity norms of a tile-packed m x n matrix: one workgroup per
+// column, max |X_ij| over i.
+__global__ __launch_bounds__(256) void col_abs_max_kernel(const double* __restrict__ X,
+                                                          int64_t m, int64_t n, int mb, int nb,
+                                                          double* __restrict__ out) {
+  __shared__ double part[256];
+  const int64_t j = blockIdx.x;
+  const int64_t tj = j / nb, jj = j - tj * nb;
+  const int64_t cols = (tj * nb + nb <= n) ? nb : n - tj * nb;
+  double mx = 0.0;
+  for (int64_t i = threadIdx.x; i < m; i += 256) {
+    const int64_t ti = i / mb, ii = i - ti * mb;
+    const int64_t rows = (ti * mb + mb <= m) ? mb : m - ti * mb;
+    mx = fmax(mx, fabs(X[tj * nb * m + ti * mb * cols + jj * rows + ii]));
+  }
+  part[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = part[0];
+}
+
+static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
+
+int convert_slab(void* dst, int dst_dtype, const void* src, int src_dtype, size_t count,
+                 hipStream_t s) {
+  if (!count) return 0;
+  if (dst_dtype == DM_F32 && src_dtype == DM_F64)
+    hipLaunchKernelGGL(convert_d2s_kernel, dim3(nblocks(count)), dim3(256), 0, s, (float*)dst,
+                       (const double*)src, count);
+  else if (dst_dtype == DM_F64 && src_dtype == DM_F32)
+    hipLaunchKernelGGL(convert_s2d_kernel, dim3(nblocks(count)), dim3(256), 0, s, (double*)dst,
+                       (const float*)src, count);
+  else
+    return arg_fail(0, "convert: unsupported dtype pair");
+  DM_LAUNCH_CHECK("convert_kernel");
+  return 0;
+}
+
+// ------------------------------------------------------------ tiled GEMM DAG
+// C = alpha op(A) op(B) + beta C on tile grids (A: m x k tiles, B: k x n
+// tiles, all with the same block size), one GEMM task per (i, j, l) with beta
+// applied at l = 0 -- the tile loop of Dagger's matmul (src/array/mul.jl).
+void build_gemm_tiled(Dag& d, char ta, char tb, double alpha, const dm_tiled& A,
+                      const dm_tiled& B, double beta, const dm_tiled& Cm);
+
+}  // namespace dm
+
+using namespace dm;
+
+extern "C" {
+
+int dm_tiled_convert(const dm_tiled* dst, const dm_tiled* src, dm_stream_t s) {
+  if (!dst || !src) return arg_fail(1, "NULL tiled matrix");
+  if (dst->m != src->m || dst->n != src->n || dst->mb != src->mb || dst->nb != src->nb)
+    return arg_fail(2, "shapes/blocks differ");
+  int rc = ensure_init();
+  if (rc) return rc;
+  return convert_slab(dst->base, dst->dtype, src->base, src->dtype, (size_t)(dst->m * dst->n),
+                      stream_of(s));
+}
+
+int dm_tiled_norm_inf(const dm_tiled* A, double* out_dev, double* work_dev, dm_stream_t s) {
+  if (!A || A->dtype != DM_F64) return arg_fail(1, "A must be an FP64 tiled matrix");
+  if (!out_dev || !work_dev) return arg_fail(2, "NULL output/work");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = stream_of(s);
+  if (A->m <= 0 || A->n <= 0) return fill_i32_launch((int*)out_dev, 0, 2, st);
+  hipLaunchKernelGGL(row_abs_sum_kernel, dim3(nblocks((size_t)A->m)), dim3(256), 0, st,
+                     (const double*)A->base, A->m, A->n, A->mb, A->nb, work_dev);
+  DM_LAUNCH_CHECK("row_abs_sum_kernel");
+  hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, st, (const double*)work_dev, A->m,
+                     out_dev);
+  DM_LAUNCH_CHECK("max_reduce_kernel");
+  return 0;
+}
+
+int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s) {
+  if (!X || X->dtype != DM_F64) return arg_fail(1, "X must be an FP64 tiled matrix");
+  if (!out_dev) return arg_fail(2, "NULL output");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (X->n <= 0) return 0;
+  hipLaunchKernelGGL(col_abs_max_kernel, dim3((unsigned)X->n), dim3(256), 0, stream_of(s),
+                     (const double*)X->base, X->m, X->n, X->mb, X->nb, out_dev);
+  DM_LAUNCH_CHECK("col_abs_max_kernel");
+  return 0;
+}
+
+int dm_tiled_axpy_s2d(const dm_tiled* X, const dm_tiled* D, dm_stream_t s) {
+  if (!X || !D || X->dtype != DM_F64 || D->dtype != DM_F32)
+    return arg_fail(1, "X must be FP64 and D FP32");
+  if (X->m != D->m || X->n != D->n || X->mb != D->mb || X->nb != D->nb)
+    return arg_fail(2, "shapes/blocks differ");
+  int rc = ensure_init();
+  if (rc) return rc;
+  const size_t count = (size_t)(X->m * X->n);
+  if (!count) return 0;
+  hipLaunchKernelGGL(axpy_s2d_kernel, dim3(nblocks(count)), dim3(256), 0, stream_of(s),
+                     (double*)X->base, (const float*)D->base, count);
+  DM_LAUNCH_CHECK("axpy_s2d_kernel");
+  return 0;
+}
+
+}  // extern "C"

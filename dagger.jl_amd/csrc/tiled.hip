@@ -251,6 +251,30 @@ void build_trsm(Dag& d, char side, char uplo, char trans, char diag, double alph
   }
 }
 
+}  // namespace
+
+// C = alpha op(A) op(B) + beta C on tile grids with one block size: one GEMM
+// task per (i, j, l), beta applied at l = 0 (the tile loop of Dagger's
+// matmul, src/array/mul.jl).
+void build_gemm_tiled(Dag& d, char ta, char tb, double alpha, const dm_tiled& A,
+                      const dm_tiled& B, double beta, const dm_tiled& Cm) {
+  TileView va(A), vb(B), vc(Cm);
+  const bool tra = is_trans(ta), trb = is_trans(tb);
+  const int K = tra ? va.mt : va.nt;
+  for (int j = 0; j < vc.nt; ++j)
+    for (int i = 0; i < vc.mt; ++i)
+      for (int l = 0; l < K; ++l) {
+        const int ai = tra ? l : i, aj = tra ? i : l;
+        const int bi = trb ? j : l, bj = trb ? l : j;
+        const int kk = tra ? va.rows(l) : va.cols(l);
+        add_gemm(d, vc.dtype, tra ? 'T' : 'N', trb ? 'T' : 'N', vc.rows(i), vc.cols(j), kk, alpha,
+                 va.ptr(ai, aj), va.ld(ai), vb.ptr(bi, bj), vb.ld(bi), l == 0 ? beta : 1.0,
+                 vc.ptr(i, j), vc.ld(i));
+      }
+}
+
+namespace {
+
 // ------------------------------------------------------------ plan cache
 struct CacheEntry {
   std::unique_ptr<Dag> dag;
@@ -413,6 +437,47 @@ int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B, const dm_pla
                     build_trsm(dag, 'L', 'L', 'N', 'N', 1.0, ta, tb);
                     build_trsm(dag, 'L', 'L', 'T', 'N', 1.0, ta, tb);
                   }
+                  return 0;
+                },
+                &d);
+  if (rc) return rc;
+  return d->run(stream_of(s));
+}
+
+int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A, const dm_tiled* B,
+                  double beta, const dm_tiled* Cm, const dm_plan_opts* opts, dm_stream_t s) {
+  if (!is_trans(transa) && !is_notrans(transa)) return arg_fail(1, "transa must be 'N','T'");
+  if (!is_trans(transb) && !is_notrans(transb)) return arg_fail(2, "transb must be 'N','T'");
+  int rc = check_tiled(A, 4);
+  if (rc) return rc;
+  rc = check_tiled(B, 5);
+  if (rc) return rc;
+  rc = check_tiled(Cm, 7);
+  if (rc) return rc;
+  if (A->dtype != B->dtype || A->dtype != Cm->dtype) return arg_fail(7, "dtypes differ");
+  const int64_t am = is_trans(transa) ? A->n : A->m, ak = is_trans(transa) ? A->m : A->n;
+  const int64_t bk = is_trans(transb) ? B->n : B->m, bn = is_trans(transb) ? B->m : B->n;
+  if (am != Cm->m || bn != Cm->n || ak != bk) return arg_fail(7, "shapes do not conform");
+  if (A->mb != A->nb || B->mb != A->mb || B->nb != A->mb || Cm->mb != A->mb ||
+      (Cm->n > 0 && Cm->nb != A->mb && Cm->n > Cm->nb))
+    return arg_fail(7, "all operands must use one square block size");
+  rc = ensure_init();
+  if (rc) return rc;
+  if (Cm->m == 0 || Cm->n == 0) return 0;
+  const dm_plan_opts o = opts_or_default(opts);
+  const char ta = is_trans(transa) ? 'T' : 'N', tb = is_trans(transb) ? 'T' : 'N';
+  int64_t ab, bb;
+  std::memcpy(&ab, &alpha, sizeof ab);
+  std::memcpy(&bb, &beta, sizeof bb);
+  char tag[16];
+  snprintf(tag, sizeof tag, "gemm%c%c", ta, tb);
+  const std::string key = key_of(tag, {A->base, B->base, Cm->base},
+                                 {A->m, A->n, B->m, B->n, Cm->m, Cm->n, A->mb, A->dtype, ab, bb},
+                                 o);
+  Dag* d = nullptr;
+  rc = get_plan(key, {A->base, B->base, Cm->base}, o,
+                [&](Dag& dag) {
+                  build_gemm_tiled(dag, ta, tb, alpha, *A, *B, beta, *Cm);
                   return 0;
                 },
                 &d);

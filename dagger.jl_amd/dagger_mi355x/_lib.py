@@ -114,6 +114,14 @@ _PROTOS = {
                              C.POINTER(dm_plan_opts), vp]),
     "dm_trsm_tiled": (i32, [c_char, c_char, c_char, c_char, f64, C.POINTER(dm_tiled),
                             C.POINTER(dm_tiled), C.POINTER(dm_plan_opts), vp]),
+    "dm_ishermitian_tiled": (i32, [C.POINTER(dm_tiled), vp, vp, vp]),
+    "dm_gemm_tiled": (i32, [c_char, c_char, f64, C.POINTER(dm_tiled), C.POINTER(dm_tiled), f64,
+                            C.POINTER(dm_tiled), C.POINTER(dm_plan_opts), vp]),
+    "dm_tiled_convert": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp]),
+    "dm_tiled_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp, vp]),
+    "dm_tiled_col_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp]),
+    "dm_tiled_axpy_s2d": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp]),
+    "dm_add_diag_d": (i32, [vp, i64, i64, vp, vp]),
     "dm_tiled_pack": (i32, [C.POINTER(dm_tiled), vp, i64, f64, vp]),
     "dm_tiled_unpack": (i32, [vp, i64, C.POINTER(dm_tiled), vp]),
     "dm_plan_cache_clear": (i32, []),

@@ -134,16 +134,42 @@ def _chol(A, uplo, via, options):
     return loc, g
 
 
-def cholesky_(A, uplo="U", check=True, via="native", options=None):
+def ishermitian(A):
+    """`ishermitian(::DArray)` (src/array/linalg.jl:90-124) on the device:
+    exact symmetry of diagonal tiles, sqrt(eps)-relative Frobenius test of
+    off-diagonal tile pairs; non-square blocks are re-blocked first."""
+    if A.ndim != 2 or A.m != A.n:
+        return False
+    sq = _square_blocks(A)
+    B = A if A.partitioning == sq else A.reblock(sq)
+    mt = B.mt
+    work = DeviceInt(max(mt * (mt + 1) // 2, 1))
+    res = DeviceInt(1)
+    t = B.tiled()
+    call("dm_ishermitian_tiled", C.byref(t), C.c_void_p(res.ptr), C.c_void_p(work.ptr), None)
+    ok = bool(res.get()[0])
+    if B is not A:
+        B.unsafe_free()
+    return ok
+
+
+def cholesky_(A, uplo="U", check=True, via="native", options=None, check_hermitian=True):
     """`cholesky!(A)` / `cholesky!(Hermitian(A, :L))` on a DArray, in place.
 
-    info is the reference's: the LAPACK info of the first diagonal tile
-    whose POTRF fails (tile-local, src/array/cholesky.jl:13-16,62); the
-    global leading-minor order is kept in `Cholesky.global_info`."""
+    As the stdlib cholesky!(::AbstractMatrix) does, a non-Hermitian input is
+    rejected with info = -1 before factoring (check_hermitian=False skips the
+    pass, as cholesky!(Hermitian(A)) would).  Otherwise info is the
+    reference's: the LAPACK info of the first diagonal tile whose POTRF fails
+    (tile-local, src/array/cholesky.jl:13-16,62); the global leading-minor
+    order is kept in `Cholesky.global_info`."""
     uplo = uplo.upper()
     if uplo not in ("U", "L"):
         raise ValueError("uplo must be 'U' or 'L'")
     options = options or DEFAULT_OPTIONS
+    if check_hermitian and not ishermitian(A):
+        if check:
+            raise PosDefException(-1)
+        return Cholesky(A, uplo, -1, -1)
     info, ginfo = _chol(A, uplo, via, options)
     if check and info != 0:
         raise PosDefException(info)

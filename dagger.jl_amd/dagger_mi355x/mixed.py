@@ -1,0 +1,304 @@
+"""Precision form: low-precision tiled Cholesky + FP64 iterative refinement.
+
+Beside the plain FP64 drop-in.  The factorization runs the same tile DAG
+(src/array/cholesky.jl:38-53) on an FP32 copy of A, so TRSM/SYRK/GEMM run on
+FP32 MFMA; the residual r = b - A x, its norms and the update x += d stay in
+FP64 on the untouched FP64 A (LAPACK dsposv's structure, on resident tiles):
+
+    x0 = F32^-1 b
+    repeat: r = b - A x                       (FP64 tiled GEMM, HBM-bound)
+            berr = max_j ||r_j|| / (||A|| ||x_j|| + ||b_j||)   (inf-norms)
+            stop if berr <= tol
+            d = F32^-1 r ; x += d              (FP32 solve, FP64 update)
+
+If refinement stalls (kappa(A) * u32 near 1), `solve` switches to GMRES-IR:
+FP64 restarted GMRES on A with the FP32 factor as a left preconditioner
+(Carson & Higham, "Accelerating the solution of linear systems by iterative
+refinement in three precisions", SISC 2018), one right-hand side at a time.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+from .darray import DArray, Blocks, DeviceBuffer, DeviceInt
+from .datadeps import PlanOptions
+from .linalg import PosDefException
+
+_OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True)
+
+
+def _convert(dst, src):
+    td, ts = dst.tiled(), src.tiled()
+    call("dm_tiled_convert", C.byref(td), C.byref(ts), None)
+
+
+def _colnorms(X, buf):
+    t = X.tiled()
+    call("dm_tiled_col_norm_inf", C.byref(t), C.c_void_p(buf.ptr), None)
+    out = np.empty(X.n)
+    call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
+         C.c_size_t(8 * X.n), None)
+    return out
+
+
+class MixedCholesky:
+    """cholesky(A) in FP32 tiles, kept with the FP64 A for refinement.
+
+    A64 must stay resident and unchanged while the factor is used."""
+
+    def __init__(self, A64, options=None):
+        if A64.dtype != np.float64:
+            raise TypeError("A must be an FP64 DArray")
+        self.A = A64
+        self.options = options or _OPTS
+        bs = min(A64.partitioning.blocksize)
+        if A64.partitioning != Blocks(bs, bs):
+            raise ValueError("mixed Cholesky needs square Blocks(nb, nb)")
+        self.nb = bs
+        self.F = DArray(np.float32, A64.shape, A64.partitioning)
+        _convert(self.F, self.A)
+        self.info = DeviceInt(1)
+        self.ginfo = DeviceInt(1)
+        t = self.F.tiled()
+        o = self.options.c()
+        call("dm_cholesky_tiled", b"U", C.byref(t), C.c_void_p(self.info.ptr),
+             C.c_void_p(self.ginfo.ptr), C.byref(o), None)
+        call("dm_stream_sync", None)
+        info = int(self.info.get()[0])
+        if info != 0:
+            raise PosDefException(info)
+        # ||A||_inf once
+        self._work = DeviceBuffer(8 * max(A64.m, 2))
+        self._norm = DeviceBuffer(16)
+        ta = self.A.tiled()
+        call("dm_tiled_norm_inf", C.byref(ta), C.c_void_p(self._norm.ptr),
+             C.c_void_p(self._work.ptr), None)
+        out = np.zeros(1)
+        call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self._norm.ptr),
+             C.c_size_t(8), None)
+        self.normA = float(out[0])
+        self._bufs = {}
+
+    # ---------------------------------------------------------------- pieces
+    def _buffers(self, nrhs):
+        b = self._bufs.get(nrhs)
+        if b is None:
+            blk = Blocks(self.nb, self.nb)
+            b = dict(X=DArray(np.float64, (self.A.m, nrhs), blk),
+                     R=DArray(np.float64, (self.A.m, nrhs), blk),
+                     D=DArray(np.float32, (self.A.m, nrhs), blk),
+                     nrm=DeviceBuffer(8 * max(nrhs, 1)))
+            self._bufs[nrhs] = b
+        return b
+
+    def _solve32(self, D):
+        """D := F32^-1 D (two FP32 tiled triangular solves)."""
+        tf, td = self.F.tiled(), D.tiled()
+        o = self.options.c()
+        call("dm_potrs_tiled", b"U", C.byref(tf), C.byref(td), C.byref(o), None)
+
+    def residual(self, B, X, R):
+        """R = B - A X in FP64."""
+        call("dm_memcpy_d2d", C.c_void_p(R.buffer.ptr), C.c_void_p(B.buffer.ptr),
+             C.c_size_t(B.m * B.n * 8), None)
+        ta, tx, tr = self.A.tiled(), X.tiled(), R.tiled()
+        o = self.options.c()
+        call("dm_gemm_tiled", b"N", b"N", -1.0, C.byref(ta), C.byref(tx), 1.0, C.byref(tr),
+             C.byref(o), None)
+
+    # ---------------------------------------------------------------- solve
+    def solve(self, B, tol=1e-13, max_iter=30, X=None):
+        """Solve A X = B (B an FP64 DArray N x nrhs, Blocks(nb, nb)).
+
+        Returns (X, info) with info = dict(iterations, berr (per column),
+        method)."""
+        if B.dtype != np.float64 or B.m != self.A.m:
+            raise ValueError("B must be an FP64 DArray with A's rows")
+        nrhs = B.n
+        buf = self._buffers(nrhs)
+        Xd = X if X is not None else buf["X"]
+        R, D, nrm = buf["R"], buf["D"], buf["nrm"]
+        # x0 = F^-1 b
+        _convert(D, B)
+        self._solve32(D)
+        _convert(Xd, D)
+        nb_ = _colnorms(B, nrm)
+        hist = []
+        it = 0
+        while True:
+            self.residual(B, Xd, R)
+            nr = _colnorms(R, nrm)
+            nx = _colnorms(Xd, nrm)
+            berr = nr / (self.normA * nx + nb_)
+            hist.append(float(berr.max()))
+            if berr.max() <= tol or it >= max_iter:
+                break
+            # stall: no factor-2 progress over two steps -> GMRES-IR
+            if it >= 3 and hist[-1] > 0.5 * hist[-3]:
+                break
+            _convert(D, R)
+            self._solve32(D)
+            td, tx = D.tiled(), Xd.tiled()
+            call("dm_tiled_axpy_s2d", C.byref(tx), C.byref(td), None)
+            it += 1
+        method = "ir"
+        if berr.max() > tol:
+            method = "gmres-ir"
+            it += self._gmres_ir(B, Xd, tol, max_iter)
+            self.residual(B, Xd, R)
+            berr = _colnorms(R, nrm) / (self.normA * _colnorms(Xd, nrm) + nb_)
+        call("dm_stream_sync", None)
+        return Xd, dict(iterations=it, berr=berr, method=method, history=hist)
+
+    def _gmres_ir(self, B, X, tol, max_outer):
+        """GMRES-IR: outer FP64 refinement; each correction solves
+        (M^-1 A) d = M^-1 r by restarted GMRES in FP64 with M = F32^T F32.
+        Vectors live on the host (n-length), products with A and M^-1 on the
+        device (dm_gemm_tiled / the FP32 tiled solve)."""
+        n = self.A.m
+        blk = Blocks(self.nb, self.nb)
+        v64 = DArray(np.float64, (n, 1), blk)
+        w64 = DArray(np.float64, (n, 1), blk)
+        d32 = DArray(np.float32, (n, 1), blk)
+        x_all = X.collect()
+        b_all = B.collect()
+
+        def put(D, v):
+            D.copy_from_host(v.reshape(n, 1))
+
+        def A_mv(v):
+            put(v64, v)
+            ta, tv, tw = self.A.tiled(), v64.tiled(), w64.tiled()
+            o = self.options.c()
+            call("dm_gemm_tiled", b"N", b"N", 1.0, C.byref(ta), C.byref(tv), 0.0, C.byref(tw),
+                 C.byref(o), None)
+            return w64.collect().ravel()
+
+        def Minv(v):
+            put(v64, v)
+            _convert(d32, v64)
+            self._solve32(d32)
+            _convert(v64, d32)
+            return v64.collect().ravel()
+
+        total = 0
+        for j in range(B.n):
+            x = x_all[:, j].copy()
+            b = b_all[:, j]
+            for outer in range(max_outer):
+                r = b - A_mv(x)
+                nr = np.abs(r).max()
+                if nr <= tol * (self.normA * np.abs(x).max() + np.abs(b).max()):
+                    break
+                d = _gmres(lambda v: Minv(A_mv(v)), Minv(r), restart=30, tol=1e-4)
+                x += d
+                total += 1
+            x_all[:, j] = x
+        X.copy_from_host(x_all)
+        return total
+
+
+def _gmres(op, rhs, restart=30, tol=1e-4):
+    """Unrestarted-per-call GMRES (one cycle of `restart` steps) in FP64,
+    modified Gram-Schmidt, Givens rotations."""
+    n = rhs.shape[0]
+    beta = np.linalg.norm(rhs)
+    if beta == 0:
+        return np.zeros(n)
+    Q = np.zeros((restart + 1, n))
+    Hm = np.zeros((restart + 1, restart))
+    cs, sn = np.zeros(restart), np.zeros(restart)
+    g = np.zeros(restart + 1)
+    g[0] = beta
+    Q[0] = rhs / beta
+    k = 0
+    for k in range(restart):
+        w = op(Q[k])
+        for i in range(k + 1):
+            Hm[i, k] = w @ Q[i]
+            w = w - Hm[i, k] * Q[i]
+        Hm[k + 1, k] = np.linalg.norm(w)
+        if Hm[k + 1, k] > 0:
+            Q[k + 1] = w / Hm[k + 1, k]
+        for i in range(k):
+            t = cs[i] * Hm[i, k] + sn[i] * Hm[i + 1, k]
+            Hm[i + 1, k] = -sn[i] * Hm[i, k] + cs[i] * Hm[i + 1, k]
+            Hm[i, k] = t
+        rho = np.hypot(Hm[k, k], Hm[k + 1, k])
+        cs[k], sn[k] = Hm[k, k] / rho, Hm[k + 1, k] / rho
+        Hm[k, k] = rho
+        Hm[k + 1, k] = 0.0
+        g[k + 1] = -sn[k] * g[k]
+        g[k] = cs[k] * g[k]
+        if abs(g[k + 1]) <= tol * beta:
+            break
+    m = k + 1
+    y = np.linalg.solve(np.triu(Hm[:m, :m]), g[:m])
+    return Q[:m].T @ y
+
+
+def householder_generate(n, blocks, kappa, seed, nrefl=64):
+    """A = H diag(s) H^T (kappa(A) = kappa exactly) built on the device from
+    the compact-WY factors of oracle-independent host algebra: A = S - U W^T
+    with U, W n x 2r (one FP64 GEMM n x n x 2r), then S added on the
+    diagonal and the result packed into tiles.  Returns (DArray, spec)."""
+    spec = HouseholderSpec(n, kappa, seed, nrefl)
+    r2 = spec.U.shape[1]
+    U = DeviceBuffer(n * r2 * 8)
+    W = DeviceBuffer(n * r2 * 8)
+    Uf = np.asfortranarray(spec.U)
+    Wf = np.asfortranarray(spec.W)
+    call("dm_memcpy_h2d", C.c_void_p(U.ptr), Uf.ctypes.data_as(C.c_void_p),
+         C.c_size_t(Uf.nbytes), None)
+    call("dm_memcpy_h2d", C.c_void_p(W.ptr), Wf.ctypes.data_as(C.c_void_p),
+         C.c_size_t(Wf.nbytes), None)
+    Fb = DeviceBuffer(n * n * 8)
+    call("dm_dgemm", b"N", b"T", n, n, r2, -1.0, C.c_void_p(U.ptr), n, C.c_void_p(W.ptr), n,
+         0.0, C.c_void_p(Fb.ptr), n, None)
+    sd = DeviceBuffer(n * 8)
+    s = np.ascontiguousarray(spec.s)
+    call("dm_memcpy_h2d", C.c_void_p(sd.ptr), s.ctypes.data_as(C.c_void_p), C.c_size_t(8 * n),
+         None)
+    call("dm_add_diag_d", C.c_void_p(Fb.ptr), n, n, C.c_void_p(sd.ptr), None)
+    D = DArray(np.float64, (n, n), blocks)
+    t = D.tiled()
+    call("dm_tiled_pack", C.byref(t), C.c_void_p(Fb.ptr), n, 0.0, None)
+    call("dm_stream_sync", None)
+    for b in (U, W, Fb, sd):
+        b.free()
+    return D, spec
+
+
+class HouseholderSpec:
+    """Host algebra of A = H diag(s) H^T (compact WY, dlarft columnwise)."""
+
+    def __init__(self, n, kappa, seed, nrefl=64):
+        self.n, self.kappa, self.r = n, float(kappa), nrefl
+        V = _philox_randn(n * nrefl, seed).reshape(nrefl, n).T.copy()
+        tau = 2.0 / np.einsum("ij,ij->j", V, V)
+        T = np.zeros((nrefl, nrefl))
+        for i in range(nrefl):
+            T[i, i] = tau[i]
+            if i:
+                T[:i, i] = -tau[i] * (T[:i, :i] @ (V[:, :i].T @ V[:, i]))
+        self.V, self.T = V, T
+        self.s = self.kappa ** (-np.arange(n) / max(n - 1, 1))
+        Y = self.s[:, None] * (V @ T.T)
+        Z = T @ (V.T @ Y)
+        Q = Y - 0.5 * (V @ Z)
+        self.U = np.hstack([Q, V])
+        self.W = np.hstack([V, Q])
+
+
+def _philox_randn(count, seed):
+    """Device Philox/Box-Muller stream, fetched (count values, offset 0)."""
+    buf = DeviceBuffer(8 * count)
+    call("dm_fill_randn_d", C.c_void_p(buf.ptr), C.c_size_t(count), C.c_uint64(seed),
+         C.c_uint64(0), None)
+    out = np.empty(count)
+    call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
+         C.c_size_t(8 * count), None)
+    buf.free()
+    return out

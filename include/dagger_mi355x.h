@@ -254,6 +254,13 @@ int dm_cholesky_tiled(char uplo, const dm_tiled* A, int* info_dev,
                       int* global_info_dev, const dm_plan_opts* opts,
                       dm_stream_t s);
 
+/* ishermitian(::DArray) (src/array/linalg.jl:90-124), the pre-check the
+ * stdlib cholesky! runs: *result_dev = 1 when every diagonal tile is exactly
+ * symmetric and every off-diagonal pair satisfies ||A_ij - A_ji^T||_F <=
+ * sqrt(eps) max(||A_ij||_F, ||A_ji||_F) (src/array/linalg.jl:38-48), else 0.
+ * work_dev: mt*(mt+1)/2 ints. */
+int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* work_dev, dm_stream_t s);
+
 /* ldiv!(C::Cholesky, B) for a matrix right-hand side (src/array/linalg.jl:
  * 385-401): two tiled trsm! passes (src/array/trsm.jl:59-192) over the
  * resident factor.  B is tiled with Blocks(nb, nb) rows matching A. */
@@ -272,6 +279,28 @@ int dm_trsm_tiled(char side, char uplo, char trans, char diag, double alpha,
                   const dm_tiled* A, const dm_tiled* B,
                   const dm_plan_opts* opts, dm_stream_t s);
 
+/* Tiled C = alpha op(A) op(B) + beta C over resident tiles of one square
+ * block size (one GEMM task per tile triple, beta at the first k-tile): the
+ * tile loop of Dagger's matmul (src/array/mul.jl), here used for the FP64
+ * residual of iterative refinement.  Cached plan per operands. */
+int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A,
+                  const dm_tiled* B, double beta, const dm_tiled* C,
+                  const dm_plan_opts* opts, dm_stream_t s);
+
+/* ---------------------------------------------------- mixed precision
+ * Building blocks of the low-precision-factor + FP64-refinement solve
+ * (dsposv structure; no reference counterpart -- the precision form beside
+ * the plain drop-in). */
+/* dst = (dst dtype) src, elementwise over the packed slab (same shape and
+ * blocks, FP64 <-> FP32). */
+int dm_tiled_convert(const dm_tiled* dst, const dm_tiled* src, dm_stream_t s);
+/* *out_dev = ||A||_inf (max row sum, FP64); work_dev: m doubles. */
+int dm_tiled_norm_inf(const dm_tiled* A, double* out_dev, double* work_dev, dm_stream_t s);
+/* out_dev[j] = max_i |X_ij| for each column j (FP64). */
+int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s);
+/* X += D with X FP64 and D FP32 (same shape and blocks). */
+int dm_tiled_axpy_s2d(const dm_tiled* X, const dm_tiled* D, dm_stream_t s);
+
 /* Full column-major matrix (leading dimension ld) <-> tile-packed layout.
  * pack also adds diag_add to the diagonal (0 for a plain copy).  Device
  * side of copyto!(DA, A)/collect and of the re-blocking copy in
@@ -279,6 +308,8 @@ int dm_trsm_tiled(char side, char uplo, char trans, char diag, double alpha,
 int dm_tiled_pack(const dm_tiled* dst, const void* src, int64_t ld, double diag_add,
                   dm_stream_t s);
 int dm_tiled_unpack(void* dst, int64_t ld, const dm_tiled* src, dm_stream_t s);
+/* A[i,i] += d[i] on a full column-major FP64 matrix (input generators). */
+int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_dev, dm_stream_t s);
 
 /* Drop every cached plan/graph (call before freeing matrices whose plans
  * were cached, or rely on dm_forget_matrix for one base pointer). */

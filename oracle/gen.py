@@ -59,6 +59,57 @@ def gaussian_matrix(n, seed, m=None):
     return randn(n * m, seed).reshape(m, n).T.copy(order="F")
 
 
+class HouseholderSPD:
+    """A = H diag(s) H^T with H = H_1 H_2 ... H_r, H_i = I - tau_i v_i v_i^T,
+    v_i seeded Gaussian vectors (Philox stream `seed`, vector i at offset
+    i*n), s log-spaced from 1 down to 1/kappa, so kappa_2(A) = kappa exactly
+    and A^-1 = H diag(1/s) H^T.
+
+    Compact WY (LAPACK dlarft, forward/columnwise): H = I - V T V^T with T
+    upper triangular.  Then A = S - Q V^T - V Q^T with
+      Y = S V T^T,  Z = T V^T Y (symmetric),  Q = Y - V Z / 2,
+    i.e. a rank-2r update of the diagonal: A = S - U W^T, U = [Q V], W = [V Q].
+    Products with A and A^-1 cost O(n r) per vector (no n x n matrix)."""
+
+    def __init__(self, n, kappa, seed, nrefl=64):
+        self.n, self.kappa, self.r = n, float(kappa), nrefl
+        V = randn(n * nrefl, seed).reshape(nrefl, n).T.copy()
+        self.V = V
+        tau = 2.0 / np.einsum("ij,ij->j", V, V)
+        T = np.zeros((nrefl, nrefl))
+        for i in range(nrefl):
+            T[i, i] = tau[i]
+            if i:
+                T[:i, i] = -tau[i] * (T[:i, :i] @ (V[:, :i].T @ V[:, i]))
+        self.T = T
+        self.s = self.kappa ** (-np.arange(n) / max(n - 1, 1))
+        Y = self.s[:, None] * (V @ T.T)
+        Z = T @ (V.T @ Y)
+        Q = Y - 0.5 * (V @ Z)
+        self.U = np.ascontiguousarray(np.hstack([Q, V]))
+        self.W = np.ascontiguousarray(np.hstack([V, Q]))
+
+    def H(self, x):
+        """H x = x - V T V^T x."""
+        return x - self.V @ (self.T @ (self.V.T @ x))
+
+    def Ht(self, x):
+        return x - self.V @ (self.T.T @ (self.V.T @ x))
+
+    def matvec(self, x):
+        """A x = s * x - U (W^T x)."""
+        x2 = x.reshape(self.n, -1)
+        return (self.s[:, None] * x2 - self.U @ (self.W.T @ x2)).reshape(x.shape)
+
+    def solve(self, b):
+        """A^-1 b = H diag(1/s) H^T b."""
+        b2 = b.reshape(self.n, -1)
+        return self.H(self.Ht(b2) / self.s[:, None]).reshape(b.shape)
+
+    def dense(self):
+        return np.diag(self.s) - self.U @ self.W.T
+
+
 def spd(n, seed):
     """A = G G^T / n + I."""
     G = gaussian_matrix(n, seed)

@@ -1,0 +1,74 @@
+// Stubs: HIP runtime (memory = malloc) and device launches (no-ops that
+// validate pointers are in the plan's allocation where cheap).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <hip/hip_runtime.h>
+#include "dm_internal.h"
+static int g_streams = 0, g_events = 0;
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (hipStream_t)(uintptr_t)(0x1000 + ++g_streams); return 0; }
+hipError_t hipStreamDestroy(hipStream_t s) { if (!s) { fprintf(stderr, "destroy null stream\n"); abort(); } return 0; }
+hipError_t hipEventCreate(hipEvent_t* e) { *e = (hipEvent_t)(uintptr_t)(0x100000 + ++g_events); return 0; }
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+hipError_t hipEventDestroy(hipEvent_t) { return 0; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { if (!e) abort(); return 0; }
+hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 1.f; return 0; }
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t e, unsigned) { if (!e) abort(); return 0; }
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return 0; }
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) { *g = (hipGraph_t)0x777; return 0; }
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t, hipGraphNode_t*, char*, size_t) { *e = (hipGraphExec_t)0x778; return 0; }
+hipError_t hipGraphGetNodes(hipGraph_t, hipGraphNode_t*, size_t* n) { *n = 0; return 0; }
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 0; }
+hipError_t hipGraphDestroy(hipGraph_t) { return 0; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return 0; }
+hipError_t hipMalloc(void** p, size_t n) { *p = malloc(n ? n : 1); return 0; }
+hipError_t hipFree(void* p) { free(p); return 0; }
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return 0; }
+hipError_t hipMemset(void* d, int v, size_t n) { memset(d, v, n); return 0; }
+hipError_t hipDeviceSynchronize() { return 0; }
+hipError_t hipGetLastError() { return 0; }
+const char* hipGetErrorString(hipError_t) { return "stub"; }
+namespace dm {
+static std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+int hip_fail(hipError_t e, const char* w) { fprintf(stderr, "hip_fail %s\n", w); return 1000 + e; }
+int arg_fail(int a, const char* w) { fprintf(stderr, "arg_fail %d %s\n", a, w); return a > 0 ? -a : -99; }
+int ensure_init() { return 0; }
+hipStream_t stream_of(dm_stream_t s) { return s ? s->h : (hipStream_t)0x42; }
+void* stream_workspace(hipStream_t, size_t) { return nullptr; }
+TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
+  TrsmShape s; s.side_l = is_left(side); const bool up = is_upper(uplo), tr = is_trans(transa);
+  if (s.side_l) { s.ttrans = tr; s.lower = tr ? up : !up; } else { s.ttrans = !tr; s.lower = tr ? !up : up; }
+  s.unit = is_unit(diag); return s;
+}
+int trsm_init_attributes() { return 0; }
+template <typename T> GemmCfgId gemm_pick_cfg(int, int, int, int) { return GemmCfgId{0, 128, 128}; }
+template <typename T> long gemm_prepare_jobs(GemmJob<T>* jobs, int nj, GemmCfgId c) {
+  long t = 0; for (int i = 0; i < nj; ++i) { jobs[i].tiles_m = (jobs[i].M + 127) / 128; jobs[i].tiles_n = (jobs[i].N + 127) / 128; jobs[i].wg_begin = (int)t; t += (long)jobs[i].tiles_m * jobs[i].tiles_n; } return t; }
+template <typename T> bool gemm_job_vec_ok(bool, bool, const GemmJob<T>&) { return true; }
+static long g_launch = 0;
+template <typename T> int gemm_launch(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int gemm_single(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t) { if (!inv) abort(); ++g_launch; return 0; }
+template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t) { if (!ws) abort(); ++g_launch; return 0; }
+template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T* ws, hipStream_t) { if (!ws) { fprintf(stderr, "potrf without workspace\n"); abort(); } ++g_launch; return 0; }
+int dgemv_launch(bool, int, int, double, const double*, int, const double*, int, double, double*, int, const int*, hipStream_t) { ++g_launch; return 0; }
+int fill_i32_launch(int*, int, size_t, hipStream_t) { ++g_launch; return 0; }
+int copy_i32_launch(int*, const int*, size_t, hipStream_t) { ++g_launch; return 0; }
+#define INST(T) \
+  template GemmCfgId gemm_pick_cfg<T>(int, int, int, int); \
+  template long gemm_prepare_jobs<T>(GemmJob<T>*, int, GemmCfgId); \
+  template bool gemm_job_vec_ok<T>(bool, bool, const GemmJob<T>&); \
+  template int gemm_launch<T>(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t); \
+  template int gemm_single<T>(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t); \
+  template int trinv_blocks_launch<T>(const TrsmShape&, const T*, int, int, T*, const int*, hipStream_t); \
+  template int trsm_launch<T>(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t); \
+  template int trsm_full<T>(char, char, char, char, int, int, T, const T*, int, T*, int, T*, const int*, hipStream_t); \
+  template int potrf_full<T>(bool, int, T*, int, int*, int*, int, T*, hipStream_t);
+INST(double)
+INST(float)
+}  // namespace dm
+extern "C" const char* dm_last_error_string(void) { return dm::g_err.c_str(); }

@@ -236,3 +236,30 @@ def test_reference_datadeps_cholesky_test(dm):
     assert np.allclose(np.tril(got), np.tril(expected), rtol=1e-8, atol=1e-8)
     # isapprox (rtol = sqrt(eps)) as in the reference test; this input is ill-conditioned
     assert O.rel_factor_diff(got, expected, "L") < np.sqrt(np.finfo(float).eps)
+
+
+def test_ishermitian_rules(dm):
+    """src/array/linalg.jl:38-48,90-124: exact on diagonal tiles, sqrt(eps)
+    Frobenius on off-diagonal pairs; cholesky! rejects with info = -1."""
+    n, nb = 96, 32
+    A = gen.spd(n, 60)
+    assert dm.ishermitian(dm.DArray.from_array(A, dm.Blocks(nb, nb)))
+    B = A.copy()
+    B[40, 5] *= 1 + 1e-12          # off-diagonal pair, within sqrt(eps)
+    assert dm.ishermitian(dm.DArray.from_array(B, dm.Blocks(nb, nb)))
+    B[40, 5] *= 1 + 1e-3           # beyond sqrt(eps)
+    assert not dm.ishermitian(dm.DArray.from_array(B, dm.Blocks(nb, nb)))
+    C = A.copy()
+    C[3, 7] += 1e-15 * C[3, 7] + 1e-300  # inside a diagonal tile: exact rule
+    C[3, 7] = np.nextafter(C[3, 7], np.inf)
+    assert not dm.ishermitian(dm.DArray.from_array(C, dm.Blocks(nb, nb)))
+    # the reference's negative case: a random (non-symmetric) 32x32, Blocks(4,4)
+    R = np.random.default_rng(0).random((32, 32))
+    assert not dm.ishermitian(dm.DArray.from_array(R, dm.Blocks(4, 4)))
+    with pytest.raises(dm.PosDefException) as e:
+        dm.cholesky_(dm.DArray.from_array(R, dm.Blocks(4, 4)))
+    assert e.value.info == -1
+    F = dm.cholesky_(dm.DArray.from_array(R, dm.Blocks(4, 4)), check=False)
+    assert F.info == -1
+    # non-square blocks are re-blocked for the check (Blocks(32, 64) -> 32)
+    assert dm.ishermitian(dm.DArray.from_array(A, dm.Blocks(32, 64)))
