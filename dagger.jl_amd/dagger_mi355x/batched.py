@@ -1,0 +1,210 @@
+"""Batching form: many uneven SPD systems factored and solved by one
+persistent kernel (dm_batched_posv), with the panel width chosen per size
+bucket by an autotuner written here.
+
+Layout: every system's A_m (n x n) and B_m (n x nrhs) are column-major with
+ld = n, packed back to back in two device slabs (like a DArray chunk each).
+The kernel's work queue is an atomic counter over a largest-first order.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+from .darray import DeviceBuffer, DeviceInt
+
+_DT = {np.dtype(np.float64): _lib.DM_F64, np.dtype(np.float32): _lib.DM_F32}
+PANELS = (16, 32, 64)
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("a_off", C.c_int64), ("b_off", C.c_int64), ("n", C.c_int), ("nrhs", C.c_int),
+                ("panel", C.c_int), ("pad_", C.c_int)]
+
+
+def bucket_of(n):
+    """Size bucket: floor(log2 n) (32-63 -> 5, ..., 1024-2047 -> 10, 2048 -> 11)."""
+    return int(np.floor(np.log2(max(n, 1))))
+
+
+class BatchedSystems:
+    """Device-resident batch of SPD systems."""
+
+    def __init__(self, sizes, nrhs, dtype=np.float64):
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        self.nrhs = np.asarray(nrhs, dtype=np.int64)
+        if len(self.sizes) != len(self.nrhs):
+            raise ValueError("sizes and nrhs differ in length")
+        if (self.nrhs < 0).any() or (self.nrhs > 8).any():
+            raise ValueError("nrhs must be in [0, 8]")
+        self.dtype = np.dtype(dtype)
+        es = self.dtype.itemsize
+        self.a_off = np.concatenate([[0], np.cumsum(self.sizes ** 2)[:-1]]).astype(np.int64)
+        self.b_off = np.concatenate([[0], np.cumsum(self.sizes * self.nrhs)[:-1]]).astype(np.int64)
+        self.a_elems = int((self.sizes ** 2).sum())
+        self.b_elems = int((self.sizes * self.nrhs).sum())
+        self.A = DeviceBuffer(max(self.a_elems * es, 8))
+        self.B = DeviceBuffer(max(self.b_elems * es, 8))
+        self.info = DeviceInt(max(len(self.sizes), 1))
+        self.count = len(self.sizes)
+
+    def a_ptr(self, m):
+        return self.A.ptr + int(self.a_off[m]) * self.dtype.itemsize
+
+    def b_ptr(self, m):
+        return self.B.ptr + int(self.b_off[m]) * self.dtype.itemsize
+
+    # ------------------------------------------------------------ data
+    def generate(self, seed):
+        """A_m = G G^T / n + I (G from the Philox stream seed + m) and B_m
+        N(0,1) (stream seed + 10^6 + m), all on the device (FP64 generation,
+        converted when the batch is FP32)."""
+        nmax = int(self.sizes.max()) if self.count else 0
+        G = DeviceBuffer(max(nmax * nmax * 8, 8))
+        F = DeviceBuffer(max(nmax * nmax * 8, 8))
+        eye = np.zeros(0)
+        for m in range(self.count):
+            n = int(self.sizes[m])
+            call("dm_fill_randn_d", C.c_void_p(G.ptr), C.c_size_t(n * n), C.c_uint64(seed + m),
+                 C.c_uint64(0), None)
+            call("dm_dgemm", b"N", b"T", n, n, n, 1.0 / n, C.c_void_p(G.ptr), n,
+                 C.c_void_p(G.ptr), n, 0.0, C.c_void_p(F.ptr), n, None)
+            t = _lib.dm_tiled(C.c_void_p(self.a_ptr(m)), n, n, n, n, _DT[self.dtype])
+            if self.dtype == np.float64:
+                call("dm_tiled_pack", C.byref(t), C.c_void_p(F.ptr), n, 1.0, None)
+            else:
+                t64 = _lib.dm_tiled(C.c_void_p(G.ptr), n, n, n, n, _lib.DM_F64)
+                call("dm_tiled_pack", C.byref(t64), C.c_void_p(F.ptr), n, 1.0, None)
+                call("dm_tiled_convert", C.byref(t), C.byref(t64), None)
+            r = int(self.nrhs[m])
+            if r:
+                if self.dtype == np.float64:
+                    call("dm_fill_randn_d", C.c_void_p(self.b_ptr(m)), C.c_size_t(n * r),
+                         C.c_uint64(seed + 1000000 + m), C.c_uint64(0), None)
+                else:
+                    call("dm_fill_randn_d", C.c_void_p(G.ptr), C.c_size_t(n * r),
+                         C.c_uint64(seed + 1000000 + m), C.c_uint64(0), None)
+                    tb = _lib.dm_tiled(C.c_void_p(self.b_ptr(m)), n, r, n, r, _lib.DM_F32)
+                    tg = _lib.dm_tiled(C.c_void_p(G.ptr), n, r, n, r, _lib.DM_F64)
+                    call("dm_tiled_convert", C.byref(tb), C.byref(tg), None)
+        call("dm_stream_sync", None)
+        G.free()
+        F.free()
+        del eye
+        return self
+
+    def get_system(self, m):
+        """Host copies of (A_m, B_m) (column-major)."""
+        n, r = int(self.sizes[m]), int(self.nrhs[m])
+        A = np.empty((n, n), dtype=self.dtype, order="F")
+        call("dm_memcpy_d2h", A.ctypes.data_as(C.c_void_p), C.c_void_p(self.a_ptr(m)),
+             C.c_size_t(A.nbytes), None)
+        B = np.empty((n, r), dtype=self.dtype, order="F")
+        if r:
+            call("dm_memcpy_d2h", B.ctypes.data_as(C.c_void_p), C.c_void_p(self.b_ptr(m)),
+                 C.c_size_t(B.nbytes), None)
+        return A, B
+
+    def snapshot(self):
+        """Device copies of both slabs (to restore between timed runs)."""
+        a = DeviceBuffer(max(self.a_elems * self.dtype.itemsize, 8))
+        b = DeviceBuffer(max(self.b_elems * self.dtype.itemsize, 8))
+        call("dm_memcpy_d2d", C.c_void_p(a.ptr), C.c_void_p(self.A.ptr),
+             C.c_size_t(self.a_elems * self.dtype.itemsize), None)
+        call("dm_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(self.B.ptr),
+             C.c_size_t(self.b_elems * self.dtype.itemsize), None)
+        return a, b
+
+    def restore(self, snap):
+        a, b = snap
+        call("dm_memcpy_d2d", C.c_void_p(self.A.ptr), C.c_void_p(a.ptr),
+             C.c_size_t(self.a_elems * self.dtype.itemsize), None)
+        call("dm_memcpy_d2d", C.c_void_p(self.B.ptr), C.c_void_p(b.ptr),
+             C.c_size_t(self.b_elems * self.dtype.itemsize), None)
+
+    # ------------------------------------------------------------ solve
+    def items(self, panels):
+        """dm_batch_item array; panels: dict bucket -> P (default 32)."""
+        it = (BatchItem * max(self.count, 1))()
+        for m in range(self.count):
+            n = int(self.sizes[m])
+            it[m].a_off = int(self.a_off[m])
+            it[m].b_off = int(self.b_off[m])
+            it[m].n = n
+            it[m].nrhs = int(self.nrhs[m])
+            it[m].panel = int(panels.get(bucket_of(n), 32))
+        return it
+
+    def order(self, subset=None):
+        idx = np.arange(self.count) if subset is None else np.asarray(subset)
+        o = idx[np.argsort(-self.sizes[idx], kind="stable")]
+        return np.ascontiguousarray(o, dtype=np.int32)
+
+    def posv(self, panels=None, factor_only=False, subset=None, workgroups=0):
+        """Factor (and solve) the batch in one persistent launch."""
+        panels = panels or {}
+        it = self.items(panels)
+        order = self.order(subset)
+        call("dm_batched_posv", _DT[self.dtype], len(order), C.c_void_p(self.A.ptr),
+             C.c_void_p(self.B.ptr), C.cast(it, C.c_void_p), order.ctypes.data_as(C.c_void_p),
+             C.c_void_p(self.info.ptr), int(bool(factor_only)), int(workgroups), None)
+        return self
+
+    def infos(self):
+        return self.info.get()[: self.count]
+
+    def flops(self):
+        """sum n^3/3 (factor) and sum 2 n^2 nrhs (solves)."""
+        n = self.sizes.astype(np.float64)
+        return float((n ** 3 / 3).sum()), float((2 * n * n * self.nrhs).sum())
+
+
+def autotune(systems, sample=24, reps=2):
+    """Pick the panel width per size bucket: for each bucket, time the
+    factorization of up to `sample` of its systems with each candidate P
+    (HIP events, factor only, systems restored between trials) and keep the
+    fastest.  Returns dict bucket -> P and the timing table."""
+    snap = systems.snapshot()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    call("dm_event_create", C.byref(e0))
+    call("dm_event_create", C.byref(e1))
+    buckets = {}
+    for m in range(systems.count):
+        buckets.setdefault(bucket_of(int(systems.sizes[m])), []).append(m)
+    choice, table = {}, {}
+    for b, members in sorted(buckets.items()):
+        sub = members[:sample]
+        best = None
+        for P in PANELS:
+            if P > 2 * int(systems.sizes[sub].max()):
+                continue
+            ms_best = None
+            for _ in range(reps):
+                systems.restore(snap)
+                call("dm_event_record", e0, None)
+                systems.posv(panels={b: P}, factor_only=True, subset=sub)
+                call("dm_event_record", e1, None)
+                call("dm_event_sync", e1)
+                ms = C.c_float()
+                call("dm_event_elapsed_ms", C.byref(ms), e0, e1)
+                ms_best = ms.value if ms_best is None else min(ms_best, ms.value)
+            table[(b, P)] = ms_best
+            if best is None or ms_best < best[1]:
+                best = (P, ms_best)
+        choice[b] = best[0]
+    systems.restore(snap)
+    call("dm_event_destroy", e0)
+    call("dm_event_destroy", e1)
+    return choice, table
+
+
+def log_uniform_sizes(count, lo, hi, seed):
+    """n ~ LogUniform[lo, hi] (integers), nrhs ~ U{1..8}, from numpy's
+    PCG64 seeded generator (host-side problem description only)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), count))).astype(np.int64)
+    sizes = np.clip(sizes, lo, hi)
+    nrhs = rng.integers(1, 9, count)
+    return sizes, nrhs

@@ -316,6 +316,23 @@ int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_dev, dm_stre
 int dm_plan_cache_clear(void);
 int dm_forget_matrix(const void* base);
 
+/* ------------------------------------------------------------- batched
+ * Many uneven SPD systems in one persistent launch (no reference
+ * counterpart: the batching form beside the plain drop-in).  System m:
+ * A_m = A_base + a_off (n x n, column-major, ld = n), B_m = B_base + b_off
+ * (n x nrhs, ld = n, nrhs <= 8), panel width P in [1, 64] (autotuned per size
+ * bucket by the host).  Factored A = U^T U in place (upper triangle) and,
+ * unless factor_only, B overwritten with the solution.  info_dev[m]: LAPACK
+ * info of system m.  order_host: processing order (largest first is best).
+ * workgroups: persistent grid (0 = two per CU). */
+typedef struct dm_batch_item {
+    int64_t a_off, b_off;
+    int n, nrhs, panel, pad_;
+} dm_batch_item;
+int dm_batched_posv(int dtype, int count, void* A_base, void* B_base,
+                    const void* items_host, const int* order_host, int* info_dev,
+                    int factor_only, int workgroups, dm_stream_t s);
+
 /* ---------------------------------------------------------- generators
  * Seeded counter-based normal variates: Philox4x32-10 keyed by `seed`,
  * counter = element index, Box-Muller on the first two 32-bit outputs.
