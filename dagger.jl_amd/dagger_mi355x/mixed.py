@@ -12,9 +12,12 @@ FP64 on the untouched FP64 A (LAPACK dsposv's structure, on resident tiles):
             d = F32^-1 r ; x += d              (FP32 solve, FP64 update)
 
 If refinement stalls (kappa(A) * u32 near 1), `solve` switches to GMRES-IR:
-FP64 restarted GMRES on A with the FP32 factor as a left preconditioner
-(Carson & Higham, "Accelerating the solution of linear systems by iterative
+FP64 GMRES on A with the FP32 factor as a left preconditioner (Carson &
+Higham, "Accelerating the solution of linear systems by iterative
 refinement in three precisions", SISC 2018), one right-hand side at a time.
+
+Every device call takes an explicit stream, so independent solves can
+overlap (the serving loop runs one per worker stream).
 """
 import ctypes as C
 
@@ -29,57 +32,64 @@ from .linalg import PosDefException
 _OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True)
 
 
-def _convert(dst, src):
+def _convert(dst, src, stream=None):
     td, ts = dst.tiled(), src.tiled()
-    call("dm_tiled_convert", C.byref(td), C.byref(ts), None)
+    call("dm_tiled_convert", C.byref(td), C.byref(ts), stream)
 
 
-def _colnorms(X, buf):
+def _colnorms(X, buf, stream=None):
     t = X.tiled()
-    call("dm_tiled_col_norm_inf", C.byref(t), C.c_void_p(buf.ptr), None)
+    call("dm_tiled_col_norm_inf", C.byref(t), C.c_void_p(buf.ptr), stream)
     out = np.empty(X.n)
     call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
-         C.c_size_t(8 * X.n), None)
+         C.c_size_t(8 * X.n), stream)
     return out
 
 
 class MixedCholesky:
     """cholesky(A) in FP32 tiles, kept with the FP64 A for refinement.
 
-    A64 must stay resident and unchanged while the factor is used."""
+    A64 must stay resident and unchanged while the factor is used.  `F`
+    may be passed to reuse a preallocated FP32 DArray of A's shape."""
 
-    def __init__(self, A64, options=None):
+    def __init__(self, A64, options=None, stream=None, F=None):
         if A64.dtype != np.float64:
             raise TypeError("A must be an FP64 DArray")
         self.A = A64
         self.options = options or _OPTS
+        self.stream = stream
         bs = min(A64.partitioning.blocksize)
         if A64.partitioning != Blocks(bs, bs):
             raise ValueError("mixed Cholesky needs square Blocks(nb, nb)")
         self.nb = bs
-        self.F = DArray(np.float32, A64.shape, A64.partitioning)
-        _convert(self.F, self.A)
-        self.info = DeviceInt(1)
-        self.ginfo = DeviceInt(1)
+        self.F = F if F is not None else DArray(np.float32, A64.shape, A64.partitioning)
+        if getattr(self.F, "_mixed_aux", None) is None:
+            self.F._mixed_aux = dict(info=DeviceInt(1), ginfo=DeviceInt(1),
+                                     work=DeviceBuffer(8 * max(A64.m, 2)), norm=DeviceBuffer(16),
+                                     bufs={})
+        aux = self.F._mixed_aux
+        self.info, self.ginfo = aux["info"], aux["ginfo"]
+        self._work, self._norm, self._bufs = aux["work"], aux["norm"], aux["bufs"]
+        self.refactor()
+
+    def refactor(self):
+        """(Re)build the FP32 factor from the current FP64 A."""
+        s = self.stream
+        _convert(self.F, self.A, s)
         t = self.F.tiled()
         o = self.options.c()
         call("dm_cholesky_tiled", b"U", C.byref(t), C.c_void_p(self.info.ptr),
-             C.c_void_p(self.ginfo.ptr), C.byref(o), None)
-        call("dm_stream_sync", None)
-        info = int(self.info.get()[0])
+             C.c_void_p(self.ginfo.ptr), C.byref(o), s)
+        info = int(self.info.get(s)[0])
         if info != 0:
             raise PosDefException(info)
-        # ||A||_inf once
-        self._work = DeviceBuffer(8 * max(A64.m, 2))
-        self._norm = DeviceBuffer(16)
         ta = self.A.tiled()
         call("dm_tiled_norm_inf", C.byref(ta), C.c_void_p(self._norm.ptr),
-             C.c_void_p(self._work.ptr), None)
+             C.c_void_p(self._work.ptr), s)
         out = np.zeros(1)
         call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self._norm.ptr),
-             C.c_size_t(8), None)
+             C.c_size_t(8), s)
         self.normA = float(out[0])
-        self._bufs = {}
 
     # ---------------------------------------------------------------- pieces
     def _buffers(self, nrhs):
@@ -97,91 +107,87 @@ class MixedCholesky:
         """D := F32^-1 D (two FP32 tiled triangular solves)."""
         tf, td = self.F.tiled(), D.tiled()
         o = self.options.c()
-        call("dm_potrs_tiled", b"U", C.byref(tf), C.byref(td), C.byref(o), None)
+        call("dm_potrs_tiled", b"U", C.byref(tf), C.byref(td), C.byref(o), self.stream)
 
     def residual(self, B, X, R):
         """R = B - A X in FP64."""
         call("dm_memcpy_d2d", C.c_void_p(R.buffer.ptr), C.c_void_p(B.buffer.ptr),
-             C.c_size_t(B.m * B.n * 8), None)
+             C.c_size_t(B.m * B.n * 8), self.stream)
         ta, tx, tr = self.A.tiled(), X.tiled(), R.tiled()
         o = self.options.c()
         call("dm_gemm_tiled", b"N", b"N", -1.0, C.byref(ta), C.byref(tx), 1.0, C.byref(tr),
-             C.byref(o), None)
+             C.byref(o), self.stream)
 
     # ---------------------------------------------------------------- solve
     def solve(self, B, tol=1e-13, max_iter=30, X=None):
         """Solve A X = B (B an FP64 DArray N x nrhs, Blocks(nb, nb)).
 
         Returns (X, info) with info = dict(iterations, berr (per column),
-        method)."""
+        method, history)."""
         if B.dtype != np.float64 or B.m != self.A.m:
             raise ValueError("B must be an FP64 DArray with A's rows")
+        s = self.stream
         nrhs = B.n
         buf = self._buffers(nrhs)
         Xd = X if X is not None else buf["X"]
         R, D, nrm = buf["R"], buf["D"], buf["nrm"]
-        # x0 = F^-1 b
-        _convert(D, B)
+        _convert(D, B, s)
         self._solve32(D)
-        _convert(Xd, D)
-        nb_ = _colnorms(B, nrm)
+        _convert(Xd, D, s)
+        nb_ = _colnorms(B, nrm, s)
         hist = []
         it = 0
         while True:
             self.residual(B, Xd, R)
-            nr = _colnorms(R, nrm)
-            nx = _colnorms(Xd, nrm)
+            nr = _colnorms(R, nrm, s)
+            nx = _colnorms(Xd, nrm, s)
             berr = nr / (self.normA * nx + nb_)
             hist.append(float(berr.max()))
             if berr.max() <= tol or it >= max_iter:
                 break
-            # stall: no factor-2 progress over two steps -> GMRES-IR
             if it >= 3 and hist[-1] > 0.5 * hist[-3]:
-                break
-            _convert(D, R)
+                break  # stalled: kappa * u32 too close to 1
+            _convert(D, R, s)
             self._solve32(D)
             td, tx = D.tiled(), Xd.tiled()
-            call("dm_tiled_axpy_s2d", C.byref(tx), C.byref(td), None)
+            call("dm_tiled_axpy_s2d", C.byref(tx), C.byref(td), s)
             it += 1
         method = "ir"
         if berr.max() > tol:
             method = "gmres-ir"
             it += self._gmres_ir(B, Xd, tol, max_iter)
             self.residual(B, Xd, R)
-            berr = _colnorms(R, nrm) / (self.normA * _colnorms(Xd, nrm) + nb_)
-        call("dm_stream_sync", None)
+            berr = _colnorms(R, nrm, s) / (self.normA * _colnorms(Xd, nrm, s) + nb_)
+        call("dm_stream_sync", s)
         return Xd, dict(iterations=it, berr=berr, method=method, history=hist)
 
     def _gmres_ir(self, B, X, tol, max_outer):
         """GMRES-IR: outer FP64 refinement; each correction solves
-        (M^-1 A) d = M^-1 r by restarted GMRES in FP64 with M = F32^T F32.
-        Vectors live on the host (n-length), products with A and M^-1 on the
-        device (dm_gemm_tiled / the FP32 tiled solve)."""
+        (M^-1 A) d = M^-1 r by GMRES in FP64 with M = F32^T F32.  Vectors
+        live on the host (n-length), products with A and M^-1 on the device."""
         n = self.A.m
+        s = self.stream
         blk = Blocks(self.nb, self.nb)
         v64 = DArray(np.float64, (n, 1), blk)
         w64 = DArray(np.float64, (n, 1), blk)
         d32 = DArray(np.float32, (n, 1), blk)
-        x_all = X.collect()
-        b_all = B.collect()
-
-        def put(D, v):
-            D.copy_from_host(v.reshape(n, 1))
+        x_all = X.collect(s)
+        b_all = B.collect(s)
 
         def A_mv(v):
-            put(v64, v)
+            v64.copy_from_host(v.reshape(n, 1), s)
             ta, tv, tw = self.A.tiled(), v64.tiled(), w64.tiled()
             o = self.options.c()
             call("dm_gemm_tiled", b"N", b"N", 1.0, C.byref(ta), C.byref(tv), 0.0, C.byref(tw),
-                 C.byref(o), None)
-            return w64.collect().ravel()
+                 C.byref(o), s)
+            return w64.collect(s).ravel()
 
         def Minv(v):
-            put(v64, v)
-            _convert(d32, v64)
+            v64.copy_from_host(v.reshape(n, 1), s)
+            _convert(d32, v64, s)
             self._solve32(d32)
-            _convert(v64, d32)
-            return v64.collect().ravel()
+            _convert(v64, d32, s)
+            return v64.collect(s).ravel()
 
         total = 0
         for j in range(B.n):
@@ -189,20 +195,18 @@ class MixedCholesky:
             b = b_all[:, j]
             for outer in range(max_outer):
                 r = b - A_mv(x)
-                nr = np.abs(r).max()
-                if nr <= tol * (self.normA * np.abs(x).max() + np.abs(b).max()):
+                if np.abs(r).max() <= tol * (self.normA * np.abs(x).max() + np.abs(b).max()):
                     break
-                d = _gmres(lambda v: Minv(A_mv(v)), Minv(r), restart=30, tol=1e-4)
-                x += d
+                x += _gmres(lambda v: Minv(A_mv(v)), Minv(r), restart=30, tol=1e-4)
                 total += 1
             x_all[:, j] = x
-        X.copy_from_host(x_all)
+        X.copy_from_host(x_all, s)
         return total
 
 
 def _gmres(op, rhs, restart=30, tol=1e-4):
-    """Unrestarted-per-call GMRES (one cycle of `restart` steps) in FP64,
-    modified Gram-Schmidt, Givens rotations."""
+    """One GMRES cycle of at most `restart` steps in FP64 (modified
+    Gram-Schmidt, Givens rotations)."""
     n = rhs.shape[0]
     beta = np.linalg.norm(rhs)
     if beta == 0:
@@ -239,44 +243,17 @@ def _gmres(op, rhs, restart=30, tol=1e-4):
     return Q[:m].T @ y
 
 
-def householder_generate(n, blocks, kappa, seed, nrefl=64):
-    """A = H diag(s) H^T (kappa(A) = kappa exactly) built on the device from
-    the compact-WY factors of oracle-independent host algebra: A = S - U W^T
-    with U, W n x 2r (one FP64 GEMM n x n x 2r), then S added on the
-    diagonal and the result packed into tiles.  Returns (DArray, spec)."""
-    spec = HouseholderSpec(n, kappa, seed, nrefl)
-    r2 = spec.U.shape[1]
-    U = DeviceBuffer(n * r2 * 8)
-    W = DeviceBuffer(n * r2 * 8)
-    Uf = np.asfortranarray(spec.U)
-    Wf = np.asfortranarray(spec.W)
-    call("dm_memcpy_h2d", C.c_void_p(U.ptr), Uf.ctypes.data_as(C.c_void_p),
-         C.c_size_t(Uf.nbytes), None)
-    call("dm_memcpy_h2d", C.c_void_p(W.ptr), Wf.ctypes.data_as(C.c_void_p),
-         C.c_size_t(Wf.nbytes), None)
-    Fb = DeviceBuffer(n * n * 8)
-    call("dm_dgemm", b"N", b"T", n, n, r2, -1.0, C.c_void_p(U.ptr), n, C.c_void_p(W.ptr), n,
-         0.0, C.c_void_p(Fb.ptr), n, None)
-    sd = DeviceBuffer(n * 8)
-    s = np.ascontiguousarray(spec.s)
-    call("dm_memcpy_h2d", C.c_void_p(sd.ptr), s.ctypes.data_as(C.c_void_p), C.c_size_t(8 * n),
-         None)
-    call("dm_add_diag_d", C.c_void_p(Fb.ptr), n, n, C.c_void_p(sd.ptr), None)
-    D = DArray(np.float64, (n, n), blocks)
-    t = D.tiled()
-    call("dm_tiled_pack", C.byref(t), C.c_void_p(Fb.ptr), n, 0.0, None)
-    call("dm_stream_sync", None)
-    for b in (U, W, Fb, sd):
-        b.free()
-    return D, spec
-
-
 class HouseholderSpec:
-    """Host algebra of A = H diag(s) H^T (compact WY, dlarft columnwise)."""
+    """Host algebra of A = H diag(s) H^T (kappa(A) = kappa exactly), H a
+    product of `nrefl` reflectors from seeded Gaussian vectors (device
+    Philox stream `seed`), s log-spaced from 1 to 1/kappa.  Compact WY
+    (dlarft, forward columnwise): H = I - V T V^T, so
+      A = S - Q V^T - V Q^T = S - U W^T,  U = [Q V], W = [V Q],
+      Y = S V T^T, Z = T V^T Y, Q = Y - V Z / 2."""
 
-    def __init__(self, n, kappa, seed, nrefl=64):
+    def __init__(self, n, kappa, seed, nrefl=64, stream=None):
         self.n, self.kappa, self.r = n, float(kappa), nrefl
-        V = _philox_randn(n * nrefl, seed).reshape(nrefl, n).T.copy()
+        V = _philox_randn(n * nrefl, seed, stream).reshape(nrefl, n).T.copy()
         tau = 2.0 / np.einsum("ij,ij->j", V, V)
         T = np.zeros((nrefl, nrefl))
         for i in range(nrefl):
@@ -291,14 +268,59 @@ class HouseholderSpec:
         self.U = np.hstack([Q, V])
         self.W = np.hstack([V, Q])
 
+    def matvec(self, x):
+        x2 = x.reshape(self.n, -1)
+        return (self.s[:, None] * x2 - self.U @ (self.W.T @ x2)).reshape(x.shape)
 
-def _philox_randn(count, seed):
-    """Device Philox/Box-Muller stream, fetched (count values, offset 0)."""
+
+def householder_fill(D, spec, stream=None):
+    """Write A = H diag(s) H^T into the resident DArray D (n_pad >= n):
+    D = [A 0; 0 I] (zero rows appended to U, W and ones to s), via one FP64
+    GEMM n_pad x n_pad x 2r, the diagonal add and the tile pack."""
+    n, npad = spec.n, D.m
+    r2 = spec.U.shape[1]
+    Up = np.zeros((npad, r2), order="F")
+    Wp = np.zeros((npad, r2), order="F")
+    Up[:n] = spec.U
+    Wp[:n] = spec.W
+    sp = np.ones(npad)
+    sp[:n] = spec.s
+    U = DeviceBuffer(npad * r2 * 8)
+    W = DeviceBuffer(npad * r2 * 8)
+    sd = DeviceBuffer(npad * 8)
+    Fb = DeviceBuffer(npad * npad * 8)
+    call("dm_memcpy_h2d", C.c_void_p(U.ptr), Up.ctypes.data_as(C.c_void_p),
+         C.c_size_t(Up.nbytes), stream)
+    call("dm_memcpy_h2d", C.c_void_p(W.ptr), Wp.ctypes.data_as(C.c_void_p),
+         C.c_size_t(Wp.nbytes), stream)
+    call("dm_memcpy_h2d", C.c_void_p(sd.ptr), sp.ctypes.data_as(C.c_void_p),
+         C.c_size_t(8 * npad), stream)
+    call("dm_dgemm", b"N", b"T", npad, npad, r2, -1.0, C.c_void_p(U.ptr), npad, C.c_void_p(W.ptr),
+         npad, 0.0, C.c_void_p(Fb.ptr), npad, stream)
+    call("dm_add_diag_d", C.c_void_p(Fb.ptr), npad, npad, C.c_void_p(sd.ptr), stream)
+    t = D.tiled()
+    call("dm_tiled_pack", C.byref(t), C.c_void_p(Fb.ptr), npad, 0.0, stream)
+    call("dm_stream_sync", stream)
+    for b in (U, W, sd, Fb):
+        b.free()
+    return D
+
+
+def householder_generate(n, blocks, kappa, seed, nrefl=64, stream=None):
+    """A = H diag(s) H^T as a new resident DArray; returns (DArray, spec)."""
+    spec = HouseholderSpec(n, kappa, seed, nrefl, stream)
+    D = DArray(np.float64, (n, n), blocks)
+    householder_fill(D, spec, stream)
+    return D, spec
+
+
+def _philox_randn(count, seed, stream=None):
+    """The device Philox/Box-Muller stream (count values, offset 0)."""
     buf = DeviceBuffer(8 * count)
     call("dm_fill_randn_d", C.c_void_p(buf.ptr), C.c_size_t(count), C.c_uint64(seed),
-         C.c_uint64(0), None)
+         C.c_uint64(0), stream)
     out = np.empty(count)
     call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
-         C.c_size_t(8 * count), None)
+         C.c_size_t(8 * count), stream)
     buf.free()
     return out
