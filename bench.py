@@ -26,7 +26,9 @@ import os
 import sys
 import time
 
-ROOT = os.path.dirname(os.path.abspath(__file__))
+import numpy as np
+
+ROOT =os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dagger.jl_amd"))
 
 METRIC = "Cholesky TFLOP/s N=32k FP64 vs mixed+IR @1e-13 bwd err; served solves/s + p99"
@@ -97,6 +99,66 @@ def gemm_roofline(dm, A0, DA, nb):
     }, {str(k): {"groups": v[0], "ms": round(v[1], 3), "flops": v[2]} for k, v in by_op.items()}
 
 
+def mixed_leg(dm, n, nb, kappa, nrhs, seed):
+    """Precision form at one size: plain FP64 cholesky!+ldiv! vs FP32
+    factor + FP64 refinement to 1e-13, on A = H diag(s) H^T (kappa exact).
+    Both timed with HIP events on the default stream after a warm-up call
+    that builds the plans/graphs."""
+    from dagger_mi355x import mixed
+    call = dm._lib.call
+    timer = Timer(call)
+    blocks = dm.Blocks(nb, nb)
+    A0, spec = mixed.householder_generate(n, blocks, kappa, seed)
+    B0 = dm.randn(blocks, n, nrhs, seed=seed + 1)
+    DA, DB = A0.copy(), B0.copy()
+    res = {}
+    # plain FP64
+    for rep in range(2):
+        restore(call, DA, A0)
+        restore(call, DB, B0)
+        timer.start()
+        F = dm.cholesky_(DA, check_hermitian=False)
+        dm.ldiv_(F, DB)
+        res["fp64_ms"] = timer.stop_ms()
+    X64 = DB.collect()
+    # mixed: FP32 factor + IR
+    Ff = dm.DArray(np.float32, (n, n), blocks)
+    for rep in range(2):
+        timer.start()
+        M = mixed.MixedCholesky(A0, F=Ff)
+        X, info = M.solve(B0, tol=1e-13, max_iter=30)
+        res["mixed_ms"] = timer.stop_ms()
+    Xm = X.collect()
+    # backward error against the closed form A x = s*x - U (W^T x), host side
+    b = B0.collect()
+    r = b - spec.matvec(Xm)
+    closed_berr = float((np.abs(r).max(0) / (M.normA * np.abs(Xm).max(0) +
+                                             np.abs(b).max(0))).max())
+    res.update(kappa=kappa, n=n, nb=nb, nrhs=nrhs, closed_form_berr=closed_berr,
+               speedup=round(res["fp64_ms"] / res["mixed_ms"], 3),
+               mixed_berr=float(np.max(info["berr"])), mixed_iters=info["iterations"],
+               method=info["method"],
+               max_rel_diff_vs_fp64=float(np.abs(Xm - X64).max() / np.abs(X64).max()))
+    for D in (A0, B0, DA, DB, Ff):
+        D.unsafe_free()
+    return res
+
+
+def serving_leg(dm, rate, count, seed, workers=4):
+    """Serving form vs one-at-a-time plain FP64 on the same Poisson trace."""
+    from dagger_mi355x import serving
+    tr1 = serving.make_trace(rate=rate, count=count, seed=seed)
+    tr2 = serving.make_trace(rate=rate, count=count, seed=seed)
+    srv = serving.Server(workers=workers, budget_bytes=200e9)
+    s = srv.run(tr1)
+    srv.close()
+    b = serving.serve_one_at_a_time(tr2)
+    return {"rate": rate, "requests": count, "workers": workers, "server": s,
+            "one_at_a_time_fp64": b,
+            "solves_ratio": round(s["solves_per_s"] / b["solves_per_s"], 3),
+            "p99_ratio": round(s["p99_ms"] / b["p99_ms"], 3)}
+
+
 def cpu_baseline(dm, n, nb, seed):
     """Oracle restatement of Dagger's CPU datadeps Cholesky, threaded."""
     from oracle import dagger_tiled as O
@@ -136,6 +198,11 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=12288,
                     help="N of the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--mixed-n", type=int, default=16384,
+                    help="N of the mixed-precision leg (0 disables it)")
+    ap.add_argument("--serving-requests", type=int, default=120,
+                    help="requests in the serving leg (0 disables it)")
+    ap.add_argument("--serving-rate", type=float, default=1000.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); this build has no "
@@ -198,6 +265,16 @@ def main():
     if not a.no_roofline:
         roof, by_op = gemm_roofline(dm, A0, DA, nb)
         log(f"roofline: {roof}")
+    mixed_res = None
+    if a.mixed_n > 0:
+        DA.unsafe_free()
+        DB.unsafe_free()
+        mixed_res = mixed_leg(dm, a.mixed_n, nb, 1e6, 32, a.seed + 7)
+        log(f"mixed: {mixed_res}")
+    serving_res = None
+    if a.serving_requests > 0:
+        serving_res = serving_leg(dm, a.serving_rate, a.serving_requests, a.seed + 11)
+        log(f"serving: {serving_res}")
     cpu = None
     if a.cpu_n > 0:
         cpu = cpu_baseline(dm, a.cpu_n, nb, a.seed)
@@ -228,6 +305,8 @@ def main():
         "first_call_s": round(first_s, 3),
         "roofline": roof,
         "groups_by_op": by_op,
+        "mixed_ir": mixed_res,
+        "serving": serving_res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
