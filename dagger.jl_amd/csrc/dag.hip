@@ -529,30 +529,37 @@ int Dag::issue(hipStream_t o) {
 }
 
 int Dag::capture() {
-  DM_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
-  capturing = true;
-  int rc = issue(origin);
-  capturing = false;
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(origin, &g);
-  if (rc) {
-    if (g) (void)hipGraphDestroy(g);
-    return rc;
-  }
-  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-  graph = g;
-  DM_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-  size_t nn = 0;
-  (void)hipGraphGetNodes(graph, nullptr, &nn);
-  graph_nodes = (int)nn;
-  return 0;
+  // Capture, instantiate (and later launch) on the big-stack helper thread:
+  // the runtime's graph walk recurses along dependency chains.
+  return run_big_stack([this]() -> int {
+    DM_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+    int rc = issue(origin);
+    capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(origin, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    graph = g;
+    DM_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    size_t nn = 0;
+    (void)hipGraphGetNodes(graph, nullptr, &nn);
+    graph_nodes = (int)nn;
+    return 0;
+  });
 }
 
 int Dag::run(hipStream_t s) {
   if (!planned) return arg_fail(1, "dag not planned");
   if (graph_exec) {
-    DM_HIP(hipGraphLaunch(graph_exec, s));
-    return 0;
+    hipGraphExec_t ex = graph_exec;
+    return run_big_stack([ex, s]() -> int {
+      DM_HIP(hipGraphLaunch(ex, s));
+      return 0;
+    });
   }
   // Eager issue: the origin stream is the caller's stream.
   return issue(s);

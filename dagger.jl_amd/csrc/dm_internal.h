@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include "../../include/dagger_mi355x.h"
 
@@ -34,6 +35,11 @@ int arg_fail(int argno, const char* what);
 
 int ensure_init();
 hipStream_t stream_of(dm_stream_t s);
+
+// Runs fn on a helper thread with a 1 GiB stack and returns its result
+// (graph capture/instantiate/launch: the HIP runtime recurses over graph
+// dependency chains).  Calls are serialized.
+int run_big_stack(std::function<int()> fn);
 
 // Per-stream scratch buffer (grown on demand, never shrunk while the
 // library is loaded).  Calls ordered on one stream may reuse it.

@@ -1,8 +1,11 @@
 // C ABI: runtime (errors, device, memory, streams, events, graphs) and the
 // BLAS/LAPACK-shaped tile entry points.
 #include <execinfo.h>
+#include <pthread.h>
 #include <unistd.h>
+#include <condition_variable>
 #include <csignal>
+#include <functional>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -79,6 +82,78 @@ int ensure_init() {
 }
 
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : g_default_stream; }
+
+// ----------------------------------------------------------- big-stack runner
+// The HIP runtime walks a captured graph recursively (recursion depth grows
+// with the longest dependency chain of the graph), which overflows a default
+// 8 MiB thread stack for a large tiled factorization.  Graph capture,
+// instantiation and launch therefore run on one helper thread with a 1 GiB
+// stack (reserved, committed lazily); callers block until their call returns.
+namespace {
+struct BigStackRunner {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::function<int()>* job = nullptr;
+  int result = 0;
+  std::string err;
+  bool done = false, started = false;
+  pthread_t th;
+
+  static void* main_fn(void* self) {
+    auto* r = (BigStackRunner*)self;
+    if (g_device >= 0) (void)hipSetDevice(g_device);
+    std::unique_lock<std::mutex> lk(r->mu);
+    for (;;) {
+      r->cv.wait(lk, [&] { return r->job != nullptr; });
+      std::function<int()>* j = r->job;
+      lk.unlock();
+      const int res = (*j)();
+      lk.lock();
+      r->err = res ? g_last_error : std::string();
+      r->result = res;
+      r->job = nullptr;
+      r->done = true;
+      r->cv.notify_all();
+    }
+    return nullptr;
+  }
+
+  int run(std::function<int()>& fn) {
+    std::unique_lock<std::mutex> lk(mu);
+    if (!started) {
+      pthread_attr_t attr;
+      pthread_attr_init(&attr);
+      pthread_attr_setstacksize(&attr, (size_t)1 << 30);
+      if (pthread_create(&th, &attr, &BigStackRunner::main_fn, this) != 0) {
+        pthread_attr_destroy(&attr);
+        lk.unlock();
+        return fn();  // fall back to the caller's stack
+      }
+      pthread_attr_destroy(&attr);
+      pthread_detach(th);
+      started = true;
+    }
+    cv.wait(lk, [&] { return job == nullptr; });
+    job = &fn;
+    done = false;
+    cv.notify_all();
+    cv.wait(lk, [&] { return done; });
+    done = false;
+    const int res = result;
+    if (res) set_error(err);  // the message was recorded on the helper thread
+    cv.notify_all();
+    return res;
+  }
+};
+BigStackRunner g_big_stack;
+std::mutex g_big_stack_serial;
+}  // namespace
+
+int run_big_stack(std::function<int()> fn) {
+  // one job at a time through the helper thread
+  std::lock_guard<std::mutex> lk(g_big_stack_serial);
+  return g_big_stack.run(fn);
+}
 
 // --------------------------------------------------------------- workspace
 namespace {
@@ -314,10 +389,13 @@ int dm_graph_end(dm_stream_t s, dm_graph_t* g) {
   hipGraph_t graph = nullptr;
   DM_HIP(hipStreamEndCapture(stream_of(s), &graph));
   hipGraphExec_t exec = nullptr;
-  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
+  int rc = run_big_stack([&]() -> int {
+    DM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return 0;
+  });
+  if (rc) {
     (void)hipGraphDestroy(graph);
-    return hip_fail(e, "hipGraphInstantiate");
+    return rc;
   }
   auto* gg = new dm_graph_s;
   gg->g = graph;
@@ -329,8 +407,12 @@ int dm_graph_end(dm_stream_t s, dm_graph_t* g) {
 int dm_graph_launch(dm_graph_t g, dm_stream_t s) {
   if (!g) return arg_fail(1, "graph is NULL");
   DM_INIT();
-  DM_HIP(hipGraphLaunch(g->exec, stream_of(s)));
-  return 0;
+  hipGraphExec_t ex = g->exec;
+  hipStream_t st = stream_of(s);
+  return run_big_stack([ex, st]() -> int {
+    DM_HIP(hipGraphLaunch(ex, st));
+    return 0;
+  });
 }
 
 int dm_graph_destroy(dm_graph_t g) {

@@ -44,6 +44,7 @@ TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
   s.unit = is_unit(diag); return s;
 }
 int trsm_init_attributes() { return 0; }
+int run_big_stack(std::function<int()> fn) { return fn(); }
 template <typename T> GemmCfgId gemm_pick_cfg(int, int, int, int) { return GemmCfgId{0, 128, 128}; }
 template <typename T> long gemm_prepare_jobs(GemmJob<T>* jobs, int nj, GemmCfgId c) {
   long t = 0; for (int i = 0; i < nj; ++i) { jobs[i].tiles_m = (jobs[i].M + 127) / 128; jobs[i].tiles_n = (jobs[i].N + 127) / 128; jobs[i].wg_begin = (int)t; t += (long)jobs[i].tiles_m * jobs[i].tiles_n; } return t; }
