@@ -247,7 +247,11 @@ int Dag::plan(const dm_plan_opts& o) {
   rc = create_streams();
   if (rc) return rc;
   planned = true;
-  if (opts.use_graph) {
+  // Graph capture is limited to plans with at most two worker streams: a
+  // capture of the three-stream look-ahead-2 schedule sent the runtime's
+  // graph walk into unbounded recursion (stack overflow in libamdhip64),
+  // so wider plans always issue eagerly.
+  if (opts.use_graph && opts.num_streams <= 2) {
     rc = capture();
     if (rc) return rc;
   }

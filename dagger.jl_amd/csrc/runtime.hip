@@ -145,14 +145,17 @@ struct BigStackRunner {
     return res;
   }
 };
-BigStackRunner g_big_stack;
-std::mutex g_big_stack_serial;
+// Heap-allocated and never destroyed: the detached helper blocks on the
+// condition variable for the life of the process, and destroying a
+// condition variable that a thread still waits on can hang process exit.
+BigStackRunner* g_big_stack = new BigStackRunner;
+std::mutex* g_big_stack_serial = new std::mutex;
 }  // namespace
 
 int run_big_stack(std::function<int()> fn) {
   // one job at a time through the helper thread
-  std::lock_guard<std::mutex> lk(g_big_stack_serial);
-  return g_big_stack.run(fn);
+  std::lock_guard<std::mutex> lk(*g_big_stack_serial);
+  return g_big_stack->run(fn);
 }
 
 // --------------------------------------------------------------- workspace

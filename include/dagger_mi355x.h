@@ -189,7 +189,9 @@ typedef struct dm_plan_opts {
     int batch;           /* 1: batch independent same-shape tasks per launch */
     int lookahead;       /* slack (in levels) below which tasks go on the
                             critical stream; 0 disables the split */
-    int use_graph;       /* 1: capture the schedule into a hipGraph */
+    int use_graph;       /* 1: capture the schedule into a hipGraph (plans
+                            with at most 2 streams; wider plans issue
+                            eagerly, dm_dag_stats.graph reports 0) */
     int profile;         /* 1 (eager runs only): time every launch group
                             with HIP events on its own stream */
 } dm_plan_opts;

@@ -82,12 +82,27 @@ def test_schedules_are_bit_identical(dm, uplo):
     for opts in (dm.SEQUENTIAL,
                  dm.PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=False),
                  dm.PlanOptions(num_streams=4, batch=True, lookahead=0, use_graph=False),
-                 dm.PlanOptions(num_streams=3, batch=True, lookahead=2, use_graph=True)):
+                 dm.PlanOptions(num_streams=3, batch=True, lookahead=2, use_graph=False),
+                 dm.PlanOptions(num_streams=2, batch=True, lookahead=2, use_graph=True)):
         for rep in range(2 if opts.use_graph else 1):
             _, f = factor_gpu(dm, A, nb, uplo, options=opts)
             outs.append(f)
     for f in outs[1:]:
         assert np.array_equal(f, outs[0])
+
+
+def test_wide_plans_issue_eagerly(dm):
+    """Graph capture is limited to plans with at most two streams; a wider
+    plan that asks for one runs eagerly and reports no graph."""
+    n, nb = 768, 256
+    DA = dm.DArray.from_array(gen.spd(n, 7), dm.Blocks(nb, nb))
+    info = dm.DeviceInt(2)
+    dag = dm.Dag.cholesky(DA, "U", info)
+    dag.plan(dm.PlanOptions(num_streams=3, batch=True, lookahead=2, use_graph=True))
+    assert dag.stats()["graph"] == 0
+    dag2 = dm.Dag.cholesky(DA, "U", info)
+    dag2.plan(dm.PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True))
+    assert dag2.stats()["graph"] == 1
 
 
 def test_graph_replays_on_same_matrix(dm):
