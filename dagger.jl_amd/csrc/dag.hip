@@ -28,9 +28,11 @@
 #include <map>
 #include <memory>
 #include <set>
+#include <type_traits>
 #include <vector>
 #include "dm_internal.h"
 #include "dag.h"
+#include <type_traits>
 
 namespace dm {
 
@@ -318,6 +320,13 @@ int Dag::build_device_tables() {
         G.njobs = (int)jobs.size();
         if (!G.njobs) return 0;
         G.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, G.njobs);
+        if constexpr (std::is_same<T, float>::value) {
+          if (opts.f32_update == 3 || opts.f32_update == 6) {
+            bool split = true;
+            for (auto& j : jobs) split = split && gemm_split_ok(ta, tb, j);
+            if (split) G.cfg = gemm_split_cfg(opts.f32_update);
+          }
+        }
         G.total_wg = gemm_prepare_jobs<T>(jobs.data(), G.njobs, G.cfg);
         G.vec = true;
         for (auto& j : jobs) G.vec = G.vec && gemm_job_vec_ok<T>(ta, tb, j);

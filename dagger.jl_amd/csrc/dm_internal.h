@@ -99,6 +99,18 @@ int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A,
                 int lda, const T* B, int ldb, T beta, T* C, int ldc, int tri,
                 const int* abort, hipStream_t s);
 
+// FP32 updates on the BF16 matrix cores with operand splitting
+// (gemm_split.hip).  GemmCfgId ids for FP32 launches; jobs tile 128x128.
+constexpr int GEMM_CFG_BF16X3 = 2, GEMM_CFG_BF16X6 = 3;
+inline GemmCfgId gemm_split_cfg(int terms) {
+  return GemmCfgId{terms == 6 ? GEMM_CFG_BF16X6 : GEMM_CFG_BF16X3, 128, 128};
+}
+// op(A) = A^T, op(B) = B (k-contiguous operands), 16-byte aligned rows.
+bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j);
+int gemm_split_launch(int terms, const GemmJob<float>* jobs_dev, int njobs,
+                      const GemmJob<float>& single, long total_wg, const int* abort,
+                      hipStream_t s);
+
 // ------------------------------------------------------------ TRSM family
 // Effective left solve  Tm X = alpha V  on L-long vectors, NV of them.
 //   Tm(i,k) = ttrans ? A[k + i*lda] : A[i + k*lda], lower or upper.

@@ -22,6 +22,8 @@
 //    touch the requested triangle are launched, and diagonal tiles store
 //    masked.
 #include "dm_internal.h"
+#include "gemm_tiles.h"
+#include <type_traits>
 
 namespace dm {
 
@@ -49,23 +51,6 @@ template <> struct VecT<double, 1> { typedef double t; };
 template <> struct VecT<double, 2> { typedef double t __attribute__((ext_vector_type(2))); };
 template <> struct VecT<float, 1> { typedef float t; };
 template <> struct VecT<float, 2> { typedef float t __attribute__((ext_vector_type(2))); };
-
-// Logical workgroup index such that each XCD receives a contiguous range.
-__device__ __forceinline__ int xcd_remap(int bid, int nwg) {
-  const int q = nwg >> 3, r = nwg & 7;
-  const int xcd = bid & 7, slot = bid >> 3;
-  const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-  return base + slot;
-}
-
-// Decode the idx-th tile (column by column) of an upper triangle of tiles.
-__device__ __forceinline__ void upper_tri_decode(int idx, int& tm, int& tn) {
-  int c = (int)((sqrtf(8.0f * (float)idx + 1.0f) - 1.0f) * 0.5f);
-  while ((c + 1) * (c + 2) / 2 <= idx) ++c;
-  while (c * (c + 1) / 2 > idx) --c;
-  tn = c;
-  tm = idx - c * (c + 1) / 2;
-}
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
 struct GemmKernel {
@@ -280,30 +265,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __
   typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC> G;
   __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
   if (abort && *abort) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
   GemmJob<T> job;
-  int local;
-  if (njobs == 0) {
-    job = single;
-    local = wg;
-  } else {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
-    }
-    job = jobs[lo];
-    local = wg - job.wg_begin;
-  }
   int tm, tn;
-  if (job.tri == 0) {
-    tm = local % job.tiles_m;
-    tn = local / job.tiles_m;
-  } else if (job.tri == 1) {
-    upper_tri_decode(local, tm, tn);
-  } else {
-    upper_tri_decode(local, tn, tm);
-  }
+  locate_tile(jobs, njobs, single, job, tm, tn);
   G::run(job, tm, tn, smem);
 }
 
@@ -382,6 +346,12 @@ int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* job
                 const GemmJob<T>& single, long total_wg, const int* abort, hipStream_t s) {
   if (total_wg <= 0) return 0;
   if (total_wg > 0x7fffffffL) return arg_fail(0, "gemm: too many workgroups");
+  if (cfg.id == GEMM_CFG_BF16X3 || cfg.id == GEMM_CFG_BF16X6) {
+    if constexpr (std::is_same<T, float>::value)
+      return gemm_split_launch(cfg.id == GEMM_CFG_BF16X6 ? 6 : 3, jobs_dev, njobs, single,
+                               total_wg, abort, s);
+    return arg_fail(0, "gemm: BF16 split configurations are FP32 only");
+  }
   if (vec) return launch_tt<T, 2>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, abort, s);
   return launch_tt<T, 1>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, abort, s);
 }

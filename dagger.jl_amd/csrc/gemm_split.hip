@@ -1,0 +1,227 @@
+// FP32 GEMM / SYRK tile updates computed on the BF16 matrix cores.
+//
+// Used for the off-diagonal trailing updates of the mixed-precision
+// Cholesky (FP32 tiles; the FP64 residual and refinement recover full
+// accuracy).  gfx950 runs dense BF16 MFMA at 16x the FP32 MFMA rate, so an
+// FP32 product evaluated as a short sum of BF16 products is still several
+// times faster than v_mfma_f32_16x16x4f32.
+//
+// Splitting (Henry, Tang & Heinecke, "Leveraging the bfloat16 Artificial
+// Intelligence Datatype For Higher-Precision Computations", ARITH 2019):
+//   x = h + m + l,  h = bf16(x), m = bf16(x - h), l = bf16(x - h - m)
+// Each difference is exact in FP32, so h + m + l carries 24 significant bits.
+//   TERMS = 6 ("bf16x6"): a*b ~ hh + (hm + mh) + (hl + lh + mm)  (FP32-like)
+//   TERMS = 3 ("bf16x3"): a*b ~ hh + (hm + mh)                   (~2^-16)
+// BF16 x BF16 products are exact in the FP32 accumulator; the small terms
+// are issued first.
+//
+// Kernel shape (wave64, v_mfma_f32_16x16x32_bf16):
+//  * 128x128 C tile per workgroup, 2x2 waves of 64x64, BK = 32 (one MFMA k).
+//  * Operands must be contiguous along k (C = alpha op(A) op(B) + beta C with
+//    op(A) = A^T and op(B) = B, the shapes of the upper tiled Cholesky's
+//    SYRK('U','T') and GEMM('T','N')): FP32 rows are staged global ->
+//    registers -> LDS ([mn][BK+4] floats, double buffered) and split into
+//    BF16 terms when the MFMA fragments are read, so LDS holds plain FP32
+//    (72 KB, two workgroups per CU).
+//  * MFMA operand layout (measured on gfx950): lane l holds row l%16,
+//    k = 8*(l/16) .. 8*(l/16)+7 of both operands; the 16x16 result D[i][j]
+//    sits in lane j + 16*(i/4), item i%4.  The MFMA computes C^T (operand
+//    roles swapped) so that a quarter-wave stores 16 consecutive rows of a
+//    column-major C.
+#include "dm_internal.h"
+#include "gemm_tiles.h"
+
+namespace dm {
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int TERMS>
+struct SplitGemm {
+  static constexpr int BM = 128, BN = 128, BK = 32, WM = 2, WN = 2;
+  static constexpr int NT = 64 * WM * WN;
+  static constexpr int WTM = BM / WM, WTN = BN / WN;
+  static constexpr int FM = WTM / 16, FN = WTN / 16;
+  static constexpr int SK = BK + 4;  // floats per LDS row (144 B: conflict-free b128 reads)
+  static constexpr int A_ELEMS = BM * SK, B_ELEMS = BN * SK;
+  static constexpr int STAGE = A_ELEMS + B_ELEMS;
+  static constexpr int LA = BM * BK / (4 * NT);  // float4 loads per thread
+  static constexpr int LB = BN * BK / (4 * NT);
+  static constexpr int P = TERMS == 6 ? 3 : 2;   // BF16 terms per operand
+  static_assert(LA * 4 * NT == BM * BK && LB * 4 * NT == BN * BK, "tile split");
+
+  // Row r (of op(X) with k contiguous: X[k + r*ld]) chunk k0 + 4c .. +3.
+  static __device__ __forceinline__ f4 load4(const float* __restrict__ X, int ld, int r, int k,
+                                             int R, int K) {
+    if (r >= R) return f4{0, 0, 0, 0};
+    const float* p = X + (size_t)r * ld + k;
+    if (k + 3 < K) return *(const f4*)p;
+    f4 v{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k + q < K) v[q] = p[q];
+    return v;
+  }
+
+  static __device__ __forceinline__ void load(const GemmJob<float>& job, int m0, int n0, int k0,
+                                              f4 (&ra)[LA], f4 (&rb)[LB]) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = threadIdx.x + i * NT;
+      ra[i] = load4(job.A, job.lda, m0 + e / (BK / 4), k0 + (e % (BK / 4)) * 4, job.M, job.K);
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = threadIdx.x + i * NT;
+      rb[i] = load4(job.B, job.ldb, n0 + e / (BK / 4), k0 + (e % (BK / 4)) * 4, job.N, job.K);
+    }
+  }
+
+  static __device__ __forceinline__ void store(float* __restrict__ st, const f4 (&ra)[LA],
+                                               const f4 (&rb)[LB]) {
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = threadIdx.x + i * NT;
+      *(f4*)&st[(e / (BK / 4)) * SK + (e % (BK / 4)) * 4] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = threadIdx.x + i * NT;
+      *(f4*)&st[A_ELEMS + (e / (BK / 4)) * SK + (e % (BK / 4)) * 4] = rb[i];
+    }
+  }
+
+  // 8 consecutive k of one row from LDS, split into P BF16 terms.
+  static __device__ __forceinline__ void frag(const float* __restrict__ row, bf8 (&t)[P]) {
+    const f4 lo = *(const f4*)row;
+    const f4 hi = *(const f4*)(row + 4);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float x = e < 4 ? lo[e] : hi[e - 4];
+      const __bf16 h = (__bf16)x;
+      float r = x - (float)h;
+      const __bf16 m = (__bf16)r;
+      t[0][e] = h;
+      t[1][e] = m;
+      if (P == 3) {
+        r -= (float)m;
+        t[P - 1][e] = (__bf16)r;
+      }
+    }
+  }
+
+  static __device__ __forceinline__ void run(const GemmJob<float>& job, int tm, int tn,
+                                             float* smem) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    f4 ra[LA], rb[LB];
+    const int nk = (job.K + BK - 1) / BK;
+    if (nk > 0) {
+      load(job, m0, n0, 0, ra, rb);
+      store(smem, ra, rb);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const float* st = smem + (kt & 1) * STAGE;
+      const bool more = kt + 1 < nk;
+      if (more) load(job, m0, n0, (kt + 1) * BK, ra, rb);
+      bf8 af[FM][P], bf[FN][P];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) frag(st + (wm0 + i * 16 + lr) * SK + 8 * lk, af[i]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) frag(st + A_ELEMS + (wn0 + j * 16 + lr) * SK + 8 * lk, bf[j]);
+      // (A term, B term) products, smallest first
+      constexpr int NPROD = TERMS;
+      constexpr int pa6[6] = {2, 0, 1, 1, 0, 0}, pb6[6] = {0, 2, 1, 0, 1, 0};
+      constexpr int pa3[3] = {1, 0, 0}, pb3[3] = {0, 1, 0};
+#pragma unroll
+      for (int q = 0; q < NPROD; ++q) {
+        const int pa = TERMS == 6 ? pa6[q] : pa3[q];
+        const int pb = TERMS == 6 ? pb6[q] : pb3[q];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[j][pb], af[i][pa], acc[i][j],
+                                                                0, 0, 0);
+      }
+      if (more) store(smem + ((kt + 1) & 1) * STAGE, ra, rb);
+      __syncthreads();
+    }
+
+    // lane holds C[m][n] with m = .. + lr, n = .. + 4*lk + r
+    const float alpha = job.alpha, beta = job.beta;
+    const int tri = job.tri;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int m = m0 + wm0 + i * 16 + lr;
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn0 + j * 16 + 4 * lk + r;
+          bool ok = m < job.M && n < job.N;
+          if (tri == 1) ok = ok && m <= n;
+          else if (tri == 2) ok = ok && m >= n;
+          if (ok) {
+            float* c = job.C + (size_t)n * job.ldc + m;
+            const float v = alpha * acc[i][j][r];
+            *c = (beta == 0.0f) ? v : v + beta * *c;
+          }
+        }
+      }
+    }
+  }
+};
+
+template <int TERMS>
+__global__ __launch_bounds__(256) void gemm_split_kernel(const GemmJob<float>* __restrict__ jobs,
+                                                         int njobs, GemmJob<float> single,
+                                                         const int* __restrict__ abort) {
+  typedef SplitGemm<TERMS> G;
+  __shared__ float smem[2 * G::STAGE];
+  if (abort && *abort) return;
+  GemmJob<float> job;
+  int tm, tn;
+  locate_tile(jobs, njobs, single, job, tm, tn);
+  G::run(job, tm, tn, smem);
+}
+
+}  // namespace
+
+bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
+  // op(A) = A^T and op(B) = B (both contiguous along k), 16-byte rows.
+  if (!ta || tb) return false;
+  if (((uintptr_t)j.A & 15) || ((uintptr_t)j.B & 15)) return false;
+  if ((j.lda & 3) || (j.ldb & 3)) return false;
+  return true;
+}
+
+int gemm_split_launch(int terms, const GemmJob<float>* jobs_dev, int njobs,
+                      const GemmJob<float>& single, long total_wg, const int* abort,
+                      hipStream_t s) {
+  if (total_wg <= 0) return 0;
+  if (total_wg > 0x7fffffffL) return arg_fail(0, "gemm: too many workgroups");
+  if (terms == 6)
+    hipLaunchKernelGGL(gemm_split_kernel<6>, dim3((unsigned)total_wg), dim3(256), 0, s, jobs_dev,
+                       njobs, single, abort);
+  else if (terms == 3)
+    hipLaunchKernelGGL(gemm_split_kernel<3>, dim3((unsigned)total_wg), dim3(256), 0, s, jobs_dev,
+                       njobs, single, abort);
+  else
+    return arg_fail(1, "split GEMM terms must be 3 or 6");
+  DM_LAUNCH_CHECK("gemm_split_kernel");
+  return 0;
+}
+
+}  // namespace dm

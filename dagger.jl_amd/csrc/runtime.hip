@@ -534,6 +534,31 @@ extern "C" int dm_sgemm(char transa, char transb, int m, int n, int k, float alp
   return gemm_api<float>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
+extern "C" int dm_sgemm_bf16x(int terms, char transa, char transb, int m, int n, int k, float alpha,
+                              const float* A, int lda, const float* B, int ldb, float beta,
+                              float* C, int ldc, dm_stream_t s) {
+  if (terms != 3 && terms != 6) return arg_fail(1, "terms must be 3 or 6");
+  if (!is_trans(transa)) return arg_fail(2, "transa must be 'T' (k-contiguous A)");
+  if (!is_notrans(transb)) return arg_fail(3, "transb must be 'N' (k-contiguous B)");
+  if (m < 0) return arg_fail(4, "m < 0");
+  if (n < 0) return arg_fail(5, "n < 0");
+  if (k < 0) return arg_fail(6, "k < 0");
+  if (lda < (k > 1 ? k : 1) || (lda & 3)) return arg_fail(9, "lda < max(1,k) or not a multiple of 4");
+  if (ldb < (k > 1 ? k : 1) || (ldb & 3)) return arg_fail(11, "ldb < max(1,k) or not a multiple of 4");
+  if (ldc < (m > 1 ? m : 1)) return arg_fail(14, "ldc < max(1,m)");
+  if (((uintptr_t)A & 15) || ((uintptr_t)B & 15)) return arg_fail(8, "A and B must be 16-byte aligned");
+  DM_INIT();
+  if (m == 0 || n == 0) return 0;
+  GemmJob<float> j{};
+  j.A = A; j.B = B; j.C = C;
+  j.M = m; j.N = n; j.K = k;
+  j.lda = lda; j.ldb = ldb; j.ldc = ldc;
+  j.alpha = alpha; j.beta = beta; j.tri = 0;
+  if (k == 0 || alpha == 0.0f) { j.alpha = 0.0f; j.K = 0; }
+  const long total = gemm_prepare_jobs<float>(&j, 1, gemm_split_cfg(terms));
+  return gemm_split_launch(terms, nullptr, 0, j, total, nullptr, stream_of(s));
+}
+
 extern "C" int dm_dtrsv(char uplo, char trans, char diag, int n, const double* A, int lda, double* x,
              int incx, dm_stream_t s) {
   if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(1, "uplo must be 'U' or 'L'");

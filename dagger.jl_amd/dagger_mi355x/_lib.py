@@ -42,7 +42,7 @@ class dm_task(C.Structure):
 
 class dm_plan_opts(C.Structure):
     _fields_ = [("num_streams", i32), ("batch", i32), ("lookahead", i32), ("use_graph", i32),
-                ("profile", i32)]
+                ("profile", i32), ("f32_update", i32)]
 
 
 class dm_group_profile(C.Structure):
@@ -97,6 +97,8 @@ _PROTOS = {
     "dm_ssyrk": (i32, [c_char, c_char, i32, i32, f32, vp, i32, f32, vp, i32, vp]),
     "dm_dgemm": (i32, [c_char, c_char, i32, i32, i32, f64, vp, i32, vp, i32, f64, vp, i32, vp]),
     "dm_sgemm": (i32, [c_char, c_char, i32, i32, i32, f32, vp, i32, vp, i32, f32, vp, i32, vp]),
+    "dm_sgemm_bf16x": (i32, [i32, c_char, c_char, i32, i32, i32, f32, vp, i32, vp, i32, f32, vp,
+                             i32, vp]),
     "dm_dtrsv": (i32, [c_char, c_char, c_char, i32, vp, i32, vp, i32, vp]),
     "dm_dgemv": (i32, [c_char, i32, i32, f64, vp, i32, vp, i32, f64, vp, i32, vp]),
     "dm_dag_create": (i32, [C.POINTER(vp)]),

@@ -160,16 +160,28 @@ def make_task(f, args):
 
 
 class PlanOptions:
-    """Executor knobs: streams, batching, look-ahead depth, hipGraph."""
+    """Executor knobs: streams, batching, look-ahead depth, hipGraph, and the
+    arithmetic of FP32 GEMM/SYRK groups (f32_update: 0 FP32 matrix cores,
+    6 or 3 BF16 matrix cores with 3- or 2-term operand splitting)."""
 
-    def __init__(self, num_streams=2, batch=True, lookahead=1, use_graph=False, profile=False):
+    def __init__(self, num_streams=2, batch=True, lookahead=1, use_graph=False, profile=False,
+                 f32_update=0):
+        if f32_update not in (0, 3, 6):
+            raise ValueError("f32_update must be 0, 3 or 6")
         self.num_streams, self.batch = num_streams, batch
         self.lookahead, self.use_graph, self.profile = lookahead, use_graph, profile
+        self.f32_update = f32_update
 
     def c(self):
         return _lib.dm_plan_opts(int(self.num_streams), int(bool(self.batch)),
                                  int(self.lookahead), int(bool(self.use_graph)),
-                                 int(bool(self.profile)))
+                                 int(bool(self.profile)), int(self.f32_update))
+
+    def replace(self, **kw):
+        d = dict(num_streams=self.num_streams, batch=self.batch, lookahead=self.lookahead,
+                 use_graph=self.use_graph, profile=self.profile, f32_update=self.f32_update)
+        d.update(kw)
+        return PlanOptions(**d)
 
 
 SEQUENTIAL = PlanOptions(num_streams=1, batch=False, lookahead=0, use_graph=False)

@@ -138,6 +138,16 @@ int dm_dgemm(char transa, char transb, int m, int n, int k, double alpha,
 int dm_sgemm(char transa, char transb, int m, int n, int k, float alpha,
              const float* A, int lda, const float* B, int ldb, float beta,
              float* C, int ldc, dm_stream_t s);
+/* FP32 GEMM on the BF16 matrix cores: every FP32 operand is split into
+ * `terms` == 6: three BF16 parts (products hh,hm,mh,hl,lh,mm) or
+ * `terms` == 3: two parts (hh,hm,mh), accumulated in FP32.  No reference
+ * counterpart: the low-precision off-diagonal update of the mixed-precision
+ * factorization (SYRK/GEMM of src/array/cholesky.jl:47,50 on FP32 tiles).
+ * Requires transa = 'T', transb = 'N', lda and ldb multiples of 4 and
+ * 16-byte aligned A and B. */
+int dm_sgemm_bf16x(int terms, char transa, char transb, int m, int n, int k,
+                   float alpha, const float* A, int lda, const float* B, int ldb,
+                   float beta, float* C, int ldc, dm_stream_t stream);
 
 /* BLAS trsv / gemv (vector right-hand sides), src/array/trsm.jl:17-51. */
 int dm_dtrsv(char uplo, char trans, char diag, int n, const double* A, int lda,
@@ -194,6 +204,11 @@ typedef struct dm_plan_opts {
                             eagerly, dm_dag_stats.graph reports 0) */
     int profile;         /* 1 (eager runs only): time every launch group
                             with HIP events on its own stream */
+    int f32_update;      /* FP32 GEMM/SYRK arithmetic: 0 FP32 matrix cores;
+                            6 or 3: BF16 matrix cores with 3- or 2-term
+                            operand splitting (6 or 3 products, FP32
+                            accumulation) where operands are k-contiguous
+                            (the upper variant), FP32 cores elsewhere */
 } dm_plan_opts;
 
 /* Per-launch-group timing of the last profiled run (dm_plan_opts.profile):

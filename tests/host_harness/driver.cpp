@@ -13,8 +13,8 @@ int main() {
   dm_tiled t{slab.data(), 1536, 1536, 256, 256, DM_F64};
   dm_tiled b{bslab.data(), 1536, 64, 256, 256, DM_F64};
   dm_tiled rag{slab.data(), 1000, 1000, 128, 128, DM_F64};
-  dm_plan_opts cfgs[] = {{1, 0, 0, 0, 0}, {2, 1, 1, 0, 0}, {4, 1, 0, 0, 0}, {3, 1, 2, 1, 0},
-                         {2, 1, 2, 1, 0}, {2, 1, 1, 0, 1}};
+  dm_plan_opts cfgs[] = {{1, 0, 0, 0, 0, 0}, {2, 1, 1, 0, 0, 0}, {4, 1, 0, 0, 0, 0},
+                         {3, 1, 2, 1, 0, 0}, {2, 1, 2, 1, 0, 0}, {2, 1, 1, 0, 1, 0}};
   for (auto& o : cfgs) {
     for (int uplo = 0; uplo < 2; ++uplo) {
       const char u = uplo ? 'L' : 'U';
@@ -46,6 +46,14 @@ int main() {
           dm_tiled bb = side == 'L' ? b : dm_tiled{bslab.data(), 64, 1536, 256, 256, DM_F64};
           if (dm_trsm_tiled(side, up, tr, 'N', 1.0, &t, &bb, &o, nullptr)) return 9;
         }
+  }
+  // FP32 tiles with the BF16-split update (upper: k-contiguous operands)
+  std::vector<float> fslab(1536 * 1536);
+  dm_tiled tf{fslab.data(), 1536, 1536, 256, 256, DM_F32};
+  for (int upd : {0, 3, 6}) {
+    dm_plan_opts o{2, 1, 1, 0, 0, upd};
+    for (char u : {'U', 'L'})
+      if (dm_cholesky_tiled(u, &tf, info, info + 1, &o, nullptr)) return 11;
   }
   dm_plan_cache_clear();
   printf("ok\n");

@@ -56,6 +56,16 @@ template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, i
 template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t) { ++g_launch; return 0; }
 template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t) { if (!ws) abort(); ++g_launch; return 0; }
 template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T* ws, hipStream_t) { if (!ws) { fprintf(stderr, "potrf without workspace\n"); abort(); } ++g_launch; return 0; }
+bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
+  return ta && !tb && !((uintptr_t)j.A & 15) && !((uintptr_t)j.B & 15) && !(j.lda & 3) &&
+         !(j.ldb & 3);
+}
+int gemm_split_launch(int terms, const GemmJob<float>*, int, const GemmJob<float>&, long,
+                      const int*, hipStream_t) {
+  if (terms != 3 && terms != 6) abort();
+  ++g_launch;
+  return 0;
+}
 int dgemv_launch(bool, int, int, double, const double*, int, const double*, int, double, double*, int, const int*, hipStream_t) { ++g_launch; return 0; }
 int fill_i32_launch(int*, int, size_t, hipStream_t) { ++g_launch; return 0; }
 int copy_i32_launch(int*, const int*, size_t, hipStream_t) { ++g_launch; return 0; }

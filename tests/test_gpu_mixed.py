@@ -20,14 +20,17 @@ def test_householder_device_matches_host(dm):
     assert np.allclose(np.linalg.eigvalsh((A + A.T) / 2)[[0, -1]], [1e-4, 1.0], rtol=1e-8)
 
 
-@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+@pytest.mark.parametrize("kappa,update", [(1e2, 0), (1e4, 0), (1e6, 0), (1e2, 6), (1e4, 6),
+                                          (1e6, 6), (1e2, 3), (1e4, 3), (1e6, 3)])
 @pytest.mark.parametrize("nrhs", [1, 32])
-def test_mixed_ir_reaches_fp64_backward_error(dm, kappa, nrhs):
+def test_mixed_ir_reaches_fp64_backward_error(dm, kappa, update, nrhs):
+    """update: FP32 trailing-update arithmetic (0 FP32 MFMA, 6/3 BF16 MFMA
+    with 3-/2-term operand splitting); refinement recovers FP64 accuracy."""
     from dagger_mi355x import mixed
     n, nb = 2048, 512
     D, spec = mixed.householder_generate(n, dm.Blocks(nb, nb), kappa, seed=11, nrefl=64)
     host = gen.HouseholderSPD(n, kappa, 11, nrefl=64)
-    F = mixed.MixedCholesky(D)
+    F = mixed.MixedCholesky(D, options=mixed._OPTS.replace(f32_update=update))
     xt = gen.randn(n * nrhs, 12).reshape(nrhs, n).T.copy()
     b = host.matvec(xt)
     DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
@@ -41,6 +44,23 @@ def test_mixed_ir_reaches_fp64_backward_error(dm, kappa, nrhs):
     ferr = np.abs(x - xt).max() / np.abs(xt).max()
     assert ferr <= 10 * kappa * 1e-13 + 1e-12
     assert info["iterations"] <= 30
+
+
+def test_split_update_factor_close_to_fp32(dm):
+    """The FP32 factor built with BF16x6 updates is as close to the FP64
+    factor as the FP32-MFMA one (both ~ kappa * 2^-24 relative)."""
+    from dagger_mi355x import mixed
+    n, nb = 2048, 512
+    A = gen.spd(n, 31)
+    U = np.linalg.cholesky(A).T
+    errs = {}
+    for update in (0, 6, 3):
+        DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+        F = mixed.MixedCholesky(DA, options=mixed._OPTS.replace(f32_update=update))
+        f = np.triu(F.F.collect().astype(np.float64))
+        errs[update] = np.linalg.norm(f - U) / np.linalg.norm(U)
+    assert errs[0] < 1e-6 and errs[6] < 1e-6, errs
+    assert errs[3] < 1e-4, errs
 
 
 def test_mixed_matches_plain_fp64_solution(dm):

@@ -214,6 +214,38 @@ def test_sgemm(dm, rng, ta, tb):
     assert relerr(dC.get(dm).astype(np.float64), ref) < 1e-5
 
 
+@pytest.mark.parametrize("terms,tol", [(6, 2e-6), (3, 1e-4)])
+@pytest.mark.parametrize("m,n,k,ld", [(512, 512, 512, 512), (200, 136, 77, 80), (129, 1, 33, 36),
+                                      (1024, 384, 1024, 1024)])
+def test_sgemm_bf16x(dm, rng, terms, tol, m, n, k, ld):
+    """FP32 GEMM on the BF16 matrix cores (operand splitting) against the
+    FP64 product of the same FP32 inputs: 3-term splitting (6 products) is
+    FP32-accurate, 2-term splitting (3 products) carries ~16 bits."""
+    A = np.zeros((ld, m), np.float32)
+    B = np.zeros((ld, n), np.float32)
+    A[:k] = rng.standard_normal((k, m))
+    B[:k] = rng.standard_normal((k, n))
+    Cm = rng.standard_normal((m, n)).astype(np.float32)
+    dA, dB, dC = Dev(dm, A), Dev(dm, B), Dev(dm, Cm)
+    dm._lib.call("dm_sgemm_bf16x", terms, b"T", b"N", m, n, k, C.c_float(-1.0), dA.p, ld, dB.p,
+                 ld, C.c_float(1.0), dC.p, m, None)
+    ref = Cm.astype(np.float64) - A[:k].T.astype(np.float64) @ B[:k].astype(np.float64)
+    assert relerr(dC.get(dm).astype(np.float64), ref) < tol * np.sqrt(k / 512 + 1)
+
+
+def test_sgemm_bf16x_rejects_other_shapes(dm):
+    A = Dev(dm, np.zeros((64, 64), np.float32))
+    with pytest.raises(dm.DMError):
+        dm._lib.call("dm_sgemm_bf16x", 6, b"N", b"N", 64, 64, 64, C.c_float(1.0), A.p, 64, A.p,
+                     64, C.c_float(0.0), A.p, 64, None)
+    with pytest.raises(dm.DMError):
+        dm._lib.call("dm_sgemm_bf16x", 4, b"T", b"N", 64, 64, 64, C.c_float(1.0), A.p, 64, A.p,
+                     64, C.c_float(0.0), A.p, 64, None)
+    with pytest.raises(dm.DMError):  # lda not a multiple of 4
+        dm._lib.call("dm_sgemm_bf16x", 6, b"T", b"N", 63, 63, 63, C.c_float(1.0), A.p, 63, A.p,
+                     63, C.c_float(0.0), A.p, 63, None)
+
+
 def test_randn_matches_host_generator(dm):
     n = 100000
     buf = dm.DeviceBuffer(8 * n)
