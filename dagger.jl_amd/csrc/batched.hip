@@ -272,27 +272,34 @@ __global__ __launch_bounds__(BNT) void batched_posv_kernel(T* __restrict__ Aall,
 
 using namespace dm;
 
-// Factor (and, unless factor_only, solve) `count` systems in one launch.
-// items/order: host arrays (copied to the device with the queue counter);
-// info_dev: count ints.  workgroups: persistent grid size (0 = 2 per CU).
-extern "C" int dm_batched_posv(int dtype, int count, void* A_base, void* B_base,
-                               const void* items_host, const int* order_host, int* info_dev,
-                               int factor_only, int workgroups, dm_stream_t s) {
+// Factor (and, unless factor_only, solve) the `count` systems listed in
+// order_host (indices into the num_items entries of items_host) in one
+// launch.  items/order: host arrays (copied to the device with the queue
+// counter); info_dev: num_items ints.  workgroups: persistent grid size
+// (0 = 2 per CU).
+extern "C" int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_base,
+                               const void* items_host, int count, const int* order_host,
+                               int* info_dev, int factor_only, int workgroups, dm_stream_t s) {
   if (dtype != DM_F64 && dtype != DM_F32) return arg_fail(1, "dtype must be DM_F64 or DM_F32");
-  if (count < 0) return arg_fail(2, "count < 0");
-  if (!items_host || !order_host || !info_dev) return arg_fail(5, "NULL items/order/info");
+  if (num_items < 0) return arg_fail(2, "num_items < 0");
+  if (count < 0) return arg_fail(6, "count < 0");
+  if (count > 0 && (!items_host || !order_host || !info_dev))
+    return arg_fail(5, "NULL items/order/info");
   int rc = ensure_init();
   if (rc) return rc;
   if (count == 0) return 0;
   hipStream_t st = stream_of(s);
   const BatchItem* items = (const BatchItem*)items_host;
-  for (int i = 0; i < count; ++i) {
+  for (int i = 0; i < num_items; ++i) {
     if (items[i].n < 0 || items[i].nrhs < 0 || items[i].nrhs > BMAXR)
       return arg_fail(5, "item with n < 0 or nrhs outside [0, 8]");
     if (items[i].panel < 1 || items[i].panel > BMAXP) return arg_fail(5, "panel outside [1, 64]");
   }
+  for (int i = 0; i < count; ++i)
+    if (order_host[i] < 0 || order_host[i] >= num_items)
+      return arg_fail(7, "order entry outside [0, num_items)");
   // device copy: [counter | pad][items][order]
-  const size_t ib = sizeof(BatchItem) * (size_t)count, ob = sizeof(int) * (size_t)count;
+  const size_t ib = sizeof(BatchItem) * (size_t)num_items, ob = sizeof(int) * (size_t)count;
   const size_t bytes = 256 + ib + ob;
   void* mem = nullptr;
   DM_HIP(hipMallocAsync(&mem, bytes, st));

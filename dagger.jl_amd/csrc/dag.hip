@@ -32,7 +32,6 @@
 #include <vector>
 #include "dm_internal.h"
 #include "dag.h"
-#include <type_traits>
 
 namespace dm {
 

@@ -9,6 +9,11 @@
 #include <string>
 #include "../../include/dagger_mi355x.h"
 
+// The dynamic LDS buffer of a kernel (sized by the launch).
+#ifndef DM_DYNAMIC_LDS
+#define DM_DYNAMIC_LDS(name) extern __shared__ unsigned char name[]
+#endif
+
 struct dm_stream_s { hipStream_t h; };
 struct dm_event_s { hipEvent_t h; };
 struct dm_graph_s { hipGraph_t g; hipGraphExec_t exec; };

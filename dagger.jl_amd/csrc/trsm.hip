@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
                                                    TrsmJob<T> single,
                                                    const int* __restrict__ abort) {
   typedef typename MfmaT<T>::v4 v4;
-  extern __shared__ unsigned char dm_trsm_smem[];
+  DM_DYNAMIC_LDS(dm_trsm_smem);
   if (abort && *abort) return;
   TrsmJob<T> job;
   int local;

@@ -147,9 +147,10 @@ class BatchedSystems:
         panels = panels or {}
         it = self.items(panels)
         order = self.order(subset)
-        call("dm_batched_posv", _DT[self.dtype], len(order), C.c_void_p(self.A.ptr),
-             C.c_void_p(self.B.ptr), C.cast(it, C.c_void_p), order.ctypes.data_as(C.c_void_p),
-             C.c_void_p(self.info.ptr), int(bool(factor_only)), int(workgroups), None)
+        call("dm_batched_posv", _DT[self.dtype], self.count, C.c_void_p(self.A.ptr),
+             C.c_void_p(self.B.ptr), C.cast(it, C.c_void_p), len(order),
+             order.ctypes.data_as(C.c_void_p), C.c_void_p(self.info.ptr),
+             int(bool(factor_only)), int(workgroups), None)
         return self
 
     def infos(self):

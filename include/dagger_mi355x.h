@@ -339,15 +339,17 @@ int dm_forget_matrix(const void* base);
  * A_m = A_base + a_off (n x n, column-major, ld = n), B_m = B_base + b_off
  * (n x nrhs, ld = n, nrhs <= 8), panel width P in [1, 64] (autotuned per size
  * bucket by the host).  Factored A = U^T U in place (upper triangle) and,
- * unless factor_only, B overwritten with the solution.  info_dev[m]: LAPACK
- * info of system m.  order_host: processing order (largest first is best).
+ * unless factor_only, B overwritten with the solution.  items_host: the
+ * num_items systems; order_host: the `count` systems to process, as indices
+ * into items_host (each in [0, num_items), processing order: largest first
+ * is best).  info_dev[m] (num_items ints): LAPACK info of processed system m.
  * workgroups: persistent grid (0 = two per CU). */
 typedef struct dm_batch_item {
     int64_t a_off, b_off;
     int n, nrhs, panel, pad_;
 } dm_batch_item;
-int dm_batched_posv(int dtype, int count, void* A_base, void* B_base,
-                    const void* items_host, const int* order_host, int* info_dev,
+int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_base,
+                    const void* items_host, int count, const int* order_host, int* info_dev,
                     int factor_only, int workgroups, dm_stream_t s);
 
 /* ---------------------------------------------------------- generators

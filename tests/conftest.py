@@ -30,6 +30,9 @@ def dm():
     """The product package, bound to device 0.  GPU tests fail (not skip)
     when the native library does not load."""
     import dagger_mi355x
+    emu = os.environ.get("DM_TEST_EMU_LIB")
+    if emu:  # CPU emulation build (tests/cpu_emu), for host-side bounds checks only
+        dagger_mi355x._lib.load(emu)
     dagger_mi355x.load_library()
     dagger_mi355x.init(0)
     return dagger_mi355x

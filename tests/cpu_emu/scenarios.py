@@ -1,0 +1,346 @@
+"""Every entry point of the library at small shapes, run on the CPU emulation
+build (tests/cpu_emu) under AddressSanitizer: an out-of-bounds access in any
+kernel or host path aborts with a report, a barrier or wave op that not all
+work-items reach aborts as a deadlock.  Results are also checked against
+numpy so that the emulated MFMA lane layouts stay honest.
+
+Run by tests/test_cpu_emulation.py as
+    LD_PRELOAD=<asan runtime> DM_TEST_EMU_LIB=<emu lib> python scenarios.py [group ...]
+"""
+import ctypes as C
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "dagger.jl_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import dagger_mi355x as dm  # noqa: E402
+from oracle import dagger_tiled as O  # noqa: E402
+from oracle import gen  # noqa: E402
+
+dm._lib.load(os.environ["DM_TEST_EMU_LIB"])
+dm.init(0)
+call = dm._lib.call
+
+
+class Dev:
+    def __init__(self, a):
+        self.a = np.asfortranarray(a)
+        self.buf = dm.DeviceBuffer(max(self.a.nbytes, 8))
+        call("dm_memcpy_h2d", C.c_void_p(self.buf.ptr), self.a.ctypes.data_as(C.c_void_p),
+             C.c_size_t(self.a.nbytes), None)
+
+    @property
+    def p(self):
+        return C.c_void_p(self.buf.ptr)
+
+    def get(self):
+        out = np.empty_like(self.a, order="F")
+        call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.buf.ptr),
+             C.c_size_t(out.nbytes), None)
+        return out
+
+
+def relerr(x, ref):
+    return float(np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def check(cond, what):
+    if not cond:
+        raise AssertionError(what)
+
+
+# ---------------------------------------------------------------- tile ops
+def tiles():
+    rng = np.random.default_rng(1)
+    for ta in "NT":
+        for tb in "NT":
+            for (m, n, k) in [(200, 136, 77), (1, 1, 1), (64, 33, 0)]:
+                A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+                B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+                Cm = rng.standard_normal((m, n))
+                dA, dB, dC = Dev(A), Dev(B), Dev(Cm)
+                call("dm_dgemm", ta.encode(), tb.encode(), m, n, k, -1.0, dA.p,
+                     max(A.shape[0], 1), dB.p, max(B.shape[0], 1), 0.5, dC.p, m, None)
+                ref = 0.5 * Cm - (A.T if ta == "T" else A) @ (B.T if tb == "T" else B)
+                check(relerr(dC.get(), ref) < 1e-13, ("dgemm", ta, tb, m, n, k))
+    # 128x128 workgroup tiles (>= 192 of them), odd leading dimensions
+    m, n, k = 2048, 1536, 4
+    A, B = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+    dA, dB, dC = Dev(A), Dev(B), Dev(np.zeros((m, n)))
+    call("dm_dgemm", b"N", b"N", m, n, k, 1.0, dA.p, m, dB.p, k, 0.0, dC.p, m, None)
+    check(relerr(dC.get(), A @ B) < 1e-13, "dgemm big tiles")
+    m, n, k, ld = 101, 67, 55, 103
+    A, B = rng.standard_normal((ld, k)), rng.standard_normal((ld, n))
+    dA, dB, dC = Dev(A), Dev(B), Dev(np.zeros((ld, n)))
+    call("dm_dgemm", b"N", b"N", m, n, k, 1.0, dA.p, ld, dB.p, ld, 0.0, dC.p, ld, None)
+    got = dC.get()
+    check(relerr(got[:m], A[:m] @ B[:k]) < 1e-13 and np.all(got[m:] == 0), "dgemm odd ld")
+    for uplo in "UL":
+        for trans in "NT":
+            for (n, k) in [(130, 70), (5, 3)]:
+                A = rng.standard_normal((k, n) if trans == "T" else (n, k))
+                Cm = rng.standard_normal((n, n))
+                dA, dC = Dev(A), Dev(Cm)
+                call("dm_dsyrk", uplo.encode(), trans.encode(), n, k, -1.0, dA.p, A.shape[0],
+                     1.0, dC.p, n, None)
+                ref = Cm.copy(order="F")
+                O.tile_syrk(uplo, trans, -1.0, np.asfortranarray(A), 1.0, ref)
+                tri = np.triu if uplo == "U" else np.tril
+                check(relerr(tri(dC.get()), tri(ref)) < 1e-13, ("dsyrk", uplo, trans, n, k))
+    for side in "LR":
+        for uplo in "UL":
+            for trans in "NT":
+                for diag in "NU":
+                    m, n = (70, 30) if side == "L" else (30, 70)
+                    ka = m if side == "L" else n
+                    A = 0.05 * rng.standard_normal((ka, ka)) + 2 * np.eye(ka)
+                    B = rng.standard_normal((m, n))
+                    dA, dB = Dev(A), Dev(B)
+                    call("dm_dtrsm", side.encode(), uplo.encode(), trans.encode(),
+                         diag.encode(), m, n, 0.75, dA.p, ka, dB.p, m, None)
+                    ref = B.copy(order="F")
+                    O.tile_trsm(side, uplo, trans, diag, 0.75, np.asfortranarray(A), ref)
+                    check(relerr(dB.get(), ref) < 1e-12, ("dtrsm", side, uplo, trans, diag))
+    # solves longer than the LDS-resident limit (host recursion)
+    for side, uplo, trans in [("L", "U", "T"), ("R", "L", "T")]:
+        ka, nv = 1100, 3
+        F, _ = O.cholesky(gen.spd(ka, 5), ka, uplo)
+        m, n = (ka, nv) if side == "L" else (nv, ka)
+        B = rng.standard_normal((m, n))
+        dA, dB = Dev(F), Dev(B)
+        call("dm_dtrsm", side.encode(), uplo.encode(), trans.encode(), b"N", m, n, 1.0, dA.p,
+             ka, dB.p, m, None)
+        ref = B.copy(order="F")
+        O.tile_trsm(side, uplo, trans, "N", 1.0, np.asfortranarray(F), ref)
+        check(relerr(dB.get(), ref) < 1e-12, ("dtrsm long", side, uplo, trans))
+    for uplo in "UL":
+        for n in (1, 64, 100, 200):
+            A = gen.spd(n, 900 + n)
+            dA = Dev(A)
+            info = dm.DeviceInt(1, value=-7)
+            call("dm_dpotrf", uplo.encode(), n, dA.p, n, C.c_void_p(info.ptr), None)
+            ref, rinfo = O.cholesky(A, n, uplo)
+            check(int(info.get()[0]) == 0 and O.rel_factor_diff(dA.get(), ref, uplo) < 1e-12,
+                  ("dpotrf", uplo, n))
+    A = gen.spd(150, 5)
+    A[130, 130] = -1.0
+    dA, info = Dev(A), dm.DeviceInt(1)
+    call("dm_dpotrf", b"U", 150, dA.p, 150, C.c_void_p(info.ptr), None)
+    check(int(info.get()[0]) == 131, "dpotrf info")
+    for trans in "NT":
+        m, n = 300, 100
+        A = rng.standard_normal((m, n))
+        x = rng.standard_normal(n if trans == "N" else m)
+        y = rng.standard_normal(m if trans == "N" else n)
+        dA, dx, dy = Dev(A), Dev(x), Dev(y)
+        call("dm_dgemv", trans.encode(), m, n, -1.0, dA.p, m, dx.p, 1, 2.0, dy.p, 1, None)
+        ref = 2.0 * y - (A.T @ x if trans == "T" else A @ x)
+        check(relerr(dy.get(), ref) < 1e-13, ("dgemv", trans))
+    for uplo in "UL":
+        for trans in "NT":
+            n = 150
+            F, _ = O.cholesky(gen.spd(n, 3), n, uplo)
+            x = gen.randn(n, 4)
+            dA, dx = Dev(F), Dev(x)
+            call("dm_dtrsv", uplo.encode(), trans.encode(), b"N", n, dA.p, n, dx.p, 1, None)
+            ref = x.copy()
+            O.tile_trsv(uplo, trans, "N", np.asfortranarray(F), ref)
+            check(relerr(dx.get(), ref) < 1e-12, ("dtrsv", uplo, trans))
+    for ta, tb in [("T", "N"), ("N", "T")]:
+        m = n = k = 96
+        A = rng.standard_normal((m, k)).astype(np.float32)
+        B = rng.standard_normal((k, n)).astype(np.float32)
+        Cm = rng.standard_normal((m, n)).astype(np.float32)
+        dA, dB, dC = Dev(A), Dev(B), Dev(Cm)
+        call("dm_sgemm", ta.encode(), tb.encode(), m, n, k, C.c_float(1.0), dA.p, m, dB.p, k,
+             C.c_float(1.0), dC.p, m, None)
+        opA = A.T if ta == "T" else A
+        opB = B.T if tb == "T" else B
+        ref = opA.astype(np.float64) @ opB.astype(np.float64) + Cm
+        check(relerr(dC.get().astype(np.float64), ref) < 1e-5, ("sgemm", ta, tb))
+    for terms, tol in [(6, 2e-6), (3, 1e-4)]:
+        for (m, n, k, ld) in [(200, 136, 77, 80), (129, 1, 33, 36), (64, 64, 64, 64)]:
+            A = np.zeros((ld, m), np.float32)
+            B = np.zeros((ld, n), np.float32)
+            A[:k] = rng.standard_normal((k, m))
+            B[:k] = rng.standard_normal((k, n))
+            Cm = rng.standard_normal((m, n)).astype(np.float32)
+            dA, dB, dC = Dev(A), Dev(B), Dev(Cm)
+            call("dm_sgemm_bf16x", terms, b"T", b"N", m, n, k, C.c_float(-1.0), dA.p, ld, dB.p,
+                 ld, C.c_float(1.0), dC.p, m, None)
+            ref = Cm.astype(np.float64) - A[:k].T.astype(np.float64) @ B[:k].astype(np.float64)
+            check(relerr(dC.get().astype(np.float64), ref) < tol, ("bf16x", terms, m, n, k))
+    n = 5000
+    buf = dm.DeviceBuffer(8 * n)
+    call("dm_fill_randn_d", C.c_void_p(buf.ptr), C.c_size_t(n), C.c_uint64(42), C.c_uint64(5),
+         None)
+    out = np.empty(n)
+    call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
+         C.c_size_t(8 * n), None)
+    check(np.max(np.abs(out - gen.randn(n, 42, offset=5))) < 1e-13, "randn")
+
+
+# ---------------------------------------------------------------- tiled path
+def tiled():
+    n, nb = 300, 64  # ragged: 4 x 64 + 44
+    A = gen.spd(n, 6)
+    outs = {}
+    for uplo in "UL":
+        ref, rinfo = O.cholesky(A, nb, uplo)
+        for opts in (dm.SEQUENTIAL,
+                     dm.PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=False),
+                     dm.PlanOptions(num_streams=3, batch=True, lookahead=2, use_graph=False),
+                     dm.PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True)):
+            DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+            dm.cholesky_(DA, uplo=uplo, options=opts)
+            f = DA.collect()
+            check(O.rel_factor_diff(f, ref, uplo) < 1e-12, ("cholesky", uplo))
+            outs.setdefault(uplo, f)
+            check(np.array_equal(f, outs[uplo]), ("schedules differ", uplo))
+        DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+        Fc = dm.cholesky_(DA, uplo=uplo)
+        for nrhs in (5, 1):
+            b = gen.randn(n * nrhs, 8).reshape(nrhs, n).T.copy()
+            DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
+            dm.ldiv_(Fc, DB)
+            check(O.backward_error(A, DB.collect(), b) < 10 * n * np.finfo(float).eps,
+                  ("ldiv", uplo, nrhs))
+        bv = gen.randn(n, 9)
+        Dv = dm.DArray.from_array(bv, dm.Blocks(nb))
+        dm.ldiv_(Fc, Dv)
+        check(O.backward_error(A, Dv.collect().reshape(-1, 1), bv.reshape(-1, 1)) <
+              10 * n * np.finfo(float).eps, ("ldiv vector", uplo))
+    # non-PD: tile-local info
+    Abad = A.copy()
+    Abad[200, 200] = -1.0
+    DA = dm.DArray.from_array(Abad, dm.Blocks(nb, nb))
+    try:
+        dm.cholesky_(DA)
+        raise AssertionError("non-PD not reported")
+    except dm.PosDefException:
+        pass
+    # tiled trsm branches and tiled gemm
+    F, _ = O.cholesky(A, nb, "U")
+    F = np.triu(F)
+    rng = np.random.default_rng(3)
+    for side, uplo, trans in [("L", "U", "T"), ("L", "U", "N"), ("L", "L", "N"), ("L", "L", "T"),
+                              ("R", "U", "N"), ("R", "L", "T")]:
+        T = F if uplo == "U" else F.T.copy()
+        shape = (n, 7) if side == "L" else (7, n)
+        B = rng.standard_normal(shape)
+        DT = dm.DArray.from_array(T, dm.Blocks(nb, nb))
+        DB = dm.DArray.from_array(B, dm.Blocks(nb, nb))
+        dm.trsm_(side, uplo, trans, "N", 0.5, DT, DB)
+        # Dagger's right-transposed branches apply alpha at every block
+        # step (src/array/trsm.jl:157-163,177-183): compare with the tiled
+        # restatement, which follows the reference loop nest.
+        ref = O.trsm(side, uplo, trans, "N", 0.5, T, B, nb)
+        check(relerr(DB.collect(), ref) < 1e-11, ("trsm tiled", side, uplo, trans))
+    X = rng.standard_normal((n, 9))
+    Y = rng.standard_normal((n, 9))
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    DX = dm.DArray.from_array(X, dm.Blocks(nb, nb))
+    DY = dm.DArray.from_array(Y, dm.Blocks(nb, nb))
+    ta, tx, ty = DA.tiled(), DX.tiled(), DY.tiled()
+    o = dm.PlanOptions().c()
+    call("dm_gemm_tiled", b"N", b"N", -1.0, C.byref(ta), C.byref(tx), 1.0, C.byref(ty),
+         C.byref(o), None)
+    check(relerr(DY.collect(), Y - A @ X) < 1e-13, "gemm tiled")
+    check(dm.ishermitian(dm.DArray.from_array(A, dm.Blocks(nb, nb))), "ishermitian")
+    Ab = A.copy()
+    Ab[5, 250] += 1e-3
+    check(not dm.ishermitian(dm.DArray.from_array(Ab, dm.Blocks(nb, nb))), "not hermitian")
+    # re-blocking (Blocks mismatch) and FP32 tiles with every update arithmetic
+    DA = dm.DArray.from_array(A, dm.Blocks(96, 64))
+    dm.cholesky_(DA)
+    check(O.rel_factor_diff(DA.collect(), O.cholesky(A, nb, "U")[0], "U") < 1e-12, "reblock")
+    for upd in (0, 6, 3):
+        F32 = dm.DArray.from_array(A.astype(np.float32), dm.Blocks(nb, nb))
+        info = dm.DeviceInt(2)
+        t = F32.tiled()
+        opts = dm.PlanOptions(f32_update=upd).c()
+        call("dm_cholesky_tiled", b"U", C.byref(t), C.c_void_p(info.ptr),
+             C.c_void_p(info.ptr + 4), C.byref(opts), None)
+        f = np.triu(F32.collect().astype(np.float64))
+        U = np.linalg.cholesky(A).T
+        check(relerr(f, U) < (1e-5 if upd == 3 else 1e-6), ("fp32 cholesky", upd))
+
+
+# ---------------------------------------------------------------- mixed
+def mixed():
+    from dagger_mi355x import mixed as M
+    n, nb = 256, 64
+    for kappa, upd in [(1e4, 0), (1e4, 6), (1e6, 3)]:
+        D, spec = M.householder_generate(n, dm.Blocks(nb, nb), kappa, seed=11, nrefl=16)
+        host = gen.HouseholderSPD(n, kappa, 11, nrefl=16)
+        check(np.allclose(D.collect(), host.dense(), rtol=0, atol=1e-12), "householder")
+        F = M.MixedCholesky(D, options=M._OPTS.replace(f32_update=upd))
+        xt = gen.randn(n * 3, 12).reshape(3, n).T.copy()
+        b = host.matvec(xt)
+        X, info = F.solve(dm.DArray.from_array(b, dm.Blocks(nb, nb)), tol=1e-13, max_iter=30)
+        check(O.backward_error(host.dense(), X.collect(), b) <= 1e-13, ("mixed", kappa, upd))
+
+
+# ---------------------------------------------------------------- batched
+def batched():
+    from dagger_mi355x import batched as Bt
+    for dtype in (np.float64, np.float32):
+        sizes, nrhs = Bt.log_uniform_sizes(8, 32, 120, seed=5)
+        sizes[:3] = [32, 33, 120]
+        bs = Bt.BatchedSystems(sizes, nrhs, dtype).generate(seed=100)
+        A0s = [bs.get_system(m) for m in range(bs.count)]
+        bs.posv(panels={})
+        check(np.all(bs.infos() == 0), "batched info")
+        for m in range(bs.count):
+            A0, B0 = A0s[m]
+            F, X = bs.get_system(m)
+            if B0.shape[1]:
+                be = O.backward_error(A0.astype(np.float64), X.astype(np.float64),
+                                      B0.astype(np.float64))
+                check(be <= 10 * A0.shape[0] * np.finfo(dtype).eps, ("batched", dtype, m))
+    sizes, nrhs = Bt.log_uniform_sizes(6, 32, 100, seed=6)
+    bs = Bt.BatchedSystems(sizes, nrhs, np.float64).generate(seed=7)
+    snap = bs.snapshot()
+    for P in Bt.PANELS:
+        bs.restore(snap)
+        bs.posv(panels={b: P for b in range(16)})
+    bs.restore(snap)
+    Bt.autotune(bs, sample=3, reps=1)
+    try:  # order entries outside the item table are rejected
+        order = np.array([0, 99], np.int32)
+        it = bs.items({})
+        call("dm_batched_posv", dm._lib.DM_F64, bs.count, C.c_void_p(bs.A.ptr),
+             C.c_void_p(bs.B.ptr), C.cast(it, C.c_void_p), 2, order.ctypes.data_as(C.c_void_p),
+             C.c_void_p(bs.info.ptr), 0, 0, None)
+        raise AssertionError("bad order accepted")
+    except dm.DMError:
+        pass
+
+
+# ---------------------------------------------------------------- serving
+def serving():
+    from dagger_mi355x import serving as S
+    trace = S.make_trace(rate=400.0, count=6, seed=3, n_lo=64, n_hi=200, k_lo=1e2, k_hi=1e4,
+                         reuse=0.3, tols=(1e-8, 1e-13), nrhs_max=4)
+    srv = S.Server(workers=2, budget_bytes=1e8)
+    stats = srv.run(trace)
+    srv.close()
+    check(stats["completed"] == len(trace), "serving completed")
+    for r in trace:
+        check(r.berr <= r.tol, ("serving tol", r.rid))
+
+
+GROUPS = dict(tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving)
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(GROUPS)
+    for name in names:
+        t = time.time()
+        GROUPS[name]()
+        print(f"{name}: ok ({time.time() - t:.1f}s)", flush=True)
+    print("ALL OK", flush=True)
