@@ -42,11 +42,21 @@ struct Group {
   // batched GEMM/SYRK
   bool ta = false, tb = false, vec = false;
   GemmCfgId cfg{0, 128, 128};
-  // batched TRSM
+  // batched TRSM: inverse blocks per distinct triangle, then the recursive
+  // solve's launches (leaf solves and GEMM updates) in order
   TrsmShape shape{};
   bool batched_trsm = false;
-  int maxL = 0;
   std::vector<InvSrc> inv_src;
+  struct Step {
+    bool leaf = true;
+    int njobs = 0, maxL = 0;
+    long total_wg = 0;
+    bool ta = false, tb = false, vec = false;  // GEMM updates
+    GemmCfgId cfg{0, 128, 128};
+    size_t table_off = 0;
+    void* table_dev = nullptr;
+  };
+  std::vector<Step> tsteps;
   // common
   double flops = 0;  // algorithmic flops of the group's tasks
   int njobs = 0;

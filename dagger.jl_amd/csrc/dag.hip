@@ -342,21 +342,10 @@ int Dag::build_device_tables() {
       G.shape = trsm_shape(t0.side, t0.uplo, t0.transa, t0.diag);
       auto fill = [&](auto* dummy) -> int {
         typedef std::remove_pointer_t<decltype(dummy)> T;
-        // batched only when every job fits the LDS-resident kernel
-        int maxL = 0;
-        bool fits = true;
-        for (int ti : G.tasks) {
-          const dm_task& t = tasks[ti].t;
-          const int L = G.shape.side_l ? t.m : t.n;
-          maxL = std::max(maxL, L);
-          if (L > trsm_max_len<T>()) fits = false;
-        }
-        G.batched_trsm = fits;
-        if (!fits) return 0;
-        // one inverse-block buffer per distinct (A, L)
+        // one inverse-block buffer per distinct (A, L); job inv pointers
+        // hold offsets into the inverse area until the allocation exists
         std::map<std::pair<const void*, int>, int> invidx;
         std::vector<TrsmJob<T>> jobs;
-        long wg = 0;
         for (int ti : G.tasks) {
           const dm_task& t = tasks[ti].t;
           const int L = G.shape.side_l ? t.m : t.n;
@@ -384,17 +373,39 @@ int Dag::build_device_tables() {
           j.inv = (const T*)(uintptr_t)G.inv_src[ii].off;  // patched after allocation
           j.L = L; j.NV = NV; j.lda = t.lda; j.ldb = t.ldc;
           j.alpha = (T)t.alpha;
-          j.wg_begin = (int)wg;
-          wg += trsm_wgs(NV);
           jobs.push_back(j);
         }
-        G.njobs = (int)jobs.size();
-        G.total_wg = wg;
-        G.maxL = maxL;
-        align(64);
-        G.table_off = blob.size();
-        const unsigned char* p = (const unsigned char*)jobs.data();
-        blob.insert(blob.end(), p, p + jobs.size() * sizeof(TrsmJob<T>));
+        G.batched_trsm = true;
+        std::vector<TrsmStep<T>> steps;
+        trsm_plan_steps<T>(G.shape, jobs, steps);
+        for (TrsmStep<T>& st : steps) {
+          Group::Step rec;
+          rec.leaf = st.leaf;
+          align(64);
+          rec.table_off = blob.size();
+          if (st.leaf) {
+            rec.njobs = (int)st.tjobs.size();
+            rec.maxL = st.maxL;
+            rec.total_wg = st.total_wg;
+            const unsigned char* p = (const unsigned char*)st.tjobs.data();
+            blob.insert(blob.end(), p, p + st.tjobs.size() * sizeof(TrsmJob<T>));
+          } else {
+            int maxM = 0, maxN = 0, maxK = 0;
+            for (const GemmJob<T>& g : st.gjobs) {
+              maxM = std::max(maxM, g.M); maxN = std::max(maxN, g.N); maxK = std::max(maxK, g.K);
+            }
+            rec.njobs = (int)st.gjobs.size();
+            rec.ta = st.ta;
+            rec.tb = st.tb;
+            rec.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, rec.njobs);
+            rec.total_wg = gemm_prepare_jobs<T>(st.gjobs.data(), rec.njobs, rec.cfg);
+            rec.vec = true;
+            for (const GemmJob<T>& g : st.gjobs) rec.vec = rec.vec && gemm_job_vec_ok<T>(st.ta, st.tb, g);
+            const unsigned char* p = (const unsigned char*)st.gjobs.data();
+            blob.insert(blob.end(), p, p + st.gjobs.size() * sizeof(GemmJob<T>));
+          }
+          G.tsteps.push_back(rec);
+        }
         return 0;
       };
       int rc = G.dtype == DM_F64 ? fill((double*)nullptr) : fill((float*)nullptr);
@@ -423,22 +434,26 @@ int Dag::build_device_tables() {
   DM_HIP(hipMalloc(&dev_mem, total));
   abort_dev = (int*)dev_mem;
   unsigned char* base = (unsigned char*)dev_mem;
-  // patch inverse pointers inside TRSM tables
+  // patch inverse pointers inside the leaf tables of TRSM groups
   for (Group& G : groups) {
-    if (G.op != DM_OP_TRSM || !G.batched_trsm || !G.njobs) continue;
-    if (G.dtype == DM_F64) {
-      auto* jobs = (TrsmJob<double>*)(blob.data() + G.table_off);
-      for (int i = 0; i < G.njobs; ++i)
-        jobs[i].inv = (const double*)(base + 256 + (uintptr_t)jobs[i].inv);
-    } else {
-      auto* jobs = (TrsmJob<float>*)(blob.data() + G.table_off);
-      for (int i = 0; i < G.njobs; ++i)
-        jobs[i].inv = (const float*)(base + 256 + (uintptr_t)jobs[i].inv);
+    if (G.op != DM_OP_TRSM) continue;
+    for (Group::Step& st : G.tsteps) {
+      if (!st.leaf) continue;
+      if (G.dtype == DM_F64) {
+        auto* jobs = (TrsmJob<double>*)(blob.data() + st.table_off);
+        for (int i = 0; i < st.njobs; ++i)
+          jobs[i].inv = (const double*)(base + 256 + (uintptr_t)jobs[i].inv);
+      } else {
+        auto* jobs = (TrsmJob<float>*)(blob.data() + st.table_off);
+        for (int i = 0; i < st.njobs; ++i)
+          jobs[i].inv = (const float*)(base + 256 + (uintptr_t)jobs[i].inv);
+      }
     }
   }
   for (Group& G : groups) {
     G.table_dev = base + tab_base + G.table_off;
     for (InvSrc& s : G.inv_src) s.dev = base + 256 + s.off;
+    for (Group::Step& st : G.tsteps) st.table_dev = base + tab_base + st.table_off;
   }
   if (!blob.empty()) DM_HIP(hipMemcpy(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice));
   DM_HIP(hipMemset(dev_mem, 0, 256));
@@ -459,11 +474,21 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       int rc = trinv_blocks_launch<T>(G.shape, (const T*)src.A, src.lda, src.L, (T*)src.dev, ab, s);
       if (rc) return rc;
     }
-    if (G.njobs == 0) return 0;
-    TrsmJob<T> dummy{};
-    dummy.L = G.maxL;
-    return trsm_launch<T>(G.shape, (const TrsmJob<T>*)G.table_dev, G.njobs, dummy, G.total_wg, ab,
-                          s);
+    for (const Group::Step& st : G.tsteps) {
+      int rc;
+      if (st.leaf) {
+        TrsmJob<T> dummy{};
+        dummy.L = st.maxL;
+        rc = trsm_launch<T>(G.shape, (const TrsmJob<T>*)st.table_dev, st.njobs, dummy, st.total_wg,
+                            ab, s);
+      } else {
+        GemmJob<T> dummy{};
+        rc = gemm_launch<T>(st.ta, st.tb, st.vec, st.cfg, (const GemmJob<T>*)st.table_dev, st.njobs,
+                            dummy, st.total_wg, ab, s);
+      }
+      if (rc) return rc;
+    }
+    return 0;
   }
   // Unbatched ops: one call per task in submission order.
   for (int ti : G.tasks) {
@@ -581,7 +606,7 @@ int Dag::launches() const {
   int n = 0;
   for (const Group& G : groups) {
     if (G.op == DM_OP_GEMM) n += G.njobs > 0 ? 1 : 0;
-    else if (G.op == DM_OP_TRSM && G.batched_trsm) n += (int)G.inv_src.size() + (G.njobs > 0);
+    else if (G.op == DM_OP_TRSM && G.batched_trsm) n += (int)(G.inv_src.size() + G.tsteps.size());
     else n += (int)G.tasks.size();
   }
   return n;

@@ -7,11 +7,17 @@
 #include <cstdint>
 #include <functional>
 #include <string>
+#include <vector>
 #include "../../include/dagger_mi355x.h"
 
 // The dynamic LDS buffer of a kernel (sized by the launch).
 #ifndef DM_DYNAMIC_LDS
 #define DM_DYNAMIC_LDS(name) extern __shared__ unsigned char name[]
+#endif
+
+// Hides a register value from the optimizer (no instruction is emitted).
+#ifndef DM_OPAQUE
+#define DM_OPAQUE(x) asm volatile("" : "+v"(x))
 #endif
 
 struct dm_stream_s { hipStream_t h; };
@@ -139,29 +145,48 @@ struct TrsmShape {
 };
 TrsmShape trsm_shape(char side, char uplo, char transa, char diag);
 
-constexpr int TRSM_BS = 32;  // diagonal block inverted up front
-template <typename T>
-constexpr int trsm_max_len() { return sizeof(T) == 8 ? 1024 : 2048; }
+constexpr int TRSM_BS = 32;     // diagonal block inverted up front
+constexpr int TRSM_LEAF = 256;  // longest solve held in LDS (longer ones recurse)
+constexpr int TRSM_VPW = 64;    // vectors per workgroup of the leaf kernel (16 per wave)
 // Workspace (elements) for any trsm_full / potrf_full call: the inverted
-// diagonal blocks of one LDS-resident solve (longer solves are split).
+// diagonal blocks of one leaf solve.
 template <typename T>
 constexpr size_t trsm_ws_elems() {
-  return (size_t)(trsm_max_len<T>() / TRSM_BS) * TRSM_BS * TRSM_BS;
+  return (size_t)(TRSM_LEAF / TRSM_BS) * TRSM_BS * TRSM_BS;
 }
 
 // Inverts the 32x32 diagonal blocks of Tm into inv (ceil(L/32)*1024 elems).
 template <typename T>
 int trinv_blocks_launch(const TrsmShape& sh, const T* A, int lda, int L, T* inv,
                         const int* abort, hipStream_t s);
-// Batched / single blocked solve (L <= trsm_max_len<T>()).
+// Batched / single leaf solve (L <= TRSM_LEAF; jobs' inv point at the
+// inverted blocks of their own triangle).  single.L: longest L of the launch.
 template <typename T>
 int trsm_launch(const TrsmShape& sh, const TrsmJob<T>* jobs_dev, int njobs,
                 const TrsmJob<T>& single, long total_wg, const int* abort,
                 hipStream_t s);
 int trsm_init_attributes();
-inline int trsm_wgs(int NV) { return (NV + 15) / 16; }
+inline int trsm_wgs(int NV) { return (NV + TRSM_VPW - 1) / TRSM_VPW; }
 
-// Full BLAS trsm (any size; host recursion above trsm_max_len).
+// Recursive batched solve: the launches (leaf solves and GEMM updates, in
+// issue order) that solve every job of `jobs` (any lengths).  Split points
+// are multiples of TRSM_BS from the top of each triangle, so a job's inverse
+// blocks (computed once over its full triangle) serve every leaf.
+template <typename T>
+struct TrsmStep {
+  bool leaf = true;
+  std::vector<TrsmJob<T>> tjobs;  // leaf solves (wg_begin filled)
+  std::vector<GemmJob<T>> gjobs;  // update C = alpha_j V2 - Tm21 X1
+  bool ta = false, tb = false;    // update operand transposes
+  int maxL = 0;
+  long total_wg = 0;
+};
+template <typename T>
+void trsm_plan_steps(const TrsmShape& sh, const std::vector<TrsmJob<T>>& jobs,
+                     std::vector<TrsmStep<T>>& steps);
+
+// Full BLAS trsm (any size; host recursion above TRSM_LEAF, inverse blocks
+// formed per leaf in ws).
 template <typename T>
 int trsm_full(char side, char uplo, char transa, char diag, int m, int n,
               T alpha, const T* A, int lda, T* B, int ldb, T* ws,

@@ -4,125 +4,177 @@
 // 2-18, spawned at :39/:84).  info follows LAPACK: 0, or k > 0 when the
 // leading minor of order k is not positive definite (pivot <= 0 or NaN).
 //
-// Structure: recursive 2x2 splitting on the host down to 64-column leaves.
-//   U: potrf(A11); A12 := U11^-T A12 (trsm); A22 -= A12^T A12 (syrk); potrf(A22)
-//   L: potrf(A11); A21 := A21 L11^-T (trsm); A22 -= A21 A21^T (syrk); potrf(A22)
-// The off-diagonal work runs on the MFMA trsm/syrk kernels.  A leaf is one
-// wave: the 64x64 block sits in LDS, lane c owns column c (U) or row c (L),
-// and each step is a square root, a row scale and a rank-1 update with two
-// single-wave barriers.  Every launch first checks the abort flag, so after
-// a failed pivot the remaining launches are no-ops.
+// Structure: blocked right-looking factorization with 64-wide panels, two
+// launches per panel:
+//  1. potrf_panel_kernel (one workgroup of 8 waves): wave 0 factors the
+//     64 x 64 diagonal block in registers (lane c owns column c of U, or
+//     row c of L = column c of L^T), broadcasting the current row through
+//     LDS -- no workgroup barriers inside the factorization; then all waves
+//     solve the panel row U12 := U11^-T A12 (U) / A21 := A21 L11^-T (L), one
+//     column (row) per lane against the factored block held in LDS;
+//  2. the trailing update A22 -= U12^T U12 (A21 A21^T for L) on the MFMA
+//     SYRK kernel.
+// Every launch first checks the abort flag (and info), so after a failed
+// pivot the remaining launches are no-ops.
 #include "dm_internal.h"
 
 namespace dm {
 
-constexpr int POTRF_LEAF = 64;
+constexpr int PANEL = 64;      // panel width: one column per lane of a wave
+constexpr int PANEL_NT = 512;  // 8 waves solve the panel row
 
-// Factor an n x n (n <= 64) block in place.  The unreferenced triangle is
-// not touched.  On failure writes info = info_base + j + 1 (local) and, if
-// abort != nullptr, *abort = global_base + j + 1.
+// Orders this wave's LDS writes before its later LDS reads (the lanes of one
+// wave exchange a row through LDS without a workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Factor the kb x kb diagonal block at (j0, j0) (kb = min(64, n - j0)) and
+// solve the panel row beside it.  Element (r, c), r <= c, of the block's
+// upper view is A[j0+r, j0+c] (U) or A[j0+c, j0+r] (L, i.e. L^T).  On a
+// failed pivot at local column j: info = j0 + j + 1, *abort = global_base +
+// j0 + j + 1, and the panel row is left untouched.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(64) void potrf_leaf_kernel(T* __restrict__ A, int n, int lda,
-                                                        int* __restrict__ info, int info_base,
-                                                        int* __restrict__ abort, int global_base) {
-  __shared__ T s[POTRF_LEAF][POTRF_LEAF + 1];  // s[r][c], symmetric view
+__global__ __launch_bounds__(PANEL_NT) void potrf_panel_kernel(T* __restrict__ A, int n, int lda,
+                                                               int j0, int* __restrict__ info,
+                                                               int* __restrict__ abort,
+                                                               int global_base) {
+  __shared__ T Us[PANEL][PANEL + 1];  // factored block, Us[r][c] for r <= c
+  __shared__ T invd[PANEL];           // 1 / Us[r][r]
+  __shared__ T rowb[PANEL];           // row of the current step
+  __shared__ T piv;
+  __shared__ int failed;
   if (abort && *abort) return;
   if (info && *info) return;
-  const int t = threadIdx.x;
-  // Load the referenced triangle into the upper view s[r][c], r <= c.
-  for (int c = 0; c < n; ++c) {
-    if (t < n) {
-      const int r = t;
-      if (UPPER) {
-        if (r <= c) s[r][c] = A[(size_t)c * lda + r];
-      } else {
-        // lower: element (c, r) with c >= r stored at s[r][c] (transpose)
-        if (r <= c) s[r][c] = A[(size_t)r * lda + c];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kb = min(PANEL, n - j0);
+  if (wave == 0) {
+    const int t = lane;
+    T col[PANEL];
+#pragma unroll
+    for (int r = 0; r < PANEL; ++r) {
+      const bool in = t < kb && r <= t;
+      col[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
+                  : T(0);
+    }
+    int fail = 0;
+#pragma unroll
+    for (int j = 0; j < PANEL; ++j) {
+      if (j < kb) {
+        if (t == j) piv = col[j];
+        wave_lds_sync();
+        const T d = piv;
+        if (!(d > T(0))) {
+          fail = j + 1;
+          break;
+        }
+        const T sd = sqrt(d), inv = T(1) / sd;
+        T u = T(0);
+        if (t == j) {
+          col[j] = sd;
+          invd[j] = inv;
+        } else if (t > j) {
+          u = col[j] * inv;
+          col[j] = u;
+        }
+        rowb[t] = u;  // row j of the factor (zero left of and on the diagonal)
+        wave_lds_sync();
+#pragma unroll
+        for (int r = j + 1; r < PANEL; ++r) col[r] -= rowb[r] * u;  // rows r <= t are kept
+        wave_lds_sync();
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PANEL; ++r) {
+      if (t < kb && r <= t) {
+        if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = col[r];
+        else A[(size_t)(j0 + r) * lda + j0 + t] = col[r];
+        Us[r][t] = col[r];
+      }
+    }
+    if (t == 0) {
+      failed = fail;
+      if (fail) {
+        if (info) *info = j0 + fail;
+        if (abort) *abort = global_base + j0 + fail;
       }
     }
   }
   __syncthreads();
-  int fail = 0;
-  for (int j = 0; j < n; ++j) {
-    const T d = s[j][j];
-    if (!(d > T(0))) { fail = j + 1; break; }
-    const T sd = sqrt(d);
-    const T inv = T(1) / sd;
-    T u = T(0);
-    if (t > j && t < n) {
-      u = s[j][t] * inv;
-      s[j][t] = u;
-    }
-    __syncthreads();
-    if (t == j) s[j][j] = sd;
-    if (t > j && t < n) {
-      for (int r = j + 1; r <= t; ++r) s[r][t] -= s[j][r] * u;
-    }
-    __syncthreads();
-  }
-  // Write back the factored part (LAPACK leaves columns past a failure as
-  // partially updated; the same holds here).
-  for (int c = 0; c < n; ++c) {
-    if (t < n) {
-      const int r = t;
-      if (r <= c) {
-        if (UPPER) A[(size_t)c * lda + r] = s[r][c];
-        else A[(size_t)r * lda + c] = s[r][c];
+  if (failed) return;
+  // Panel row: column cc of A12 (U) / row cc of A21 (L), element r at local
+  // row r of the block.  A panel row exists only beside a full 64-wide
+  // block (a narrower block is the last one), so no row masks are needed.
+  const int c_begin = j0 + kb, rest = n - c_begin;
+  if (rest <= 0) return;
+  const size_t step = UPPER ? 1 : (size_t)lda;
+  for (int c0 = wave * 64; c0 < rest; c0 += (PANEL_NT / 64) * 64) {
+    const int cc = c_begin + c0 + lane;
+    if (c0 + lane < rest) {
+      T* const base = UPPER ? A + (size_t)cc * lda + j0 : A + (size_t)j0 * lda + cc;
+      T x[PANEL];
+      const T* px = base;
+#pragma unroll
+      for (int r = 0; r < PANEL; ++r) {
+        x[r] = *px;
+        px += step;
+        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
+      }
+      // U11^T x = b, forward over the rows of the block
+#pragma unroll
+      for (int p = 0; p < PANEL; ++p) {
+        x[p] *= invd[p];
+#pragma unroll
+        for (int r = p + 1; r < PANEL; ++r) x[r] -= Us[p][r] * x[p];
+        __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of one step together
+      }
+      T* qx = base;
+      DM_OPAQUE(qx);  // recompute the store addresses instead of keeping 64 live
+#pragma unroll
+      for (int r = 0; r < PANEL; ++r) {
+        *qx = x[r];
+        qx += step;
+        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
-  if (fail && t == 0) {
-    if (info) *info = info_base + fail;
-    if (abort) *abort = global_base + fail;
-  }
-}
-
-template <typename T>
-static int potrf_rec(bool upper, int n, T* A, int lda, int* info, int info_base, int* abort,
-                     int global_base, const int* chk, T* ws, hipStream_t s) {
-  if (n <= 0) return 0;
-  if (n <= POTRF_LEAF) {
-    if (upper)
-      hipLaunchKernelGGL((potrf_leaf_kernel<T, true>), dim3(1), dim3(64), 0, s, A, n, lda, info,
-                         info_base, abort, global_base);
-    else
-      hipLaunchKernelGGL((potrf_leaf_kernel<T, false>), dim3(1), dim3(64), 0, s, A, n, lda, info,
-                         info_base, abort, global_base);
-    DM_LAUNCH_CHECK("potrf_leaf_kernel");
-    return 0;
-  }
-  // split point: half, rounded up to a multiple of the leaf size
-  const int n1 = ((n / 2) + POTRF_LEAF - 1) / POTRF_LEAF * POTRF_LEAF;
-  const int n2 = n - n1;
-  T* A11 = A;
-  T* A22 = A + (size_t)n1 * lda + n1;
-  int rc = potrf_rec<T>(upper, n1, A11, lda, info, info_base, abort, global_base, chk, ws, s);
-  if (rc) return rc;
-  if (upper) {
-    T* A12 = A + (size_t)n1 * lda;
-    rc = trsm_full<T>('L', 'U', 'T', 'N', n1, n2, T(1), A11, lda, A12, lda, ws, chk, s);
-    if (rc) return rc;
-    rc = gemm_single<T>(true, false, n2, n2, n1, T(-1), A12, lda, A12, lda, T(1), A22, lda, 1, chk,
-                        s);
-  } else {
-    T* A21 = A + n1;
-    rc = trsm_full<T>('R', 'L', 'T', 'N', n2, n1, T(1), A11, lda, A21, lda, ws, chk, s);
-    if (rc) return rc;
-    rc = gemm_single<T>(false, true, n2, n2, n1, T(-1), A21, lda, A21, lda, T(1), A22, lda, 2, chk,
-                        s);
-  }
-  if (rc) return rc;
-  return potrf_rec<T>(upper, n2, A22, lda, info, info_base + n1, abort, global_base + n1, chk, ws,
-                      s);
 }
 
 template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset, T* ws,
                hipStream_t s) {
-  // The trsm/syrk launches inside the recursion stop once a pivot failed:
-  // they watch the plan-level abort flag when there is one, else info.
+  (void)ws;
+  // The SYRK launches stop once a pivot failed: they watch the plan-level
+  // abort flag when there is one, else info.
   const int* chk = abort ? abort : info;
-  return potrf_rec<T>(upper, n, A, lda, info, 0, abort, global_offset, chk, ws, s);
+  for (int j0 = 0; j0 < n; j0 += PANEL) {
+    if (upper)
+      hipLaunchKernelGGL((potrf_panel_kernel<T, true>), dim3(1), dim3(PANEL_NT), 0, s, A, n, lda, j0,
+                         info, abort, global_offset);
+    else
+      hipLaunchKernelGGL((potrf_panel_kernel<T, false>), dim3(1), dim3(PANEL_NT), 0, s, A, n, lda,
+                         j0, info, abort, global_offset);
+    DM_LAUNCH_CHECK("potrf_panel_kernel");
+    const int kb = n - j0 < PANEL ? n - j0 : PANEL;
+    const int rest = n - j0 - kb;
+    if (rest <= 0) break;
+    int rc;
+    if (upper) {
+      const T* A12 = A + (size_t)(j0 + kb) * lda + j0;
+      T* A22 = A + (size_t)(j0 + kb) * lda + j0 + kb;
+      rc = gemm_single<T>(true, false, rest, rest, kb, T(-1), A12, lda, A12, lda, T(1), A22, lda, 1,
+                          chk, s);
+    } else {
+      const T* A21 = A + (size_t)j0 * lda + j0 + kb;
+      T* A22 = A + (size_t)(j0 + kb) * lda + j0 + kb;
+      rc = gemm_single<T>(false, true, rest, rest, kb, T(-1), A21, lda, A21, lda, T(1), A22, lda, 2,
+                          chk, s);
+    }
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 template int potrf_full<double>(bool, int, double*, int, int*, int*, int, double*, hipStream_t);

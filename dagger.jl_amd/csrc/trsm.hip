@@ -8,17 +8,19 @@
 //     transposed, lower or upper triangular;
 //   * the right-hand-side vectors are the columns (side L) or rows (side R)
 //     of B, solved in place.
-// The kernel works on 32-row blocks of Tm in solve order.  A workgroup owns
-// 16 vectors and keeps their solved part in LDS (so later blocks read it at
-// LDS speed and no global write->read ordering is needed).  For each block
-// the four waves split the k-range of  alpha*V_I - Tm[I,solved] X[solved]
-// on FP64/FP32 MFMA, reduce through LDS, and one MFMA product with the
-// pre-inverted 32x32 diagonal block gives X_I.  The diagonal blocks are
+// Solves longer than 256 are split on the host at a multiple of 32 from the
+// top of the triangle: solve the first part, update the second part with one
+// (batched) GEMM on the MFMA GEMM kernel, solve the second part -- so most of
+// the flops of a long solve run at GEMM speed.
+// A leaf (L <= 256) is one kernel: a workgroup owns 64 vectors, each of its
+// four waves 16 of them, with no cross-wave synchronisation.  A wave keeps its
+// solved part in LDS and walks 32-row blocks of Tm in solve order: MFMA
+// products of the block's rows with the solved part, then one MFMA product
+// with the pre-inverted 32x32 diagonal block.  The diagonal blocks are
 // inverted up front by a small kernel (one thread per column, exact
 // substitution), so the serial part of each block is a short MFMA chain.
-// Lengths above 1024 (FP64) / 2048 (FP32) are split on the host into
-// solve / GEMM update / solve.
 #include "dm_internal.h"
+#include "trsm_plan.h"
 
 namespace dm {
 
@@ -40,23 +42,6 @@ template <> struct MfmaT<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
 };
-
-TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
-  TrsmShape s;
-  s.side_l = is_left(side);
-  const bool up = is_upper(uplo);
-  const bool tr = is_trans(transa);
-  if (s.side_l) {
-    s.ttrans = tr;
-    s.lower = tr ? up : !up;
-  } else {
-    // X op(A) = B  <=>  op(A)^T X^T = B^T
-    s.ttrans = !tr;
-    s.lower = tr ? !up : up;
-  }
-  s.unit = is_unit(diag);
-  return s;
-}
 
 template <typename T, bool TTRANS>
 __device__ __forceinline__ T tm_at(const T* __restrict__ A, int lda, int i, int k) {
@@ -101,6 +86,13 @@ __global__ __launch_bounds__(32) void trinv_blocks_kernel(const T* __restrict__ 
   for (int r = 0; r < TRSM_BS; ++r) out[(size_t)c * TRSM_BS + r] = xs[r][c];
 }
 
+// Orders this wave's LDS writes before its later LDS reads.
+__device__ __forceinline__ void trsm_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T, bool LOWER, bool TTRANS, bool SIDE_L>
 __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict__ jobs, int njobs,
                                                    TrsmJob<T> single,
@@ -123,27 +115,30 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
     job = jobs[lo];
     local = wg - job.wg_begin;
   }
+  // The LDS layout follows the launch's longest solve (single.L).
+  const int Lmax = ((single.L + TRSM_BS - 1) / TRSM_BS) * TRSM_BS;
   const int L = job.L, NV = job.NV;
   const int nblk = (L + TRSM_BS - 1) / TRSM_BS;
   const int Lp = nblk * TRSM_BS;
-  T* X = reinterpret_cast<T*>(dm_trsm_smem);      // [Lp][16]
-  T* red = X + (size_t)Lp * 16;                    // [4][32][16]
-  T* bt = red + 4 * TRSM_BS * 16;                  // [32][16]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int j0 = local * 16;
+  T* X = reinterpret_cast<T*>(dm_trsm_smem) + (size_t)wave * (Lmax + TRSM_BS) * 16;  // [Lp][16]
+  T* bt = X + (size_t)Lmax * 16;                                                     // [32][16]
+  const int j0 = local * TRSM_VPW + wave * 16;  // first vector of this wave
+  if (j0 >= NV) return;                         // waves never synchronise with each other
   const T* __restrict__ A = job.A;
   T* __restrict__ B = job.B;
   const int lda = job.lda, ldb = job.ldb;
   const T alpha = job.alpha;
+  const int gj = j0 + lr;  // this lane's vector in the MFMA output layout
 
-  for (int e = tid; e < Lp * 16; e += 256) X[e] = T(0);
-  __syncthreads();
+  for (int e = lane; e < Lp * 16; e += 64) X[e] = T(0);
+  trsm_wave_sync();
 
   for (int bi = 0; bi < nblk; ++bi) {
     const int b = LOWER ? bi : nblk - 1 - bi;
     const int r0 = b * TRSM_BS;
-    // ---- phase 1: partial products with already-solved rows
+    // ---- acc = Tm[I, solved] X[solved]  (rows r0 + 0..15 and r0 + 16..31)
     const int kbeg = LOWER ? 0 : r0 + TRSM_BS;
     const int kend = LOWER ? r0 : L;
     const int nsteps = (kend - kbeg + 3) >> 2;
@@ -151,56 +146,61 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
 #pragma unroll
     for (int h = 0; h < 2; ++h) acc[h][0] = acc[h][1] = v4{0, 0, 0, 0};
     const int ra = r0 + lr, rb = r0 + 16 + lr;
-    for (int st = wave, par = 0; st < nsteps; st += 4, par ^= 1) {
-      const int k = kbeg + 4 * st + lk;
-      const bool kok = k < kend;
-      const T t0 = (kok && ra < L) ? tm_at<T, TTRANS>(A, lda, ra, k) : T(0);
-      const T t1 = (kok && rb < L) ? tm_at<T, TTRANS>(A, lda, rb, k) : T(0);
-      const T x = kok ? X[k * 16 + lr] : T(0);
-      if (par == 0) {
-        acc[0][0] = MfmaT<T>::mac(t0, x, acc[0][0]);
-        acc[1][0] = MfmaT<T>::mac(t1, x, acc[1][0]);
-      } else {
-        acc[0][1] = MfmaT<T>::mac(t0, x, acc[0][1]);
-        acc[1][1] = MfmaT<T>::mac(t1, x, acc[1][1]);
-      }
+    const bool raok = ra < L, rbok = rb < L;
+    int st = 0;
+    for (; st + 2 <= nsteps; st += 2) {
+      const int k0 = kbeg + 4 * st + lk, k1 = k0 + 4;
+      const bool ok0 = k0 < kend, ok1 = k1 < kend;
+      const T t00 = (ok0 && raok) ? tm_at<T, TTRANS>(A, lda, ra, k0) : T(0);
+      const T t10 = (ok0 && rbok) ? tm_at<T, TTRANS>(A, lda, rb, k0) : T(0);
+      const T t01 = (ok1 && raok) ? tm_at<T, TTRANS>(A, lda, ra, k1) : T(0);
+      const T t11 = (ok1 && rbok) ? tm_at<T, TTRANS>(A, lda, rb, k1) : T(0);
+      const T x0 = ok0 ? X[k0 * 16 + lr] : T(0);
+      const T x1 = ok1 ? X[k1 * 16 + lr] : T(0);
+      acc[0][0] = MfmaT<T>::mac(t00, x0, acc[0][0]);
+      acc[1][0] = MfmaT<T>::mac(t10, x0, acc[1][0]);
+      acc[0][1] = MfmaT<T>::mac(t01, x1, acc[0][1]);
+      acc[1][1] = MfmaT<T>::mac(t11, x1, acc[1][1]);
     }
+    if (st < nsteps) {
+      const int k0 = kbeg + 4 * st + lk;
+      const bool ok0 = k0 < kend;
+      const T t00 = (ok0 && raok) ? tm_at<T, TTRANS>(A, lda, ra, k0) : T(0);
+      const T t10 = (ok0 && rbok) ? tm_at<T, TTRANS>(A, lda, rb, k0) : T(0);
+      const T x0 = ok0 ? X[k0 * 16 + lr] : T(0);
+      acc[0][0] = MfmaT<T>::mac(t00, x0, acc[0][0]);
+      acc[1][0] = MfmaT<T>::mac(t10, x0, acc[1][0]);
+    }
+    // ---- bt = alpha V_I - acc  (lane: rows h*16 + row(lk, r), vector lr)
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        red[(wave * TRSM_BS + h * 16 + MfmaT<T>::row(lk, r)) * 16 + lr] = acc[h][0][r] + acc[h][1][r];
-    __syncthreads();
-    // ---- phase 2: bt = alpha*V_I - sum of partials
-    for (int e = tid; e < TRSM_BS * 16; e += 256) {
-      int i, j;
-      if (SIDE_L) { i = e % TRSM_BS; j = e / TRSM_BS; }  // rows contiguous in B
-      else { j = e % 16; i = e / 16; }                   // vectors contiguous in B
-      const int gi = r0 + i, gj = j0 + j;
-      T v = T(0);
-      if (gi < L && gj < NV) {
-        const T bv = SIDE_L ? B[(size_t)gj * ldb + gi] : B[(size_t)gi * ldb + gj];
-        const int q = i * 16 + j;
-        v = alpha * bv - (red[q] + red[TRSM_BS * 16 + q] + red[2 * TRSM_BS * 16 + q] +
-                          red[3 * TRSM_BS * 16 + q]);
+      for (int r = 0; r < 4; ++r) {
+        const int i = h * 16 + MfmaT<T>::row(lk, r);
+        const int gi = r0 + i;
+        T v = T(0);
+        if (gi < L && gj < NV) {
+          const T bv = SIDE_L ? B[(size_t)gj * ldb + gi] : B[(size_t)gi * ldb + gj];
+          v = alpha * bv - (acc[h][0][r] + acc[h][1][r]);
+        }
+        bt[i * 16 + lr] = v;
       }
-      bt[i * 16 + j] = v;
-    }
-    __syncthreads();
-    // ---- phase 3: X_I = inv(Tm_II) * bt  (waves 0,1 each own 16 rows)
-    if (wave < 2) {
-      const T* __restrict__ iv = job.inv + (size_t)b * TRSM_BS * TRSM_BS;
+    trsm_wave_sync();
+    // ---- X_I = inv(Tm_II) bt
+    const T* __restrict__ iv = job.inv + (size_t)b * TRSM_BS * TRSM_BS;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
       v4 xa = v4{0, 0, 0, 0}, xb = v4{0, 0, 0, 0};
 #pragma unroll
       for (int kk = 0; kk < TRSM_BS / 4; kk += 2) {
         const int c0 = 4 * kk + lk, c1 = 4 * (kk + 1) + lk;
-        xa = MfmaT<T>::mac(iv[c0 * TRSM_BS + wave * 16 + lr], bt[c0 * 16 + lr], xa);
-        xb = MfmaT<T>::mac(iv[c1 * TRSM_BS + wave * 16 + lr], bt[c1 * 16 + lr], xb);
+        xa = MfmaT<T>::mac(iv[c0 * TRSM_BS + h * 16 + lr], bt[c0 * 16 + lr], xa);
+        xb = MfmaT<T>::mac(iv[c1 * TRSM_BS + h * 16 + lr], bt[c1 * 16 + lr], xb);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = wave * 16 + MfmaT<T>::row(lk, r);
-        const int gi = r0 + i, gj = j0 + lr;
+        const int i = h * 16 + MfmaT<T>::row(lk, r);
+        const int gi = r0 + i;
         const T v = xa[r] + xb[r];
         X[gi * 16 + lr] = v;
         if (gi < L && gj < NV) {
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
         }
       }
     }
-    __syncthreads();
+    trsm_wave_sync();
   }
 }
 
@@ -238,7 +238,7 @@ namespace {
 template <typename T>
 size_t trsm_smem_bytes(int L) {
   const int Lp = ((L + TRSM_BS - 1) / TRSM_BS) * TRSM_BS;
-  return sizeof(T) * ((size_t)Lp * 16 + 4 * TRSM_BS * 16 + TRSM_BS * 16);
+  return sizeof(T) * 4 * (size_t)(Lp + TRSM_BS) * 16;  // per wave: X [Lp][16], bt [32][16]
 }
 template <typename T, bool LO, bool TT, bool SL>
 int trsm_go(const TrsmJob<T>* jobs, int njobs, const TrsmJob<T>& single, long total, size_t smem,
@@ -252,7 +252,7 @@ template <typename T, bool LO, bool TT, bool SL>
 int trsm_attr() {
   DM_HIP(hipFuncSetAttribute((const void*)trsm_kernel<T, LO, TT, SL>,
                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)trsm_smem_bytes<T>(trsm_max_len<T>())));
+                             (int)trsm_smem_bytes<T>(TRSM_LEAF)));
   return 0;
 }
 template <typename T>
@@ -299,18 +299,12 @@ int trsm_launch(const TrsmShape& sh, const TrsmJob<T>* jobs_dev, int njobs,
   return trsm_go<T, false, false, false>(jobs_dev, njobs, single, total_wg, smem, abort, s);
 }
 
-// Pointer to Tm[r, c] sub-block origin and whether it reads as op 'T' on A.
-template <typename T>
-static inline const T* tm_sub(const TrsmShape& sh, const T* A, int lda, int r, int c) {
-  return sh.ttrans ? A + (size_t)r * lda + c : A + (size_t)c * lda + r;
-}
-
-// Effective solve of L-long vectors with host recursion over trsm_max_len.
+// Unbatched solve with the diagonal blocks of each leaf inverted into ws.
 template <typename T>
 static int trsm_eff(const TrsmShape& sh, int L, int NV, T alpha, const T* A, int lda, T* B,
                     int ldb, T* ws, const int* abort, hipStream_t s) {
   if (L <= 0 || NV <= 0) return 0;
-  if (L <= trsm_max_len<T>()) {
+  if (L <= TRSM_LEAF) {
     if (!ws) return arg_fail(0, "trsm: no workspace");
     T* inv = ws;
     int rc = trinv_blocks_launch<T>(sh, A, lda, L, inv, abort, s);
@@ -319,33 +313,17 @@ static int trsm_eff(const TrsmShape& sh, int L, int NV, T alpha, const T* A, int
     j.A = A; j.B = B; j.inv = inv; j.L = L; j.NV = NV; j.lda = lda; j.ldb = ldb; j.alpha = alpha;
     return trsm_launch<T>(sh, nullptr, 0, j, trsm_wgs(NV), abort, s);
   }
-  // Split at a multiple of the block size: first solve the leading part in
-  // solve order, update the rest, then solve the rest.
-  int L1 = ((L / 2) + TRSM_BS - 1) / TRSM_BS * TRSM_BS;
-  const int L2 = L - L1;
-  // Vectors sub-block origin: side L -> row offset, side R -> column offset.
-  auto vsub = [&](int r) -> T* { return sh.side_l ? B + r : B + (size_t)r * ldb; };
-  // Lower: leading L1 rows first; upper: trailing L1 rows first.
-  const int first_n = L1, second_n = L2;
-  const int first_r = sh.lower ? 0 : L2;
-  const int second_r = sh.lower ? L1 : 0;
-  int rc = trsm_eff<T>(sh, first_n, NV, alpha, tm_sub(sh, A, lda, first_r, first_r), lda,
-                       vsub(first_r), ldb, ws, abort, s);
+  const TrsmSplit sp = trsm_split(sh, L);
+  int rc = trsm_eff<T>(sh, sp.first_n, NV, alpha, tm_sub(sh, A, lda, sp.first_r, sp.first_r), lda,
+                       vsub(sh, B, ldb, sp.first_r), ldb, ws, abort, s);
   if (rc) return rc;
-  // V2 := alpha*V2 - Tm[second, first] X1
-  const T* t21 = tm_sub(sh, A, lda, second_r, first_r);
-  if (sh.side_l) {
-    // rows: B[second] (second_n x NV) -= Tm21 (second_n x first_n) * X1 (first_n x NV)
-    rc = gemm_single<T>(sh.ttrans, false, second_n, NV, first_n, T(-1), t21, lda, vsub(first_r),
-                        ldb, alpha, vsub(second_r), ldb, 0, abort, s);
-  } else {
-    // columns: B[:, second] (NV x second_n) -= X1^T (NV x first_n) * Tm21^T (first_n x second_n)
-    rc = gemm_single<T>(false, !sh.ttrans, NV, second_n, first_n, T(-1), vsub(first_r), ldb, t21,
-                        lda, alpha, vsub(second_r), ldb, 0, abort, s);
-  }
+  bool ta, tb;
+  const GemmJob<T> g = trsm_update_job<T>(sh, sp, NV, alpha, A, lda, B, ldb, ta, tb);
+  rc = gemm_single<T>(ta, tb, g.M, g.N, g.K, g.alpha, g.A, g.lda, g.B, g.ldb, g.beta, g.C, g.ldc, 0,
+                      abort, s);
   if (rc) return rc;
-  return trsm_eff<T>(sh, second_n, NV, T(1), tm_sub(sh, A, lda, second_r, second_r), lda,
-                     vsub(second_r), ldb, ws, abort, s);
+  return trsm_eff<T>(sh, sp.second_n, NV, T(1), tm_sub(sh, A, lda, sp.second_r, sp.second_r), lda,
+                     vsub(sh, B, ldb, sp.second_r), ldb, ws, abort, s);
 }
 
 template <typename T>

@@ -178,7 +178,7 @@ void wave_exchange(const void* in, size_t bytes, void* all) {
   const long op = w.lane_ops[lane]++;
   const int par = (int)(op & 1);
   if (op != w.ops) die("wave op: lanes of one wavefront diverged");
-  std::memcpy(w.buf[par][lane], in, bytes);
+  if (bytes) std::memcpy(w.buf[par][lane], in, bytes);
   w.bytes[par] = bytes;
   if (++w.arrived[par] == 64) {
     w.arrived[par] = 0;
@@ -192,7 +192,8 @@ void wave_exchange(const void* in, size_t bytes, void* all) {
     to_scheduler();
     if (w.ops <= op) die("wave op resumed early");
   }
-  for (int l = 0; l < 64; ++l) std::memcpy((char*)all + l * bytes, w.buf[par][l], bytes);
+  if (bytes)
+    for (int l = 0; l < 64; ++l) std::memcpy((char*)all + l * bytes, w.buf[par][l], bytes);
 }
 
 // ------------------------------------------------------------ MFMA

@@ -126,6 +126,7 @@ inline T shfl_down(T v, unsigned d, int width) {
 #define __device__
 #define __host__
 #define __shared__ static
+#define DM_OPAQUE(x) asm volatile("" : "+r"(x))
 #define DM_DYNAMIC_LDS(name) unsigned char* name = ::emu::dynamic_shared()
 #define __forceinline__ inline
 #define __launch_bounds__(...)
@@ -134,6 +135,9 @@ inline T shfl_down(T v, unsigned d, int width) {
 #define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) \
   ::emu::mfma_f32_16x16x32_bf16((a), (b), (c))
 #define __shfl_down(v, d, w) ::emu::shfl_down((v), (d), (w))
+#define __builtin_amdgcn_wave_barrier() ::emu::wave_exchange(nullptr, 0, nullptr)
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+#define __builtin_amdgcn_sched_barrier(mask) ((void)0)
 #define hipLaunchKernelGGL(K, G, B, SH, ST, ...) \
   ::emu::launch(dim3(G), dim3(B), (size_t)(SH), (ST), [=]() { K(__VA_ARGS__); })
 

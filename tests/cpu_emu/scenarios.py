@@ -215,6 +215,19 @@ def tiled():
         dm.ldiv_(Fc, Dv)
         check(O.backward_error(A, Dv.collect().reshape(-1, 1), bv.reshape(-1, 1)) <
               10 * n * np.finfo(float).eps, ("ldiv vector", uplo))
+    # tiles longer than the leaf solve: the executor's recursive TRSM
+    n2, nb2 = 600, 288
+    A2 = gen.spd(n2, 16)
+    for uplo in "UL":
+        ref, _ = O.cholesky(A2, nb2, uplo)
+        DA = dm.DArray.from_array(A2, dm.Blocks(nb2, nb2))
+        Fc = dm.cholesky_(DA, uplo=uplo)
+        check(O.rel_factor_diff(DA.collect(), ref, uplo) < 1e-12, ("cholesky long tiles", uplo))
+        b = gen.randn(n2 * 3, 17).reshape(3, n2).T.copy()
+        DB = dm.DArray.from_array(b, dm.Blocks(nb2, nb2))
+        dm.ldiv_(Fc, DB)
+        check(O.backward_error(A2, DB.collect(), b) < 10 * n2 * np.finfo(float).eps,
+              ("ldiv long tiles", uplo))
     # non-PD: tile-local info
     Abad = A.copy()
     Abad[200, 200] = -1.0

@@ -38,11 +38,6 @@ int arg_fail(int a, const char* w) { fprintf(stderr, "arg_fail %d %s\n", a, w); 
 int ensure_init() { return 0; }
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : (hipStream_t)0x42; }
 void* stream_workspace(hipStream_t, size_t) { return nullptr; }
-TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
-  TrsmShape s; s.side_l = is_left(side); const bool up = is_upper(uplo), tr = is_trans(transa);
-  if (s.side_l) { s.ttrans = tr; s.lower = tr ? up : !up; } else { s.ttrans = !tr; s.lower = tr ? !up : up; }
-  s.unit = is_unit(diag); return s;
-}
 int trsm_init_attributes() { return 0; }
 int run_big_stack(std::function<int()> fn) { return fn(); }
 template <typename T> GemmCfgId gemm_pick_cfg(int, int, int, int) { return GemmCfgId{0, 128, 128}; }

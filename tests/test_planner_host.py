@@ -1,4 +1,4 @@
-"""The native planner (dag.hip, tiled.hip) compiled for the host with HIP and
+"""The native planner (dag.hip, tiled.hip, trsm_plan.hip) compiled for the host with HIP and
 the device launches stubbed out, run under AddressSanitizer and UBSan over
 every executor option set (sequential, batched multi-stream look-ahead,
 graph capture, profiling).  CPU only: catches host memory errors in
@@ -21,6 +21,7 @@ def test_planner_host_under_sanitizers(tmp_path):
            "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-x", "c++",
            "-I", H, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
            os.path.join(CSRC, "dag.hip"), os.path.join(CSRC, "tiled.hip"),
+           os.path.join(CSRC, "trsm_plan.hip"),
            os.path.join(H, "stubs.cpp"), os.path.join(H, "driver.cpp"), "-o", str(exe)]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1")
