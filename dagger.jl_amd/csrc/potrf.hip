@@ -4,15 +4,15 @@
 // 2-18, spawned at :39/:84).  info follows LAPACK: 0, or k > 0 when the
 // leading minor of order k is not positive definite (pivot <= 0 or NaN).
 //
-// Structure: blocked right-looking factorization with 64-wide panels, two
+// Structure: blocked right-looking factorization with 64-wide panels, three
 // launches per panel:
-//  1. potrf_panel_kernel (one workgroup of 8 waves): wave 0 factors the
-//     64 x 64 diagonal block in registers (lane c owns column c of U, or
-//     row c of L = column c of L^T), broadcasting the current row through
-//     LDS -- no workgroup barriers inside the factorization; then all waves
-//     solve the panel row U12 := U11^-T A12 (U) / A21 := A21 L11^-T (L), one
-//     column (row) per lane against the factored block held in LDS;
-//  2. the trailing update A22 -= U12^T U12 (A21 A21^T for L) on the MFMA
+//  1. potrf_diag_kernel (one wave): factors the 64 x 64 diagonal block in
+//     registers (lane c owns column c of U, or row c of L = column c of
+//     L^T), broadcasting the current row through LDS -- no barriers;
+//  2. potrf_row_kernel (one wave per 64 columns): the panel row
+//     U12 := U11^-T A12 (U) / A21 := A21 L11^-T (L), one column (row) per
+//     lane against the factored block held in LDS;
+//  3. the trailing update A22 -= U12^T U12 (A21 A21^T for L) on the MFMA
 //     SYRK kernel.
 // Every launch first checks the abort flag (and info), so after a failed
 // pivot the remaining launches are no-ops.
@@ -20,125 +20,133 @@
 
 namespace dm {
 
-constexpr int PANEL = 64;      // panel width: one column per lane of a wave
-constexpr int PANEL_NT = 512;  // 8 waves solve the panel row
+constexpr int PANEL = 64;  // panel width: one column per lane of a wave
 
-// Orders this wave's LDS writes before its later LDS reads (the lanes of one
-// wave exchange a row through LDS without a workgroup barrier).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Factor the kb x kb diagonal block at (j0, j0) (kb = min(64, n - j0)) and
-// solve the panel row beside it.  Element (r, c), r <= c, of the block's
-// upper view is A[j0+r, j0+c] (U) or A[j0+c, j0+r] (L, i.e. L^T).  On a
-// failed pivot at local column j: info = j0 + j + 1, *abort = global_base +
-// j0 + j + 1, and the panel row is left untouched.
+// Factor the kb x kb diagonal block at (j0, j0), kb = min(64, n - j0), in one
+// wave.  Element (r, c), r <= c, of the block's upper view is A[j0+r, j0+c]
+// (U) or A[j0+c, j0+r] (L, i.e. L^T); lane c holds column c of that view in
+// registers.  On a failed pivot at local column j: info = j0 + j + 1 and
+// *abort = global_base + j0 + j + 1.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(PANEL_NT) void potrf_panel_kernel(T* __restrict__ A, int n, int lda,
-                                                               int j0, int* __restrict__ info,
-                                                               int* __restrict__ abort,
-                                                               int global_base) {
-  __shared__ T Us[PANEL][PANEL + 1];  // factored block, Us[r][c] for r <= c
-  __shared__ T invd[PANEL];           // 1 / Us[r][r]
-  __shared__ T rowb[PANEL];           // row of the current step
-  __shared__ T piv;
-  __shared__ int failed;
+__global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n, int lda, int j0,
+                                                        int* __restrict__ info,
+                                                        int* __restrict__ abort,
+                                                        int global_base) {
+  __shared__ T rowb[PANEL];  // row of the current step
+  __shared__ T pv;           // next pivot
   if (abort && *abort) return;
   if (info && *info) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int t = threadIdx.x;
   const int kb = min(PANEL, n - j0);
-  if (wave == 0) {
-    const int t = lane;
-    T col[PANEL];
+  T col[PANEL];
 #pragma unroll
-    for (int r = 0; r < PANEL; ++r) {
-      const bool in = t < kb && r <= t;
-      col[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
-                  : T(0);
-    }
-    int fail = 0;
+  for (int r = 0; r < PANEL; ++r) {
+    const bool in = t < kb && r <= t;
+    col[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
+                : T(0);
+  }
+  // One wave, so its LDS accesses execute in order: broadcasts through LDS
+  // need only compiler ordering (wave_barrier), no waits.  The pivot of step
+  // j+1 is lane j+1's diagonal after step j, which that lane updates with its
+  // own u: it is broadcast together with row j, so the next square root and
+  // reciprocal overlap this step's rank-1 update.
+  int fail = 0;
+  if (t == 0) pv = col[0];
+  __builtin_amdgcn_wave_barrier();
+  T d = pv;
+  __builtin_amdgcn_wave_barrier();  // every lane has read pv before step 0 rewrites it
+  T sd = sqrt(d), inv = T(1) / sd;
 #pragma unroll
-    for (int j = 0; j < PANEL; ++j) {
-      if (j < kb) {
-        if (t == j) piv = col[j];
-        wave_lds_sync();
-        const T d = piv;
-        if (!(d > T(0))) {
-          fail = j + 1;
-          break;
-        }
-        const T sd = sqrt(d), inv = T(1) / sd;
-        T u = T(0);
-        if (t == j) {
-          col[j] = sd;
-          invd[j] = inv;
-        } else if (t > j) {
-          u = col[j] * inv;
-          col[j] = u;
-        }
-        rowb[t] = u;  // row j of the factor (zero left of and on the diagonal)
-        wave_lds_sync();
-#pragma unroll
-        for (int r = j + 1; r < PANEL; ++r) col[r] -= rowb[r] * u;  // rows r <= t are kept
-        wave_lds_sync();
+  for (int j = 0; j < PANEL; ++j) {
+    if (j < kb) {
+      if (!(d > T(0))) {
+        fail = j + 1;
+        break;
       }
-    }
+      T u = T(0);
+      if (t == j) col[j] = sd;
+      else if (t > j) {
+        u = col[j] * inv;
+        col[j] = u;
+      }
+      if (j + 1 < PANEL && t == j + 1) pv = col[j + 1] - u * u;
+      rowb[t] = u;  // row j of the factor (zero left of and on the diagonal)
+      __builtin_amdgcn_wave_barrier();
+      if (j + 1 < PANEL) {
+        d = pv;
+        sd = sqrt(d);
+        inv = T(1) / sd;
+      }
 #pragma unroll
-    for (int r = 0; r < PANEL; ++r) {
-      if (t < kb && r <= t) {
-        if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = col[r];
-        else A[(size_t)(j0 + r) * lda + j0 + t] = col[r];
-        Us[r][t] = col[r];
-      }
-    }
-    if (t == 0) {
-      failed = fail;
-      if (fail) {
-        if (info) *info = j0 + fail;
-        if (abort) *abort = global_base + j0 + fail;
-      }
+      for (int r = j + 1; r < PANEL; ++r) col[r] -= rowb[r] * u;  // rows r <= t are kept
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
-  __syncthreads();
-  if (failed) return;
-  // Panel row: column cc of A12 (U) / row cc of A21 (L), element r at local
-  // row r of the block.  A panel row exists only beside a full 64-wide
-  // block (a narrower block is the last one), so no row masks are needed.
-  const int c_begin = j0 + kb, rest = n - c_begin;
-  if (rest <= 0) return;
-  const size_t step = UPPER ? 1 : (size_t)lda;
-  for (int c0 = wave * 64; c0 < rest; c0 += (PANEL_NT / 64) * 64) {
-    const int cc = c_begin + c0 + lane;
-    if (c0 + lane < rest) {
-      T* const base = UPPER ? A + (size_t)cc * lda + j0 : A + (size_t)j0 * lda + cc;
-      T x[PANEL];
-      const T* px = base;
 #pragma unroll
-      for (int r = 0; r < PANEL; ++r) {
-        x[r] = *px;
-        px += step;
-        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
-      }
-      // U11^T x = b, forward over the rows of the block
-#pragma unroll
-      for (int p = 0; p < PANEL; ++p) {
-        x[p] *= invd[p];
-#pragma unroll
-        for (int r = p + 1; r < PANEL; ++r) x[r] -= Us[p][r] * x[p];
-        __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of one step together
-      }
-      T* qx = base;
-      DM_OPAQUE(qx);  // recompute the store addresses instead of keeping 64 live
-#pragma unroll
-      for (int r = 0; r < PANEL; ++r) {
-        *qx = x[r];
-        qx += step;
-        if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
-      }
+  for (int r = 0; r < PANEL; ++r) {
+    if (t < kb && r <= t) {
+      if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = col[r];
+      else A[(size_t)(j0 + r) * lda + j0 + t] = col[r];
     }
+  }
+  if (t == 0 && fail) {
+    if (info) *info = j0 + fail;
+    if (abort) *abort = global_base + j0 + fail;
+  }
+}
+
+// Panel row beside a factored full 64 x 64 block: U12 := U11^-T A12 (U) or
+// A21 := A21 L11^-T (L).  One wave per 64 columns of A12 (rows of A21), one
+// column (row) per lane in registers; the factored block sits in LDS.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(64) void potrf_row_kernel(T* __restrict__ A, int n, int lda, int j0,
+                                                       const int* __restrict__ info,
+                                                       const int* __restrict__ abort) {
+  __shared__ T Us[PANEL][PANEL + 1];  // upper view of the factored block
+  __shared__ T invd[PANEL];
+  if (abort && *abort) return;
+  if (info && *info) return;
+  const int t = threadIdx.x;
+  // coalesced copy of the block: U reads columns, L reads rows of L = L^T columns
+  for (int c = 0; c < PANEL; ++c) {
+    // U: element (t, c) = A[j0+t, j0+c] for t <= c; L: element (t, c) = A[j0+c, j0+t]
+    T v = T(0);
+    if (t <= c) v = UPPER ? A[(size_t)(j0 + c) * lda + j0 + t] : A[(size_t)(j0 + t) * lda + j0 + c];
+    Us[t][c] = v;
+  }
+  __syncthreads();
+  invd[t] = T(1) / Us[t][t];
+  __syncthreads();
+  const int c_begin = j0 + PANEL, rest = n - c_begin;
+  const int c0 = blockIdx.x * 64;
+  if (c0 + t >= rest) return;  // no barrier follows
+  const int cc = c_begin + c0 + t;
+  const size_t step = UPPER ? 1 : (size_t)lda;
+  T* const base = UPPER ? A + (size_t)cc * lda + j0 : A + (size_t)j0 * lda + cc;
+  T x[PANEL];
+  const T* px = base;
+#pragma unroll
+  for (int r = 0; r < PANEL; ++r) {
+    x[r] = *px;
+    px += step;
+    if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
+  }
+  // U11^T x = b, forward over the rows of the block
+#pragma unroll
+  for (int p = 0; p < PANEL; ++p) {
+    x[p] *= invd[p];
+#pragma unroll
+    for (int r = p + 1; r < PANEL; ++r) x[r] -= Us[p][r] * x[p];
+    __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of one step together
+  }
+  T* qx = base;
+  DM_OPAQUE(qx);  // recompute the store addresses instead of keeping 64 live
+#pragma unroll
+  for (int r = 0; r < PANEL; ++r) {
+    *qx = x[r];
+    qx += step;
+    if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -146,20 +154,28 @@ template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset, T* ws,
                hipStream_t s) {
   (void)ws;
-  // The SYRK launches stop once a pivot failed: they watch the plan-level
+  // Launches after a failed pivot are no-ops: they watch the plan-level
   // abort flag when there is one, else info.
   const int* chk = abort ? abort : info;
   for (int j0 = 0; j0 < n; j0 += PANEL) {
     if (upper)
-      hipLaunchKernelGGL((potrf_panel_kernel<T, true>), dim3(1), dim3(PANEL_NT), 0, s, A, n, lda, j0,
-                         info, abort, global_offset);
+      hipLaunchKernelGGL((potrf_diag_kernel<T, true>), dim3(1), dim3(64), 0, s, A, n, lda, j0, info,
+                         abort, global_offset);
     else
-      hipLaunchKernelGGL((potrf_panel_kernel<T, false>), dim3(1), dim3(PANEL_NT), 0, s, A, n, lda,
-                         j0, info, abort, global_offset);
-    DM_LAUNCH_CHECK("potrf_panel_kernel");
+      hipLaunchKernelGGL((potrf_diag_kernel<T, false>), dim3(1), dim3(64), 0, s, A, n, lda, j0, info,
+                         abort, global_offset);
+    DM_LAUNCH_CHECK("potrf_diag_kernel");
     const int kb = n - j0 < PANEL ? n - j0 : PANEL;
     const int rest = n - j0 - kb;
-    if (rest <= 0) break;
+    if (rest <= 0) break;  // a block narrower than 64 is the last one
+    const unsigned nwg = (unsigned)((rest + 63) / 64);
+    if (upper)
+      hipLaunchKernelGGL((potrf_row_kernel<T, true>), dim3(nwg), dim3(64), 0, s, A, n, lda, j0, info,
+                         abort);
+    else
+      hipLaunchKernelGGL((potrf_row_kernel<T, false>), dim3(nwg), dim3(64), 0, s, A, n, lda, j0,
+                         info, abort);
+    DM_LAUNCH_CHECK("potrf_row_kernel");
     int rc;
     if (upper) {
       const T* A12 = A + (size_t)(j0 + kb) * lda + j0;

@@ -123,7 +123,9 @@ class MixedCholesky:
         """Solve A X = B (B an FP64 DArray N x nrhs, Blocks(nb, nb)).
 
         Returns (X, info) with info = dict(iterations, berr (per column),
-        method, history)."""
+        method, history, escalated).  A factor built with the 2-term BF16
+        update that does not reach tol is rebuilt with the 3-term update
+        (escalated = True)."""
         if B.dtype != np.float64 or B.m != self.A.m:
             raise ValueError("B must be an FP64 DArray with A's rows")
         s = self.stream
@@ -159,7 +161,18 @@ class MixedCholesky:
             self.residual(B, Xd, R)
             berr = _colnorms(R, nrm, s) / (self.normA * _colnorms(Xd, nrm, s) + nb_)
         call("dm_stream_sync", s)
-        return Xd, dict(iterations=it, berr=berr, method=method, history=hist)
+        if berr.max() > tol and self.options.f32_update == 3:
+            # The 2-term BF16 update (about 16 significant bits) is too coarse
+            # a preconditioner for this matrix (kappa * 2^-16 near or above
+            # 1): refactor with the 3-term update (FP32-accurate) and solve
+            # again from scratch.
+            self.options = self.options.replace(f32_update=6)
+            self.refactor()
+            Xd, info = self.solve(B, tol=tol, max_iter=max_iter, X=X)
+            info["iterations"] += it
+            info["escalated"] = True
+            return Xd, info
+        return Xd, dict(iterations=it, berr=berr, method=method, history=hist, escalated=False)
 
     def _gmres_ir(self, B, X, tol, max_outer):
         """GMRES-IR: outer FP64 refinement; each correction solves

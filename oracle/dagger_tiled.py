@@ -494,6 +494,15 @@ def reconstruction_error(F, A, uplo="U"):
     return float(np.linalg.norm(A - R) / np.linalg.norm(A))
 
 
+def backward_error_columns(A, X, B):
+    """Largest per-column normwise backward error
+    max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) -- the
+    per-right-hand-side criterion of LAPACK's iterative refinement (dsposv)."""
+    X2 = X.reshape(X.shape[0], -1)
+    B2 = B.reshape(B.shape[0], -1)
+    return max(backward_error(A, X2[:, j], B2[:, j]) for j in range(X2.shape[1]))
+
+
 def backward_error(A, X, B):
     """Normwise ||B - A X||_inf / (||A||_inf ||X||_inf + ||B||_inf)."""
     X2 = X.reshape(X.shape[0], -1)

@@ -139,7 +139,16 @@ void run_block(Run& r) {
 #endif
       r.cur = -1;
     }
-    if (!progressed) die("deadlock: work-items wait at a barrier or wave op that not all reach");
+    if (!progressed) {
+      int cnt[4] = {0, 0, 0, 0};
+      for (int i = 0; i < n; ++i) ++cnt[r.f[i].st];
+      fprintf(stderr, "emu: block (%u,%u,%u): %d runnable, %d at barrier, %d at wave op, %d done\n",
+              r.bid.x, r.bid.y, r.bid.z, cnt[RUNNABLE], cnt[AT_BARRIER], cnt[AT_WAVE], cnt[DONE]);
+      for (size_t w = 0; w < r.waves.size(); ++w)
+        fprintf(stderr, "emu:   wave %zu: ops %ld, arrived %d/%d\n", w, r.waves[w].ops,
+                r.waves[w].arrived[0], r.waves[w].arrived[1]);
+      die("deadlock: work-items wait at a barrier or wave op that not all reach");
+    }
   }
 }
 

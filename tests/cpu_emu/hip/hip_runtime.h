@@ -11,6 +11,11 @@
 // exact-size heap block, device memory is host memory (exact-size
 // allocations, so ASan flags any access past an allocation), launches run
 // synchronously, and stream capture records launches for replay.
+// The lanes of a wave run one after another between sync points, so lanes
+// that exchange data through LDS without a workgroup barrier must separate
+// each write from the other lanes' reads (and each read from a later
+// overwrite) by __builtin_amdgcn_wave_barrier(), which emits nothing on the
+// hardware, where a wave executes in lockstep.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -115,6 +120,11 @@ inline T shfl_down(T v, unsigned d, int width) {
   const unsigned base = lane - lane % w;
   return (lane % w) + d < w ? all[base + (lane % w) + d] : v;
 }
+inline int readlane(int v, int l) {
+  int all[64];
+  wave_exchange(&v, sizeof(int), all);
+  return all[l];
+}
 }  // namespace emu
 
 #define threadIdx (::emu::thread_idx())
@@ -138,6 +148,7 @@ inline T shfl_down(T v, unsigned d, int width) {
 #define __builtin_amdgcn_wave_barrier() ::emu::wave_exchange(nullptr, 0, nullptr)
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)
+#define __builtin_amdgcn_readlane(v, l) ::emu::readlane((v), (l))
 #define hipLaunchKernelGGL(K, G, B, SH, ST, ...) \
   ::emu::launch(dim3(G), dim3(B), (size_t)(SH), (ST), [=]() { K(__VA_ARGS__); })
 

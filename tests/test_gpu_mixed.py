@@ -36,14 +36,19 @@ def test_mixed_ir_reaches_fp64_backward_error(dm, kappa, update, nrhs):
     DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
     X, info = F.solve(DB, tol=1e-13, max_iter=30)
     x = X.collect()
-    # backward error from the closed form (A x via the compact WY product)
+    # per-column backward error from the closed form
     A = host.dense()
-    berr = O.backward_error(A, x, b)
+    berr = O.backward_error_columns(A, x, b)
     assert berr <= 1e-13, (berr, info)
     # forward error bounded by kappa * backward error (with slack)
     ferr = np.abs(x - xt).max() / np.abs(xt).max()
-    assert ferr <= 10 * kappa * 1e-13 + 1e-12
-    assert info["iterations"] <= 30
+    # (kappa is the 2-norm condition number; the inf-norm one can be larger by
+    # a factor up to sqrt(n), hence the slack)
+    assert ferr <= 100 * kappa * 1e-13 + 1e-12
+    assert info["iterations"] <= 60
+    assert info["berr"].max() <= 1e-13
+    if update == 6 or kappa <= 1e4:
+        assert not info["escalated"]
 
 
 def test_split_update_factor_close_to_fp32(dm):

@@ -159,6 +159,21 @@ def test_ldiv_solves_system(uplo, vector):
     assert O.backward_error(A, X, B) < 10 * n * np.finfo(float).eps
 
 
+def test_backward_error_known_values():
+    """Closed forms: an exact solution has zero backward error; with A = I,
+    X = 0 the error is ||B|| / ||B|| = 1; the per-column form is the largest
+    single-column value, and the matrix form sums rows across columns."""
+    A = np.eye(3)
+    B = np.array([[1.0, 2.0], [0.0, -4.0], [3.0, 0.5]])
+    assert O.backward_error(A, B.copy(), B) == 0.0
+    assert O.backward_error(A, np.zeros_like(B), B) == 1.0
+    X = B.copy()
+    X[0, 0] += 1.0  # column 0: ||r|| = 1, ||x|| = 3, ||b|| = 3 -> 1/6
+    assert np.isclose(O.backward_error_columns(A, X, B), 1.0 / 6.0)
+    # rows of |X| sum to 4, 4, 3.5 and rows of |B| to 3, 4, 3.5 -> 1 / (4 + 4)
+    assert np.isclose(O.backward_error(A, X, B), 1.0 / 8.0)
+
+
 def test_threaded_executor_equals_sequential():
     A = gen.spd(256, 8)
     with O.blas_single_thread():
