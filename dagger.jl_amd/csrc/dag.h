@@ -65,6 +65,9 @@ struct Group {
   void* table_dev = nullptr;
 };
 
+// Plans above this many flops issue eagerly rather than through a graph.
+constexpr double kGraphMaxFlops = 2e11;
+
 struct Dag {
   std::vector<TaskRec> tasks;
   std::vector<Region> regions;
@@ -103,7 +106,7 @@ struct Dag {
 
  private:
   int build_device_tables();
-  int create_streams();
+  int create_streams(bool eager);
   int capture();
   int issue(hipStream_t origin_stream);
   template <typename T>

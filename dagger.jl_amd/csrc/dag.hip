@@ -245,23 +245,53 @@ int Dag::plan(const dm_plan_opts& o) {
   }
   int rc = build_device_tables();
   if (rc) return rc;
-  rc = create_streams();
+  // Graph capture is limited to plans with at most two worker streams (a
+  // capture of the three-stream look-ahead-2 schedule sent the runtime's
+  // graph walk into unbounded recursion, a stack overflow in libamdhip64)
+  // and to plans small enough for launch overhead to matter: larger plans
+  // issue eagerly, which keeps the CU reservation of their critical stream.
+  double total_flops = 0;
+  for (const Group& G : groups) total_flops += G.flops;
+  const bool want_graph =
+      opts.use_graph && opts.num_streams <= 2 && total_flops <= kGraphMaxFlops;
+  rc = create_streams(!want_graph);
   if (rc) return rc;
   planned = true;
-  // Graph capture is limited to plans with at most two worker streams: a
-  // capture of the three-stream look-ahead-2 schedule sent the runtime's
-  // graph walk into unbounded recursion (stack overflow in libamdhip64),
-  // so wider plans always issue eagerly.
-  if (opts.use_graph && opts.num_streams <= 2) {
+  if (want_graph) {
     rc = capture();
     if (rc) return rc;
   }
   return 0;
 }
 
-int Dag::create_streams() {
+int Dag::create_streams(bool eager) {
   streams.resize(opts.num_streams);
-  for (auto& s : streams) DM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  // With look-ahead, the streams other than the critical one are kept off
+  // an eighth of the CUs (spread over the CU mask so that every shader
+  // engine keeps some): the bulk trailing update would otherwise occupy
+  // every CU for milliseconds and delay each small launch of the critical
+  // path (diagonal POTRF, panel TRSM) until bulk workgroups retire.
+  const bool reserve = eager && opts.lookahead > 0 && opts.num_streams >= 2;
+  std::vector<uint32_t> mask;
+  if (reserve) {
+    int dev = 0, cus = 0;
+    DM_HIP(hipGetDevice(&dev));
+    DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus >= 16) {
+      mask.assign((size_t)(cus + 31) / 32, 0u);
+      for (int i = 0; i < cus; ++i) mask[i / 32] |= 1u << (i % 32);
+      for (int k = 0; k < cus / 8; ++k) {
+        const int i = k * 8 + (k % 8);  // residues mod 8 and blocks of 32 both covered evenly
+        if (i < cus) mask[i / 32] &= ~(1u << (i % 32));
+      }
+    }
+  }
+  for (size_t i = 0; i < streams.size(); ++i) {
+    if (i > 0 && !mask.empty())
+      DM_HIP(hipExtStreamCreateWithCUMask(&streams[i], (uint32_t)mask.size(), mask.data()));
+    else
+      DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
+  }
   DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   events.resize(groups.size());
   for (auto& e : events) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));

@@ -198,10 +198,14 @@ typedef struct dm_plan_opts {
     int num_streams;     /* streams to spread groups over (>=1) */
     int batch;           /* 1: batch independent same-shape tasks per launch */
     int lookahead;       /* slack (in levels) below which tasks go on the
-                            critical stream; 0 disables the split */
+                            critical stream; 0 disables the split.  Eager
+                            plans with look-ahead keep their other streams
+                            off 1/8 of the CUs, reserved for the critical
+                            stream */
     int use_graph;       /* 1: capture the schedule into a hipGraph (plans
-                            with at most 2 streams; wider plans issue
-                            eagerly, dm_dag_stats.graph reports 0) */
+                            with at most 2 streams and at most 2e11 flops;
+                            other plans issue eagerly, and
+                            dm_dag_stats.graph reports 0) */
     int profile;         /* 1 (eager runs only): time every launch group
                             with HIP events on its own stream */
     int f32_update;      /* FP32 GEMM/SYRK arithmetic: 0 FP32 matrix cores;

@@ -62,6 +62,7 @@ hipError_t hipDeviceGetAttribute(int*, hipDeviceAttribute_t, int);
 hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int);
 hipError_t hipStreamCreateWithFlags(hipStream_t*, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);
+hipError_t hipExtStreamCreateWithCUMask(hipStream_t*, uint32_t, const uint32_t*);
 hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipEventCreate(hipEvent_t*);
 hipError_t hipEventCreateWithFlags(hipEvent_t*, unsigned);

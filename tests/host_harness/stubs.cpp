@@ -8,6 +8,12 @@
 #include "dm_internal.h"
 static int g_streams = 0, g_events = 0;
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (hipStream_t)(uintptr_t)(0x1000 + ++g_streams); return 0; }
+hipError_t hipExtStreamCreateWithCUMask(hipStream_t* s, uint32_t n, const uint32_t* m) {
+  if (!n || !m) abort();
+  return hipStreamCreateWithFlags(s, 0);
+}
+hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 256; return 0; }
 hipError_t hipStreamDestroy(hipStream_t s) { if (!s) { fprintf(stderr, "destroy null stream\n"); abort(); } return 0; }
 hipError_t hipEventCreate(hipEvent_t* e) { *e = (hipEvent_t)(uintptr_t)(0x100000 + ++g_events); return 0; }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
