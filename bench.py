@@ -297,8 +297,8 @@ def main():
             "workload": (f"plain FP64 tiled Cholesky (cholesky!, U) + ldiv! with {nrhs} RHS, "
                          f"N={n}, Blocks({nb},{nb}), A=G*G^T/N+I (Philox seed {a.seed})"),
             "n": n, "nb": nb, "nrhs": nrhs,
-            "parallelism": ("1 GPU: 2 HIP streams, look-ahead 1, eager issue with the bulk "
-                            "stream kept off 1/8 of the CUs (reserved for the critical path)"),
+            "parallelism": ("1 GPU: 2 HIP streams, look-ahead 1, eager issue; bulk launches "
+                            "capped at 7/8 of the CUs, the rest kept for the critical path"),
         },
         "cholesky_ms": round(chol_ms, 3),
         "ldiv_ms": round(solve_ms, 3),

@@ -248,8 +248,8 @@ int Dag::plan(const dm_plan_opts& o) {
   // Graph capture is limited to plans with at most two worker streams (a
   // capture of the three-stream look-ahead-2 schedule sent the runtime's
   // graph walk into unbounded recursion, a stack overflow in libamdhip64)
-  // and to plans small enough for launch overhead to matter: larger plans
-  // issue eagerly, which keeps the CU reservation of their critical stream.
+  // and to plans small enough for launch overhead to matter (larger plans
+  // issue eagerly).
   double total_flops = 0;
   for (const Group& G : groups) total_flops += G.flops;
   const bool want_graph =
@@ -266,32 +266,11 @@ int Dag::plan(const dm_plan_opts& o) {
 
 int Dag::create_streams(bool eager) {
   streams.resize(opts.num_streams);
-  // With look-ahead, the streams other than the critical one are kept off
-  // an eighth of the CUs (spread over the CU mask so that every shader
-  // engine keeps some): the bulk trailing update would otherwise occupy
-  // every CU for milliseconds and delay each small launch of the critical
-  // path (diagonal POTRF, panel TRSM) until bulk workgroups retire.
-  const bool reserve = eager && opts.lookahead > 0 && opts.num_streams >= 2;
-  std::vector<uint32_t> mask;
-  if (reserve) {
-    int dev = 0, cus = 0;
-    DM_HIP(hipGetDevice(&dev));
-    DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (cus >= 16) {
-      mask.assign((size_t)(cus + 31) / 32, 0u);
-      for (int i = 0; i < cus; ++i) mask[i / 32] |= 1u << (i % 32);
-      for (int k = 0; k < cus / 8; ++k) {
-        const int i = k * 8 + (k % 8);  // residues mod 8 and blocks of 32 both covered evenly
-        if (i < cus) mask[i / 32] &= ~(1u << (i % 32));
-      }
-    }
-  }
-  for (size_t i = 0; i < streams.size(); ++i) {
-    if (i > 0 && !mask.empty())
-      DM_HIP(hipExtStreamCreateWithCUMask(&streams[i], (uint32_t)mask.size(), mask.data()));
-    else
-      DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
-  }
+  (void)eager;
+  int dev = 0;
+  DM_HIP(hipGetDevice(&dev));
+  DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  for (auto& st : streams) DM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   events.resize(groups.size());
   for (auto& e : events) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -493,11 +472,16 @@ int Dag::build_device_tables() {
 template <typename T>
 int Dag::launch_group_t(const Group& G, hipStream_t s) {
   const int* ab = abort_dev;
+  // Bulk (non-critical) launches of a look-ahead schedule use at most 7/8 of
+  // the CUs, so the critical stream's small launches always find free CUs
+  // instead of queueing behind milliseconds of trailing update.
+  const bool cap = opts.lookahead > 0 && opts.num_streams >= 2 && G.crit != 0 && cus > 0;
+  const long bulk_cus = cus - cus / 8;
   if (G.op == DM_OP_GEMM) {
     if (G.njobs == 0) return 0;
     GemmJob<T> dummy{};
     return gemm_launch<T>(G.ta, G.tb, G.vec, G.cfg, (const GemmJob<T>*)G.table_dev, G.njobs, dummy,
-                          G.total_wg, ab, s);
+                          G.total_wg, ab, s, cap ? bulk_cus * gemm_wgs_per_cu(G.cfg) : 0);
   }
   if (G.op == DM_OP_TRSM && G.batched_trsm) {
     for (const InvSrc& src : G.inv_src) {
@@ -509,12 +493,13 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       if (st.leaf) {
         TrsmJob<T> dummy{};
         dummy.L = st.maxL;
+        // leaf LDS: 147 KB (FP64) -> one workgroup per CU, two for FP32
         rc = trsm_launch<T>(G.shape, (const TrsmJob<T>*)st.table_dev, st.njobs, dummy, st.total_wg,
-                            ab, s);
+                            ab, s, cap ? bulk_cus * (sizeof(T) == 8 ? 1 : 2) : 0);
       } else {
         GemmJob<T> dummy{};
         rc = gemm_launch<T>(st.ta, st.tb, st.vec, st.cfg, (const GemmJob<T>*)st.table_dev, st.njobs,
-                            dummy, st.total_wg, ab, s);
+                            dummy, st.total_wg, ab, s, cap ? bulk_cus * gemm_wgs_per_cu(st.cfg) : 0);
       }
       if (rc) return rc;
     }

@@ -99,10 +99,13 @@ long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg);
 // vec: every job satisfies gemm_job_vec_ok (2-wide operand loads).
 template <typename T>
 bool gemm_job_vec_ok(bool ta, bool tb, const GemmJob<T>& j);
+// max_wg > 0 caps the grid (workgroups then loop over their tiles).
 template <typename T>
 int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* jobs_dev,
                 int njobs, const GemmJob<T>& single, long total_wg,
-                const int* abort, hipStream_t s);
+                const int* abort, hipStream_t s, long max_wg = 0);
+// Workgroups of a GEMM configuration that fit on one CU at once.
+int gemm_wgs_per_cu(GemmCfgId cfg);
 
 // Convenience: one GEMM call (BLAS semantics incl. tri for SYRK).
 template <typename T>
@@ -119,8 +122,8 @@ inline GemmCfgId gemm_split_cfg(int terms) {
 // op(A) = A^T, op(B) = B (k-contiguous operands), 16-byte aligned rows.
 bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j);
 int gemm_split_launch(int terms, const GemmJob<float>* jobs_dev, int njobs,
-                      const GemmJob<float>& single, long total_wg, const int* abort,
-                      hipStream_t s);
+                      const GemmJob<float>& single, long total_wg, long grid,
+                      const int* abort, hipStream_t s);
 
 // ------------------------------------------------------------ TRSM family
 // Effective left solve  Tm X = alpha V  on L-long vectors, NV of them.
@@ -164,7 +167,7 @@ int trinv_blocks_launch(const TrsmShape& sh, const T* A, int lda, int L, T* inv,
 template <typename T>
 int trsm_launch(const TrsmShape& sh, const TrsmJob<T>* jobs_dev, int njobs,
                 const TrsmJob<T>& single, long total_wg, const int* abort,
-                hipStream_t s);
+                hipStream_t s, long max_wg = 0);
 int trsm_init_attributes();
 inline int trsm_wgs(int NV) { return (NV + TRSM_VPW - 1) / TRSM_VPW; }
 

@@ -15,9 +15,12 @@
 //    as 16 rows x 4 k per MFMA; mn-contiguous operands are stored
 //    [k][MN+16] so a quarter-wave reads 128 contiguous bytes.
 //  * A launch is a batch of independent jobs (one per Dagger tile task);
-//    workgroups find their job by binary search over wg_begin, and the
-//    blockIdx -> logical workgroup map keeps consecutive tiles of one job on
-//    one XCD (blockIdx is dealt round-robin over the 8 XCDs).
+//    workgroups find their job by binary search over wg_begin, and each XCD
+//    (blockIdx is dealt round-robin over the 8 XCDs) owns a contiguous range
+//    of tiles, so consecutive tiles of one job share an L2.  A launch may use
+//    fewer workgroups than tiles (each then loops over its share): the
+//    executor caps bulk launches below the CU count so that the critical
+//    path's small launches always find free CUs.
 //  * SYRK is the same kernel with a triangle mask: only workgroup tiles that
 //    touch the requested triangle are launched, and diagonal tiles store
 //    masked.
@@ -261,14 +264,18 @@ struct GemmKernel {
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __restrict__ jobs,
                                                             int njobs, GemmJob<T> single,
+                                                            int total,
                                                             const int* __restrict__ abort) {
   typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC> G;
   __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
   if (abort && *abort) return;
-  GemmJob<T> job;
-  int tm, tn;
-  locate_tile(jobs, njobs, single, job, tm, tn);
-  G::run(job, tm, tn, smem);
+  for (TileRange r = tile_range(total); r.t < r.end; r.t += r.step) {
+    GemmJob<T> job;
+    int tm, tn;
+    locate_tile(jobs, njobs, single, r.t, job, tm, tn);
+    G::run(job, tm, tn, smem);
+    __syncthreads();  // the next tile's prologue rewrites LDS
+  }
 }
 
 // ------------------------------------------------------------ host side
@@ -307,26 +314,27 @@ long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg) {
 
 namespace {
 template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int VEC>
-int launch_one(const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total,
+int launch_one(const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total, long grid,
                const int* abort, hipStream_t s) {
-  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16, WM, WN, TA, TB, VEC>), dim3((unsigned)total),
-                     dim3(64 * WM * WN), 0, s, jobs, njobs, single, abort);
+  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16, WM, WN, TA, TB, VEC>), dim3((unsigned)grid),
+                     dim3(64 * WM * WN), 0, s, jobs, njobs, single, (int)total, abort);
   DM_LAUNCH_CHECK("gemm_kernel");
   return 0;
 }
 template <typename T, bool TA, bool TB, int VEC>
 int launch_cfg(int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total,
-               const int* abort, hipStream_t s) {
-  if (id == 0) return launch_one<T, 128, 128, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, abort, s);
-  return launch_one<T, 64, 64, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, abort, s);
+               long grid, const int* abort, hipStream_t s) {
+  if (id == 0)
+    return launch_one<T, 128, 128, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
+  return launch_one<T, 64, 64, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
 }
 template <typename T, int VEC>
 int launch_tt(bool ta, bool tb, int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single,
-              long total, const int* abort, hipStream_t s) {
-  if (ta && tb) return launch_cfg<T, true, true, VEC>(id, jobs, njobs, single, total, abort, s);
-  if (ta) return launch_cfg<T, true, false, VEC>(id, jobs, njobs, single, total, abort, s);
-  if (tb) return launch_cfg<T, false, true, VEC>(id, jobs, njobs, single, total, abort, s);
-  return launch_cfg<T, false, false, VEC>(id, jobs, njobs, single, total, abort, s);
+              long total, long grid, const int* abort, hipStream_t s) {
+  if (ta && tb) return launch_cfg<T, true, true, VEC>(id, jobs, njobs, single, total, grid, abort, s);
+  if (ta) return launch_cfg<T, true, false, VEC>(id, jobs, njobs, single, total, grid, abort, s);
+  if (tb) return launch_cfg<T, false, true, VEC>(id, jobs, njobs, single, total, grid, abort, s);
+  return launch_cfg<T, false, false, VEC>(id, jobs, njobs, single, total, grid, abort, s);
 }
 // Vector (2-wide) loads need: even leading dimensions, 2-element aligned
 // pointers and even extents along each operand's contiguous axis.
@@ -343,21 +351,29 @@ bool job_vec_ok(bool ta, bool tb, const GemmJob<T>& j) {
 
 template <typename T>
 int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* jobs_dev, int njobs,
-                const GemmJob<T>& single, long total_wg, const int* abort, hipStream_t s) {
+                const GemmJob<T>& single, long total_wg, const int* abort, hipStream_t s,
+                long max_wg) {
   if (total_wg <= 0) return 0;
   if (total_wg > 0x7fffffffL) return arg_fail(0, "gemm: too many workgroups");
+  const long grid = (max_wg > 0 && max_wg < total_wg) ? max_wg : total_wg;
   if (cfg.id == GEMM_CFG_BF16X3 || cfg.id == GEMM_CFG_BF16X6) {
     if constexpr (std::is_same<T, float>::value)
       return gemm_split_launch(cfg.id == GEMM_CFG_BF16X6 ? 6 : 3, jobs_dev, njobs, single,
-                               total_wg, abort, s);
+                               total_wg, grid, abort, s);
     return arg_fail(0, "gemm: BF16 split configurations are FP32 only");
   }
-  if (vec) return launch_tt<T, 2>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, abort, s);
-  return launch_tt<T, 1>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, abort, s);
+  if (vec) return launch_tt<T, 2>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, grid, abort, s);
+  return launch_tt<T, 1>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, grid, abort, s);
 }
 
 template <typename T>
 bool gemm_job_vec_ok(bool ta, bool tb, const GemmJob<T>& j) { return job_vec_ok(ta, tb, j); }
+
+int gemm_wgs_per_cu(GemmCfgId cfg) {
+  // 128x128 FP64: ~350 of the 512 registers per SIMD lane -> one workgroup;
+  // 64x64 and the BF16-split kernel fit two.
+  return cfg.id == 0 ? 1 : 2;
+}
 
 template <typename T>
 int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A, int lda, const T* B,
@@ -371,7 +387,7 @@ int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A, int 
   if (K <= 0 || alpha == T(0)) { j.alpha = T(0); j.K = 0; }
   GemmCfgId cfg = gemm_pick_cfg<T>(M, N, K, 1);
   long total = gemm_prepare_jobs<T>(&j, 1, cfg);
-  return gemm_launch<T>(ta, tb, job_vec_ok(ta, tb, j), cfg, nullptr, 0, j, total, abort, s);
+  return gemm_launch<T>(ta, tb, job_vec_ok(ta, tb, j), cfg, nullptr, 0, j, total, abort, s, 0);
 }
 
 template GemmCfgId gemm_pick_cfg<double>(int, int, int, int);
@@ -379,9 +395,9 @@ template GemmCfgId gemm_pick_cfg<float>(int, int, int, int);
 template long gemm_prepare_jobs<double>(GemmJob<double>*, int, GemmCfgId);
 template long gemm_prepare_jobs<float>(GemmJob<float>*, int, GemmCfgId);
 template int gemm_launch<double>(bool, bool, bool, GemmCfgId, const GemmJob<double>*, int,
-                                 const GemmJob<double>&, long, const int*, hipStream_t);
+                                 const GemmJob<double>&, long, const int*, hipStream_t, long);
 template int gemm_launch<float>(bool, bool, bool, GemmCfgId, const GemmJob<float>*, int,
-                                const GemmJob<float>&, long, const int*, hipStream_t);
+                                const GemmJob<float>&, long, const int*, hipStream_t, long);
 template bool gemm_job_vec_ok<double>(bool, bool, const GemmJob<double>&);
 template bool gemm_job_vec_ok<float>(bool, bool, const GemmJob<float>&);
 template int gemm_single<double>(bool, bool, int, int, int, double, const double*, int,

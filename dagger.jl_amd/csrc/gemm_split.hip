@@ -187,14 +187,18 @@ struct SplitGemm {
 template <int TERMS>
 __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmJob<float>* __restrict__ jobs,
                                                          int njobs, GemmJob<float> single,
+                                                         int total,
                                                          const int* __restrict__ abort) {
   typedef SplitGemm<TERMS> G;
   __shared__ float smem[2 * G::STAGE];
   if (abort && *abort) return;
-  GemmJob<float> job;
-  int tm, tn;
-  locate_tile(jobs, njobs, single, job, tm, tn);
-  G::run(job, tm, tn, smem);
+  for (TileRange r = tile_range(total); r.t < r.end; r.t += r.step) {
+    GemmJob<float> job;
+    int tm, tn;
+    locate_tile(jobs, njobs, single, r.t, job, tm, tn);
+    G::run(job, tm, tn, smem);
+    __syncthreads();  // the next tile's prologue rewrites LDS
+  }
 }
 
 }  // namespace
@@ -208,16 +212,17 @@ bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
 }
 
 int gemm_split_launch(int terms, const GemmJob<float>* jobs_dev, int njobs,
-                      const GemmJob<float>& single, long total_wg, const int* abort,
+                      const GemmJob<float>& single, long total_wg, long grid, const int* abort,
                       hipStream_t s) {
   if (total_wg <= 0) return 0;
   if (total_wg > 0x7fffffffL) return arg_fail(0, "gemm: too many workgroups");
+  if (grid <= 0 || grid > total_wg) grid = total_wg;
   if (terms == 6)
-    hipLaunchKernelGGL(gemm_split_kernel<6>, dim3((unsigned)total_wg), dim3(256), 0, s, jobs_dev,
-                       njobs, single, abort);
+    hipLaunchKernelGGL(gemm_split_kernel<6>, dim3((unsigned)grid), dim3(256), 0, s, jobs_dev,
+                       njobs, single, (int)total_wg, abort);
   else if (terms == 3)
-    hipLaunchKernelGGL(gemm_split_kernel<3>, dim3((unsigned)total_wg), dim3(256), 0, s, jobs_dev,
-                       njobs, single, abort);
+    hipLaunchKernelGGL(gemm_split_kernel<3>, dim3((unsigned)grid), dim3(256), 0, s, jobs_dev,
+                       njobs, single, (int)total_wg, abort);
   else
     return arg_fail(1, "split GEMM terms must be 3 or 6");
   DM_LAUNCH_CHECK("gemm_split_kernel");

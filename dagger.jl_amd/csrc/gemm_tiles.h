@@ -4,14 +4,6 @@
 
 namespace dm {
 
-// Logical workgroup index such that each XCD receives a contiguous range.
-__device__ __forceinline__ int xcd_remap(int bid, int nwg) {
-  const int q = nwg >> 3, r = nwg & 7;
-  const int xcd = bid & 7, slot = bid >> 3;
-  const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-  return base + slot;
-}
-
 // Decode the idx-th tile (column by column) of an upper triangle of tiles.
 __device__ __forceinline__ void upper_tri_decode(int idx, int& tm, int& tn) {
   int c = (int)((sqrtf(8.0f * (float)idx + 1.0f) - 1.0f) * 0.5f);
@@ -21,13 +13,32 @@ __device__ __forceinline__ void upper_tri_decode(int idx, int& tm, int& tn) {
   tm = idx - c * (c + 1) / 2;
 }
 
-// Finds the job and the C tile (tm, tn) of this workgroup in a batched
+// The logical tiles of this workgroup in a launch of `total` tiles over
+// gridDim.x <= total workgroups: tiles t, t + step, ... below end.
+// Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups round-robin
+// over the 8 XCDs); each XCD owns a contiguous range of tiles, in proportion
+// to its workgroups, so neighbouring tiles of one job share an L2.  With one
+// workgroup per tile every workgroup gets exactly one tile.
+struct TileRange {
+  int t, end, step;
+};
+__device__ __forceinline__ TileRange tile_range(int total) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int x = b & 7, slot = b >> 3;
+  const int q = G >> 3, r = G & 7;
+  const int nx = q + (x < r ? 1 : 0);                  // workgroups on XCD x
+  const int base = x * q + (x < r ? x : r);            // workgroups on XCDs < x
+  const int lo = (int)((long)total * base / G);
+  const int hi = (int)((long)total * (base + nx) / G);
+  return TileRange{lo + slot, hi, nx};
+}
+
+// Finds the job and the C tile (tm, tn) of logical tile `wg` in a batched
 // launch (binary search over wg_begin) or a single-job launch (njobs == 0).
 template <typename T>
 __device__ __forceinline__ void locate_tile(const GemmJob<T>* __restrict__ jobs, int njobs,
-                                            const GemmJob<T>& single, GemmJob<T>& job, int& tm,
-                                            int& tn) {
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+                                            const GemmJob<T>& single, int wg, GemmJob<T>& job,
+                                            int& tm, int& tn) {
   int local;
   if (njobs == 0) {
     job = single;

@@ -556,7 +556,7 @@ extern "C" int dm_sgemm_bf16x(int terms, char transa, char transb, int m, int n,
   j.alpha = alpha; j.beta = beta; j.tri = 0;
   if (k == 0 || alpha == 0.0f) { j.alpha = 0.0f; j.K = 0; }
   const long total = gemm_prepare_jobs<float>(&j, 1, gemm_split_cfg(terms));
-  return gemm_split_launch(terms, nullptr, 0, j, total, nullptr, stream_of(s));
+  return gemm_split_launch(terms, nullptr, 0, j, total, total, nullptr, stream_of(s));
 }
 
 extern "C" int dm_dtrsv(char uplo, char trans, char diag, int n, const double* A, int lda, double* x,

@@ -95,18 +95,21 @@ __device__ __forceinline__ void trsm_wave_sync() {
 
 template <typename T, bool LOWER, bool TTRANS, bool SIDE_L>
 __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict__ jobs, int njobs,
-                                                   TrsmJob<T> single,
+                                                   TrsmJob<T> single, int total,
                                                    const int* __restrict__ abort) {
   typedef typename MfmaT<T>::v4 v4;
   DM_DYNAMIC_LDS(dm_trsm_smem);
   if (abort && *abort) return;
+  // A launch may have fewer workgroups than units of 64 vectors: each
+  // workgroup then loops over its share (waves never synchronise, so a wave
+  // simply moves on to its next unit).
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
   TrsmJob<T> job;
   int local;
   if (njobs == 0) {
     job = single;
-    local = blockIdx.x;
+    local = wg;
   } else {
-    const int wg = blockIdx.x;
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
   T* X = reinterpret_cast<T*>(dm_trsm_smem) + (size_t)wave * (Lmax + TRSM_BS) * 16;  // [Lp][16]
   T* bt = X + (size_t)Lmax * 16;                                                     // [32][16]
   const int j0 = local * TRSM_VPW + wave * 16;  // first vector of this wave
-  if (j0 >= NV) return;                         // waves never synchronise with each other
+  if (j0 >= NV) continue;                       // waves never synchronise with each other
   const T* __restrict__ A = job.A;
   T* __restrict__ B = job.B;
   const int lda = job.lda, ldb = job.ldb;
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
     }
     trsm_wave_sync();
   }
+  }
 }
 
 // ------------------------------------------------------------ host side
@@ -241,10 +245,10 @@ size_t trsm_smem_bytes(int L) {
   return sizeof(T) * 4 * (size_t)(Lp + TRSM_BS) * 16;  // per wave: X [Lp][16], bt [32][16]
 }
 template <typename T, bool LO, bool TT, bool SL>
-int trsm_go(const TrsmJob<T>* jobs, int njobs, const TrsmJob<T>& single, long total, size_t smem,
-            const int* abort, hipStream_t s) {
-  hipLaunchKernelGGL((trsm_kernel<T, LO, TT, SL>), dim3((unsigned)total), dim3(256), smem, s, jobs,
-                     njobs, single, abort);
+int trsm_go(const TrsmJob<T>* jobs, int njobs, const TrsmJob<T>& single, long total, long grid,
+            size_t smem, const int* abort, hipStream_t s) {
+  hipLaunchKernelGGL((trsm_kernel<T, LO, TT, SL>), dim3((unsigned)grid), dim3(256), smem, s, jobs,
+                     njobs, single, (int)total, abort);
   DM_LAUNCH_CHECK("trsm_kernel");
   return 0;
 }
@@ -279,24 +283,22 @@ int trsm_init_attributes() {
 
 template <typename T>
 int trsm_launch(const TrsmShape& sh, const TrsmJob<T>* jobs_dev, int njobs,
-                const TrsmJob<T>& single, long total_wg, const int* abort, hipStream_t s) {
+                const TrsmJob<T>& single, long total_wg, const int* abort, hipStream_t s,
+                long max_wg) {
   if (total_wg <= 0) return 0;
+  if (total_wg > 0x7fffffffL) return arg_fail(0, "trsm: too many workgroups");
+  const long grid = (max_wg > 0 && max_wg < total_wg) ? max_wg : total_wg;
   // LDS is sized for the longest job; batched callers pass the max L in single.L.
   const size_t smem = trsm_smem_bytes<T>(single.L);
+#define DM_TRSM_GO(LO, TT, SL) \
+  trsm_go<T, LO, TT, SL>(jobs_dev, njobs, single, total_wg, grid, smem, abort, s)
   if (sh.lower) {
-    if (sh.ttrans) {
-      if (sh.side_l) return trsm_go<T, true, true, true>(jobs_dev, njobs, single, total_wg, smem, abort, s);
-      return trsm_go<T, true, true, false>(jobs_dev, njobs, single, total_wg, smem, abort, s);
-    }
-    if (sh.side_l) return trsm_go<T, true, false, true>(jobs_dev, njobs, single, total_wg, smem, abort, s);
-    return trsm_go<T, true, false, false>(jobs_dev, njobs, single, total_wg, smem, abort, s);
+    if (sh.ttrans) return sh.side_l ? DM_TRSM_GO(true, true, true) : DM_TRSM_GO(true, true, false);
+    return sh.side_l ? DM_TRSM_GO(true, false, true) : DM_TRSM_GO(true, false, false);
   }
-  if (sh.ttrans) {
-    if (sh.side_l) return trsm_go<T, false, true, true>(jobs_dev, njobs, single, total_wg, smem, abort, s);
-    return trsm_go<T, false, true, false>(jobs_dev, njobs, single, total_wg, smem, abort, s);
-  }
-  if (sh.side_l) return trsm_go<T, false, false, true>(jobs_dev, njobs, single, total_wg, smem, abort, s);
-  return trsm_go<T, false, false, false>(jobs_dev, njobs, single, total_wg, smem, abort, s);
+  if (sh.ttrans) return sh.side_l ? DM_TRSM_GO(false, true, true) : DM_TRSM_GO(false, true, false);
+  return sh.side_l ? DM_TRSM_GO(false, false, true) : DM_TRSM_GO(false, false, false);
+#undef DM_TRSM_GO
 }
 
 // Unbatched solve with the diagonal blocks of each leaf inverted into ws.
@@ -345,9 +347,9 @@ template int trinv_blocks_launch<double>(const TrsmShape&, const double*, int, i
 template int trinv_blocks_launch<float>(const TrsmShape&, const float*, int, int, float*,
                                         const int*, hipStream_t);
 template int trsm_launch<double>(const TrsmShape&, const TrsmJob<double>*, int,
-                                 const TrsmJob<double>&, long, const int*, hipStream_t);
+                                 const TrsmJob<double>&, long, const int*, hipStream_t, long);
 template int trsm_launch<float>(const TrsmShape&, const TrsmJob<float>*, int,
-                                const TrsmJob<float>&, long, const int*, hipStream_t);
+                                const TrsmJob<float>&, long, const int*, hipStream_t, long);
 template int trsm_full<double>(char, char, char, char, int, int, double, const double*, int,
                                double*, int, double*, const int*, hipStream_t);
 template int trsm_full<float>(char, char, char, char, int, int, float, const float*, int, float*,

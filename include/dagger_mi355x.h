@@ -198,10 +198,10 @@ typedef struct dm_plan_opts {
     int num_streams;     /* streams to spread groups over (>=1) */
     int batch;           /* 1: batch independent same-shape tasks per launch */
     int lookahead;       /* slack (in levels) below which tasks go on the
-                            critical stream; 0 disables the split.  Eager
-                            plans with look-ahead keep their other streams
-                            off 1/8 of the CUs, reserved for the critical
-                            stream */
+                            critical stream; 0 disables the split.  Bulk
+                            (non-critical) GEMM/TRSM launches then run on
+                            at most 7/8 of the CUs, leaving the rest free
+                            for the critical path */
     int use_graph;       /* 1: capture the schedule into a hipGraph (plans
                             with at most 2 streams and at most 2e11 flops;
                             other plans issue eagerly, and

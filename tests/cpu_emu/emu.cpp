@@ -321,7 +321,12 @@ static void host_op(std::function<void()> f) {
 hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
-hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 256; return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+  // EMU_CUS lowers the CU count so that capped (looping) launches are exercised
+  const char* e = std::getenv("EMU_CUS");
+  *v = e ? std::atoi(e) : 256;
+  return hipSuccess;
+}
 hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
   static int ids = 0;

@@ -348,7 +348,27 @@ def serving():
         check(r.berr <= r.tol, ("serving tol", r.rid))
 
 
-GROUPS = dict(tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving)
+def capped():
+    """Bulk launches capped below their tile count (EMU_CUS=8 in the
+    environment makes the executor cap bulk launches at 7 CUs' worth), so
+    GEMM and TRSM workgroups loop over several tiles each."""
+    n, nb = 600, 288
+    A = gen.spd(n, 26)
+    for uplo in "UL":
+        ref, _ = O.cholesky(A, nb, uplo)
+        DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+        opts = dm.PlanOptions(2, True, 1, use_graph=False)
+        F = dm.cholesky_(DA, uplo=uplo, options=opts)
+        check(O.rel_factor_diff(DA.collect(), ref, uplo) < 1e-12, ("capped cholesky", uplo))
+        DB = dm.DArray.from_array(gen.randn(n * 70, 27).reshape(70, n).T.copy(), dm.Blocks(nb, nb))
+        b = DB.collect()
+        dm.ldiv_(F, DB, options=opts)
+        check(O.backward_error_columns(A, DB.collect(), b) < 10 * n * np.finfo(float).eps,
+              ("capped ldiv", uplo))
+
+
+GROUPS = dict(tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
+              capped=capped)
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(GROUPS)

@@ -51,17 +51,18 @@ template <typename T> long gemm_prepare_jobs(GemmJob<T>* jobs, int nj, GemmCfgId
   long t = 0; for (int i = 0; i < nj; ++i) { jobs[i].tiles_m = (jobs[i].M + 127) / 128; jobs[i].tiles_n = (jobs[i].N + 127) / 128; jobs[i].wg_begin = (int)t; t += (long)jobs[i].tiles_m * jobs[i].tiles_n; } return t; }
 template <typename T> bool gemm_job_vec_ok(bool, bool, const GemmJob<T>&) { return true; }
 static long g_launch = 0;
-template <typename T> int gemm_launch(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int gemm_launch(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t, long max_wg) { if (max_wg < 0) abort(); ++g_launch; return 0; }
+int gemm_wgs_per_cu(GemmCfgId c) { return c.id == 0 ? 1 : 2; }
 template <typename T> int gemm_single(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t) { ++g_launch; return 0; }
 template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t) { if (!inv) abort(); ++g_launch; return 0; }
-template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long max_wg) { if (max_wg < 0) abort(); ++g_launch; return 0; }
 template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t) { if (!ws) abort(); ++g_launch; return 0; }
 template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T* ws, hipStream_t) { if (!ws) { fprintf(stderr, "potrf without workspace\n"); abort(); } ++g_launch; return 0; }
 bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
   return ta && !tb && !((uintptr_t)j.A & 15) && !((uintptr_t)j.B & 15) && !(j.lda & 3) &&
          !(j.ldb & 3);
 }
-int gemm_split_launch(int terms, const GemmJob<float>*, int, const GemmJob<float>&, long,
+int gemm_split_launch(int terms, const GemmJob<float>*, int, const GemmJob<float>&, long, long,
                       const int*, hipStream_t) {
   if (terms != 3 && terms != 6) abort();
   ++g_launch;
@@ -74,10 +75,10 @@ int copy_i32_launch(int*, const int*, size_t, hipStream_t) { ++g_launch; return 
   template GemmCfgId gemm_pick_cfg<T>(int, int, int, int); \
   template long gemm_prepare_jobs<T>(GemmJob<T>*, int, GemmCfgId); \
   template bool gemm_job_vec_ok<T>(bool, bool, const GemmJob<T>&); \
-  template int gemm_launch<T>(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t); \
+  template int gemm_launch<T>(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t, long); \
   template int gemm_single<T>(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t); \
   template int trinv_blocks_launch<T>(const TrsmShape&, const T*, int, int, T*, const int*, hipStream_t); \
-  template int trsm_launch<T>(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t); \
+  template int trsm_launch<T>(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long); \
   template int trsm_full<T>(char, char, char, char, int, int, T, const T*, int, T*, int, T*, const int*, hipStream_t); \
   template int potrf_full<T>(bool, int, T*, int, int*, int*, int, T*, hipStream_t);
 INST(double)
