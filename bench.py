@@ -298,7 +298,7 @@ def main():
                          f"N={n}, Blocks({nb},{nb}), A=G*G^T/N+I (Philox seed {a.seed})"),
             "n": n, "nb": nb, "nrhs": nrhs,
             "parallelism": ("1 GPU: 2 HIP streams, look-ahead 1, eager issue; bulk launches "
-                            "capped at 7/8 of the CUs, the rest kept for the critical path"),
+                            "leave 1/16 of the CUs to the critical path"),
         },
         "cholesky_ms": round(chol_ms, 3),
         "ldiv_ms": round(solve_ms, 3),

@@ -24,6 +24,7 @@
 // diagonal POTRF, the panel TRSM and the next panel's update run on the
 // critical stream while the rest of the trailing update runs beside them.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -270,6 +271,9 @@ int Dag::create_streams(bool eager) {
   int dev = 0;
   DM_HIP(hipGetDevice(&dev));
   DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // CUs kept free of bulk launches: DM_CRIT_CUS, else 1/16 of the device
+  crit_cus = cus / 16;
+  if (const char* e = std::getenv("DM_CRIT_CUS")) crit_cus = std::atoi(e);
   for (auto& st : streams) DM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   events.resize(groups.size());
@@ -472,11 +476,12 @@ int Dag::build_device_tables() {
 template <typename T>
 int Dag::launch_group_t(const Group& G, hipStream_t s) {
   const int* ab = abort_dev;
-  // Bulk (non-critical) launches of a look-ahead schedule use at most 7/8 of
-  // the CUs, so the critical stream's small launches always find free CUs
-  // instead of queueing behind milliseconds of trailing update.
-  const bool cap = opts.lookahead > 0 && opts.num_streams >= 2 && G.crit != 0 && cus > 0;
-  const long bulk_cus = cus - cus / 8;
+  // Bulk (non-critical) launches of a look-ahead schedule leave crit_cus CUs
+  // free, so the critical stream's small launches find free CUs instead of
+  // queueing behind milliseconds of trailing update.
+  const bool cap = opts.lookahead > 0 && opts.num_streams >= 2 && G.crit != 0 && cus > 0 &&
+                   crit_cus > 0 && crit_cus < cus;
+  const long bulk_cus = cus - crit_cus;
   if (G.op == DM_OP_GEMM) {
     if (G.njobs == 0) return 0;
     GemmJob<T> dummy{};

@@ -269,30 +269,44 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __
   typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC> G;
   __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
   if (abort && *abort) return;
+  constexpr bool packed = BM == BN;  // SYRK jobs enumerate only triangle tiles
   for (TileRange r = tile_range(total); r.t < r.end; r.t += r.step) {
     GemmJob<T> job;
     int tm, tn;
-    locate_tile(jobs, njobs, single, r.t, job, tm, tn);
+    locate_tile(jobs, njobs, single, r.t, job, tm, tn, packed);
+    if (!packed && job.tri) {  // rectangular tiles: skip those outside the triangle
+      const int m0 = tm * BM, n0 = tn * BN;
+      if (job.tri == 1 ? m0 > n0 + BN - 1 : n0 > m0 + BM - 1) continue;
+    }
     G::run(job, tm, tn, smem);
     __syncthreads();  // the next tile's prologue rewrites LDS
   }
 }
 
 // ------------------------------------------------------------ host side
+// Configurations (FP64; measured on the k-contiguous Cholesky update shape):
+//   id 4: 256x128 tile, 16 waves of 64x32, BK 16 (110 KB LDS) -- big launches
+//   id 0: 128x128 tile, 16 waves of 32x32, BK 32 (139 KB LDS)
+//   id 1:  64x64  tile,  4 waves of 32x32, BK 16 -- small launches
+// FP32 keeps 128x128 with 4 waves of 64x64 (id 0) and 64x64 (id 1).
+// 16-wave workgroups put four waves on each SIMD, which hides LDS and
+// global latency that a single 64x64-per-wave workgroup leaves exposed.
 namespace {
-struct CfgDesc { int bm, bn; };
-// 0: 128x128 (2x2 waves of 64x64), 1: 64x64 (2x2 waves of 32x32)
-const CfgDesc kCfgs[] = {{128, 128}, {64, 64}};
+constexpr int GEMM_CFG_256x128 = 4;
 }
 
 template <typename T>
 GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs) {
   (void)K;
-  // Prefer the big tile when the launch has enough workgroups to fill the
-  // 256 CUs; otherwise trade reuse for parallelism.
-  const long big = (long)((M + 127) / 128) * ((N + 127) / 128) * (njobs > 0 ? njobs : 1);
-  const int id = big >= 192 ? 0 : 1;
-  return GemmCfgId{id, kCfgs[id].bm, kCfgs[id].bn};
+  const long nj = njobs > 0 ? njobs : 1;
+  if (sizeof(T) == 8) {
+    // Largest tile whose launch still puts about one workgroup on every CU.
+    const long t256 = (long)((M + 255) / 256) * ((N + 127) / 128) * nj;
+    if (t256 >= 256) return GemmCfgId{GEMM_CFG_256x128, 256, 128};
+  }
+  const long t128 = (long)((M + 127) / 128) * ((N + 127) / 128) * nj;
+  if (t128 >= 192) return GemmCfgId{0, 128, 128};
+  return GemmCfgId{1, 64, 64};
 }
 
 template <typename T>
@@ -304,8 +318,8 @@ long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg) {
     j.tiles_n = (j.N + cfg.bn - 1) / cfg.bn;
     j.wg_begin = (int)total;
     long cnt;
-    if (j.tri == 0) cnt = (long)j.tiles_m * j.tiles_n;
-    else cnt = (long)j.tiles_m * (j.tiles_m + 1) / 2;  // square (M == N)
+    if (j.tri == 0 || cfg.bm != cfg.bn) cnt = (long)j.tiles_m * j.tiles_n;
+    else cnt = (long)j.tiles_m * (j.tiles_m + 1) / 2;  // square tiles of a square (M == N) C
     if (j.M <= 0 || j.N <= 0) cnt = 0;
     total += cnt;
   }
@@ -313,10 +327,10 @@ long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg) {
 }
 
 namespace {
-template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int VEC>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
 int launch_one(const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total, long grid,
                const int* abort, hipStream_t s) {
-  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16, WM, WN, TA, TB, VEC>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TA, TB, VEC>), dim3((unsigned)grid),
                      dim3(64 * WM * WN), 0, s, jobs, njobs, single, (int)total, abort);
   DM_LAUNCH_CHECK("gemm_kernel");
   return 0;
@@ -324,9 +338,21 @@ int launch_one(const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long
 template <typename T, bool TA, bool TB, int VEC>
 int launch_cfg(int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total,
                long grid, const int* abort, hipStream_t s) {
-  if (id == 0)
-    return launch_one<T, 128, 128, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
-  return launch_one<T, 64, 64, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
+  if constexpr (sizeof(T) == 8) {
+    if (id == GEMM_CFG_256x128)
+      return launch_one<T, 256, 128, 16, 4, 4, TA, TB, VEC>(jobs, njobs, single, total, grid, abort,
+                                                            s);
+    if (id == 0)
+      return launch_one<T, 128, 128, 32, 4, 4, TA, TB, VEC>(jobs, njobs, single, total, grid, abort,
+                                                            s);
+  } else {
+    if (id == 0)
+      return launch_one<T, 128, 128, 16, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort,
+                                                            s);
+  }
+  if (id == 1)
+    return launch_one<T, 64, 64, 16, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
+  return arg_fail(0, "gemm: unknown configuration");
 }
 template <typename T, int VEC>
 int launch_tt(bool ta, bool tb, int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single,
@@ -370,9 +396,9 @@ template <typename T>
 bool gemm_job_vec_ok(bool ta, bool tb, const GemmJob<T>& j) { return job_vec_ok(ta, tb, j); }
 
 int gemm_wgs_per_cu(GemmCfgId cfg) {
-  // 128x128 FP64: ~350 of the 512 registers per SIMD lane -> one workgroup;
-  // 64x64 and the BF16-split kernel fit two.
-  return cfg.id == 0 ? 1 : 2;
+  // the 128x128 and 256x128 workgroups fill a CU's LDS (or, FP32 128x128,
+  // its registers); 64x64 and the BF16-split kernel fit two
+  return (cfg.id == 0 || cfg.id == GEMM_CFG_256x128) ? 1 : 2;
 }
 
 template <typename T>

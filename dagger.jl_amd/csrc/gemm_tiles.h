@@ -35,10 +35,13 @@ __device__ __forceinline__ TileRange tile_range(int total) {
 
 // Finds the job and the C tile (tm, tn) of logical tile `wg` in a batched
 // launch (binary search over wg_begin) or a single-job launch (njobs == 0).
+// packed: SYRK (tri) jobs enumerate only the tiles of their triangle (square
+// tiles); otherwise every tile of a tri job is enumerated and the caller
+// skips those outside the triangle.
 template <typename T>
 __device__ __forceinline__ void locate_tile(const GemmJob<T>* __restrict__ jobs, int njobs,
                                             const GemmJob<T>& single, int wg, GemmJob<T>& job,
-                                            int& tm, int& tn) {
+                                            int& tm, int& tn, bool packed = true) {
   int local;
   if (njobs == 0) {
     job = single;
@@ -52,7 +55,7 @@ __device__ __forceinline__ void locate_tile(const GemmJob<T>* __restrict__ jobs,
     job = jobs[lo];
     local = wg - job.wg_begin;
   }
-  if (job.tri == 0) {
+  if (job.tri == 0 || !packed) {
     tm = local % job.tiles_m;
     tn = local / job.tiles_m;
   } else if (job.tri == 1) {

@@ -199,9 +199,9 @@ typedef struct dm_plan_opts {
     int batch;           /* 1: batch independent same-shape tasks per launch */
     int lookahead;       /* slack (in levels) below which tasks go on the
                             critical stream; 0 disables the split.  Bulk
-                            (non-critical) GEMM/TRSM launches then run on
-                            at most 7/8 of the CUs, leaving the rest free
-                            for the critical path */
+                            (non-critical) GEMM/TRSM launches then leave
+                            1/16 of the CUs (environment DM_CRIT_CUS: a
+                            count) free for the critical path */
     int use_graph;       /* 1: capture the schedule into a hipGraph (plans
                             with at most 2 streams and at most 2e11 flops;
                             other plans issue eagerly, and

@@ -74,6 +74,25 @@ def tiles():
     dA, dB, dC = Dev(A), Dev(B), Dev(np.zeros((m, n)))
     call("dm_dgemm", b"N", b"N", m, n, k, 1.0, dA.p, m, dB.p, k, 0.0, dC.p, m, None)
     check(relerr(dC.get(), A @ B) < 1e-13, "dgemm big tiles")
+    # 256x128 workgroup tiles (>= 256 of them), plain and as a SYRK whose
+    # rectangular tiles outside the triangle are skipped
+    m, n, k = 4096, 2048, 4
+    A, B = rng.standard_normal((k, m)), rng.standard_normal((k, n))
+    dA, dB, dC = Dev(A), Dev(B), Dev(np.zeros((m, n)))
+    call("dm_dgemm", b"T", b"N", m, n, k, 1.0, dA.p, k, dB.p, k, 0.0, dC.p, m, None)
+    check(relerr(dC.get(), A.T @ B) < 1e-13, "dgemm 256x128 tiles")
+    for uplo in "UL":
+        n = 4096
+        A = rng.standard_normal((k, n))
+        Cm = rng.standard_normal((n, n))
+        dA, dC = Dev(A), Dev(Cm)
+        call("dm_dsyrk", uplo.encode(), b"T", n, k, -1.0, dA.p, k, 1.0, dC.p, n, None)
+        got = dC.get()
+        tri = np.triu if uplo == "U" else np.tril
+        other = np.tril if uplo == "U" else np.triu
+        off = -1 if uplo == "U" else 1
+        check(relerr(tri(got), tri(Cm - A.T @ A)) < 1e-13 and
+              np.array_equal(other(got, off), other(Cm, off)), ("dsyrk 256x128 tiles", uplo))
     m, n, k, ld = 101, 67, 55, 103
     A, B = rng.standard_normal((ld, k)), rng.standard_normal((ld, n))
     dA, dB, dC = Dev(A), Dev(B), Dev(np.zeros((ld, n)))
