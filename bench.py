@@ -297,8 +297,7 @@ def main():
             "workload": (f"plain FP64 tiled Cholesky (cholesky!, U) + ldiv! with {nrhs} RHS, "
                          f"N={n}, Blocks({nb},{nb}), A=G*G^T/N+I (Philox seed {a.seed})"),
             "n": n, "nb": nb, "nrhs": nrhs,
-            "parallelism": ("1 GPU: 2 HIP streams, look-ahead 1, eager issue; bulk launches "
-                            "leave 1/16 of the CUs to the critical path"),
+            "parallelism": "1 GPU: 2 HIP streams (critical path + bulk update), look-ahead 1, eager issue",
         },
         "cholesky_ms": round(chol_ms, 3),
         "ldiv_ms": round(solve_ms, 3),

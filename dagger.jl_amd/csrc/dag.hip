@@ -271,8 +271,10 @@ int Dag::create_streams(bool eager) {
   int dev = 0;
   DM_HIP(hipGetDevice(&dev));
   DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  // CUs kept free of bulk launches: DM_CRIT_CUS, else 1/16 of the device
-  crit_cus = cus / 16;
+  // CUs kept free of bulk launches (environment DM_CRIT_CUS; none by
+  // default: measured at N = 32768, reserving 16 or 32 CUs cost more bulk
+  // throughput than it saved on the critical path)
+  crit_cus = 0;
   if (const char* e = std::getenv("DM_CRIT_CUS")) crit_cus = std::atoi(e);
   for (auto& st : streams) DM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
