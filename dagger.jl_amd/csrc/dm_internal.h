@@ -150,6 +150,7 @@ TrsmShape trsm_shape(char side, char uplo, char transa, char diag);
 
 constexpr int TRSM_BS = 32;     // diagonal block inverted up front
 constexpr int TRSM_LEAF = 256;  // longest solve held in LDS (longer ones recurse)
+int trsm_leaf_len();            // leaf length in use (TRSM_LEAF unless tuned down)
 constexpr int TRSM_VPW = 64;    // vectors per workgroup of the leaf kernel (16 per wave)
 // Workspace (elements) for any trsm_full / potrf_full call: the inverted
 // diagonal blocks of one leaf solve.

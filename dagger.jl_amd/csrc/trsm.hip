@@ -151,6 +151,24 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
     const int ra = r0 + lr, rb = r0 + 16 + lr;
     const bool raok = ra < L, rbok = rb < L;
     int st = 0;
+    // four k-steps per iteration: eight Tm loads in flight per lane (a leaf
+    // workgroup runs one wave per SIMD, so latency is hidden by ILP only)
+    for (; st + 4 <= nsteps; st += 4) {
+      T ta[4], tb[4], xv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = kbeg + 4 * (st + q) + lk;
+        const bool ok = k < kend;
+        ta[q] = (ok && raok) ? tm_at<T, TTRANS>(A, lda, ra, k) : T(0);
+        tb[q] = (ok && rbok) ? tm_at<T, TTRANS>(A, lda, rb, k) : T(0);
+        xv[q] = ok ? X[k * 16 + lr] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc[0][q & 1] = MfmaT<T>::mac(ta[q], xv[q], acc[0][q & 1]);
+        acc[1][q & 1] = MfmaT<T>::mac(tb[q], xv[q], acc[1][q & 1]);
+      }
+    }
     for (; st + 2 <= nsteps; st += 2) {
       const int k0 = kbeg + 4 * st + lk, k1 = k0 + 4;
       const bool ok0 = k0 < kend, ok1 = k1 < kend;
@@ -306,7 +324,7 @@ template <typename T>
 static int trsm_eff(const TrsmShape& sh, int L, int NV, T alpha, const T* A, int lda, T* B,
                     int ldb, T* ws, const int* abort, hipStream_t s) {
   if (L <= 0 || NV <= 0) return 0;
-  if (L <= TRSM_LEAF) {
+  if (L <= trsm_leaf_len()) {
     if (!ws) return arg_fail(0, "trsm: no workspace");
     T* inv = ws;
     int rc = trinv_blocks_launch<T>(sh, A, lda, L, inv, abort, s);

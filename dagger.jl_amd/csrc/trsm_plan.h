@@ -17,7 +17,7 @@ inline T* vsub(const TrsmShape& sh, T* B, int ldb, int r) {
   return sh.side_l ? B + r : B + (size_t)r * ldb;
 }
 
-// Split of a solve of length L > TRSM_LEAF at P (a multiple of TRSM_BS from
+// Split of a solve of length L > trsm_leaf_len() at P (a multiple of TRSM_BS from
 // the top of the triangle): rows [0, P) and [P, L).  Lower triangles solve
 // [0, P) first, upper ones [P, L) first.
 struct TrsmSplit {

@@ -1,10 +1,21 @@
 // Host planning of batched recursive triangular solves (no device code):
 // the effective-solve shape of a BLAS trsm call and the leaf/GEMM launch
 // sequence of a group of solves.  Used by the datadeps executor.
+#include <cstdlib>
 #include <map>
 #include "trsm_plan.h"
 
 namespace dm {
+
+int trsm_leaf_len() {
+  // environment DM_TRSM_LEAF: a multiple of 32 in [64, TRSM_LEAF] (tuning)
+  static const int len = [] {
+    const char* e = std::getenv("DM_TRSM_LEAF");
+    const int v = e ? std::atoi(e) : TRSM_LEAF;
+    return (v >= 64 && v <= TRSM_LEAF && v % TRSM_BS == 0) ? v : TRSM_LEAF;
+  }();
+  return len;
+}
 
 TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
   TrsmShape s;
@@ -28,7 +39,7 @@ template <typename T>
 static void plan_rec(const TrsmShape& sh, int L, const std::vector<TrsmJob<T>>& jobs,
                      std::vector<TrsmStep<T>>& steps) {
   if (jobs.empty() || L <= 0) return;
-  if (L <= TRSM_LEAF) {
+  if (L <= trsm_leaf_len()) {
     TrsmStep<T> st;
     st.leaf = true;
     st.maxL = L;
