@@ -121,13 +121,18 @@ def mixed_leg(dm, n, nb, kappa, nrhs, seed):
         dm.ldiv_(F, DB)
         res["fp64_ms"] = timer.stop_ms()
     X64 = DB.collect()
-    # mixed: FP32 factor + IR
+    # mixed: FP32 factor + IR; trailing updates on FP32 MFMA (update 0) and
+    # on BF16 MFMA with 3-term operand splitting (update 6, the default)
     Ff = dm.DArray(np.float32, (n, n), blocks)
-    for rep in range(2):
-        timer.start()
-        M = mixed.MixedCholesky(A0, F=Ff)
-        X, info = M.solve(B0, tol=1e-13, max_iter=30)
-        res["mixed_ms"] = timer.stop_ms()
+    for update in (0, 6):
+        opts = mixed._OPTS.replace(f32_update=update)
+        for rep in range(2):
+            timer.start()
+            M = mixed.MixedCholesky(A0, options=opts, F=Ff)
+            X, info = M.solve(B0, tol=1e-13, max_iter=30)
+            ms = timer.stop_ms()
+        res[f"mixed_ms_update{update}"] = ms
+    res["mixed_ms"] = ms  # the default (update 6) run
     Xm = X.collect()
     # backward error against the closed form A x = s*x - U (W^T x), host side
     b = B0.collect()
@@ -136,6 +141,7 @@ def mixed_leg(dm, n, nb, kappa, nrhs, seed):
                                              np.abs(b).max(0))).max())
     res.update(kappa=kappa, n=n, nb=nb, nrhs=nrhs, closed_form_berr=closed_berr,
                speedup=round(res["fp64_ms"] / res["mixed_ms"], 3),
+               speedup_fp32_update=round(res["fp64_ms"] / res["mixed_ms_update0"], 3),
                mixed_berr=float(np.max(info["berr"])), mixed_iters=info["iterations"],
                method=info["method"],
                max_rel_diff_vs_fp64=float(np.abs(Xm - X64).max() / np.abs(X64).max()))
@@ -198,7 +204,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=12288,
                     help="N of the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--mixed-n", type=int, default=16384,
+    ap.add_argument("--mixed-n", type=int, default=32768,
                     help="N of the mixed-precision leg (0 disables it)")
     ap.add_argument("--serving-requests", type=int, default=120,
                     help="requests in the serving leg (0 disables it)")

@@ -8,11 +8,14 @@
 namespace dm {
 
 int trsm_leaf_len() {
-  // environment DM_TRSM_LEAF: a multiple of 32 in [64, TRSM_LEAF] (tuning)
+  // 128 by default: leaves of 128 rows need half the LDS of 256-row leaves,
+  // so two leaf workgroups share a CU (measured at N = 32768: ldiv! 33.5 ms
+  // against 36.6 ms, cholesky! unchanged).  Environment DM_TRSM_LEAF: a
+  // multiple of 32 in [64, TRSM_LEAF].
   static const int len = [] {
     const char* e = std::getenv("DM_TRSM_LEAF");
-    const int v = e ? std::atoi(e) : TRSM_LEAF;
-    return (v >= 64 && v <= TRSM_LEAF && v % TRSM_BS == 0) ? v : TRSM_LEAF;
+    const int v = e ? std::atoi(e) : 128;
+    return (v >= 64 && v <= TRSM_LEAF && v % TRSM_BS == 0) ? v : 128;
   }();
   return len;
 }

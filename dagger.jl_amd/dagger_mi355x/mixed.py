@@ -1,8 +1,9 @@
 """Precision form: low-precision tiled Cholesky + FP64 iterative refinement.
 
 Beside the plain FP64 drop-in.  The factorization runs the same tile DAG
-(src/array/cholesky.jl:38-53) on an FP32 copy of A, so TRSM/SYRK/GEMM run on
-FP32 MFMA; the residual r = b - A x, its norms and the update x += d stay in
+(src/array/cholesky.jl:38-53) on an FP32 copy of A: TRSM and POTRF in FP32,
+the SYRK/GEMM trailing updates on the BF16 matrix cores with 3-term operand
+splitting (FP32-accurate); the residual r = b - A x, its norms and the update x += d stay in
 FP64 on the untouched FP64 A (LAPACK dsposv's structure, on resident tiles):
 
     x0 = F32^-1 b
@@ -29,7 +30,9 @@ from .darray import DArray, Blocks, DeviceBuffer, DeviceInt
 from .datadeps import PlanOptions
 from .linalg import PosDefException
 
-_OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True)
+# Trailing updates of the FP32 factor on the BF16 matrix cores with 3-term
+# operand splitting (FP32-accurate, several times the FP32 MFMA rate).
+_OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True, f32_update=6)
 
 
 def _convert(dst, src, stream=None):
