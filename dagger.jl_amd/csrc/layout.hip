@@ -73,6 +73,40 @@ __global__ void add_diag_kernel(double* __restrict__ A, int64_t ld, int64_t n,
 }
 }  // namespace dm
 
+namespace dm {
+// Tile k of a one-tile-column matrix (m x nb, Blocks(nb, nb)) := [I_r 0],
+// r = rows of tile k (its ld).
+template <typename T>
+__global__ void identity_tiles_kernel(T* __restrict__ D, int64_t m, int nb) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * nb) return;
+  const int64_t k = e / ((int64_t)nb * nb), w = e - k * nb * nb;
+  const int64_t r = (k * nb + nb <= m) ? nb : m - k * nb;  // rows (= ld) of tile k
+  const int64_t j = w / r, i = w - j * r;
+  if (j < nb) D[e] = (i == j) ? T(1) : T(0);
+}
+}  // namespace dm
+
+extern "C" int dm_tiled_identity(const dm_tiled* D, dm_stream_t s) {
+  if (!D) return arg_fail(1, "D is NULL");
+  if (D->n != D->nb || D->mb != D->nb) return arg_fail(1, "D must be m x nb with Blocks(nb, nb)");
+  int rc = ensure_init();
+  if (rc) return rc;
+  const int64_t count = D->m * D->nb;
+  if (count <= 0) return 0;
+  const unsigned grid = (unsigned)((count + 255) / 256);
+  if (D->dtype == DM_F64)
+    hipLaunchKernelGGL((identity_tiles_kernel<double>), dim3(grid), dim3(256), 0, stream_of(s),
+                       (double*)D->base, D->m, D->nb);
+  else if (D->dtype == DM_F32)
+    hipLaunchKernelGGL((identity_tiles_kernel<float>), dim3(grid), dim3(256), 0, stream_of(s),
+                       (float*)D->base, D->m, D->nb);
+  else
+    return arg_fail(1, "unsupported dtype");
+  DM_LAUNCH_CHECK("identity_tiles_kernel");
+  return 0;
+}
+
 // A[i,i] += d[i] on a full column-major FP64 matrix (generator helper).
 extern "C" int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_dev,
                              dm_stream_t s) {

@@ -37,22 +37,39 @@ __global__ void axpy_s2d_kernel(double* __restrict__ x, const float* __restrict_
   if (i < count) x[i] += (double)d[i];
 }
 
-// Row sums of |A| for a tile-packed m x n matrix: one thread per row, loop
-// over the tiles of that row (consecutive threads read consecutive rows of a
-// tile, so loads are coalesced).  rowsum[i] = sum_j |A_ij|.
-__global__ void row_abs_sum_kernel(const double* __restrict__ A, int64_t m, int64_t n, int mb,
-                                   int nb, double* __restrict__ rowsum) {
+// Partial row sums of |A| for a tile-packed m x n matrix: workgroup
+// (blockIdx.x, blockIdx.y) sums the 256 rows of row block blockIdx.x over the
+// columns of column tile blockIdx.y (consecutive threads read consecutive
+// rows of a tile: coalesced).  part[tj * m + i] = sum_{j in tile tj} |A_ij|.
+__global__ __launch_bounds__(256) void row_abs_part_kernel(const double* __restrict__ A,
+                                                           int64_t m, int64_t n, int mb, int nb,
+                                                           double* __restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tj = blockIdx.y;
   if (i >= m) return;
   const int64_t ti = i / mb, ii = i - ti * mb;
   const int64_t rows = (ti * mb + mb <= m) ? mb : m - ti * mb;
-  double s = 0.0;
-  const int64_t nt = (n + nb - 1) / nb;
-  for (int64_t tj = 0; tj < nt; ++tj) {
-    const int64_t cols = (tj * nb + nb <= n) ? nb : n - tj * nb;
-    const double* t = A + tj * nb * m + ti * mb * cols;
-    for (int64_t jj = 0; jj < cols; ++jj) s += fabs(t[jj * rows + ii]);
+  const int64_t cols = (tj * nb + nb <= n) ? nb : n - tj * nb;
+  const double* t = A + tj * nb * m + ti * mb * cols;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t jj = 0;
+  for (; jj + 4 <= cols; jj += 4) {
+    s0 += fabs(t[(jj + 0) * rows + ii]);
+    s1 += fabs(t[(jj + 1) * rows + ii]);
+    s2 += fabs(t[(jj + 2) * rows + ii]);
+    s3 += fabs(t[(jj + 3) * rows + ii]);
   }
+  for (; jj < cols; ++jj) s0 += fabs(t[jj * rows + ii]);
+  part[tj * m + i] = (s0 + s1) + (s2 + s3);
+}
+
+// rowsum[i] = sum over column tiles of part (fixed order).
+__global__ void row_abs_sum_kernel(const double* __restrict__ part, int64_t m, int64_t nt,
+                                   double* __restrict__ rowsum) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int64_t tj = 0; tj < nt; ++tj) s += part[tj * m + i];
   rowsum[i] = s;
 }
 
@@ -143,8 +160,13 @@ int dm_tiled_norm_inf(const dm_tiled* A, double* out_dev, double* work_dev, dm_s
   if (rc) return rc;
   hipStream_t st = stream_of(s);
   if (A->m <= 0 || A->n <= 0) return fill_i32_launch((int*)out_dev, 0, 2, st);
+  const int64_t nt = (A->n + A->nb - 1) / A->nb;
+  double* part = work_dev + A->m;  // work: m row sums, then nt * m partial sums
+  hipLaunchKernelGGL(row_abs_part_kernel, dim3(nblocks((size_t)A->m), (unsigned)nt), dim3(256), 0,
+                     st, (const double*)A->base, A->m, A->n, A->mb, A->nb, part);
+  DM_LAUNCH_CHECK("row_abs_part_kernel");
   hipLaunchKernelGGL(row_abs_sum_kernel, dim3(nblocks((size_t)A->m)), dim3(256), 0, st,
-                     (const double*)A->base, A->m, A->n, A->mb, A->nb, work_dev);
+                     (const double*)part, A->m, nt, work_dev);
   DM_LAUNCH_CHECK("row_abs_sum_kernel");
   hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, st, (const double*)work_dev, A->m,
                      out_dev);

@@ -122,6 +122,7 @@ _PROTOS = {
                             C.POINTER(dm_tiled), C.POINTER(dm_plan_opts), vp]),
     "dm_tiled_convert": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp]),
     "dm_tiled_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp, vp]),
+    "dm_tiled_identity": (i32, [C.POINTER(dm_tiled), vp]),
     "dm_tiled_col_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp]),
     "dm_tiled_axpy_s2d": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp]),
     "dm_add_diag_d": (i32, [vp, i64, i64, vp, vp]),

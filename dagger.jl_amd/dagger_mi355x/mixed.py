@@ -26,13 +26,73 @@ import numpy as np
 
 from . import _lib
 from ._lib import call
-from .darray import DArray, Blocks, DeviceBuffer, DeviceInt
-from .datadeps import PlanOptions
+from .darray import DArray, Blocks, DeviceBuffer, DeviceInt, Tile
+from .datadeps import PlanOptions, Dag, make_task, BLAS, In, Out, InOut
 from .linalg import PosDefException
 
 # Trailing updates of the FP32 factor on the BF16 matrix cores with 3-term
 # operand splitting (FP32-accurate, several times the FP32 MFMA rate).
 _OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True, f32_update=6)
+
+
+def _square(t):
+    """The leading rows x rows part of a tile (ld unchanged)."""
+    return Tile(t.ptr, t.rows, t.rows, t.ld, t.dtype, owner=None, index=None)
+
+
+class _DiagInverseSolve:
+    """F32^-1 R for the FP32 factor F = U (U^T U = A32), in place, as two
+    blocked substitutions whose diagonal steps are products with the
+    inverted diagonal tiles U_kk^-1: every launch is a GEMM, batched per step
+    by the executor, so a solve with few right-hand sides is a short chain of
+    bandwidth-bound launches instead of 2 * mt dependent leaf solves.
+    (The refinement loop's own solver; the plain drop-in's ldiv! keeps
+    Dagger's trsm! loop nest.)  Rounding differs from substitution by the
+    inverse's conditioning, which refinement absorbs like the FP32 factor's
+    own error."""
+
+    def __init__(self, F, options, stream):
+        self.F, self.options, self.stream = F, options, stream
+        nb = F.mb
+        self.Dinv = DArray(np.float32, (F.m, nb), Blocks(nb, nb))
+        tasks = [make_task(BLAS.trsm, ("L", "U", "N", "N", 1.0, In(F.tile(k, k)),
+                                        InOut(_square(self.Dinv.tile(k, 0)))))
+                 for k in range(F.mt)]
+        self._inv = Dag(tasks).plan(options.replace(use_graph=False))
+        self._solves = {}
+
+    def invert(self):
+        """Dinv_k := U_kk^-1 for every diagonal tile (after each refactor)."""
+        t = self.Dinv.tiled()
+        call("dm_tiled_identity", C.byref(t), self.stream)
+        self._inv.run(self.stream, sync=False)
+
+    def _dag(self, R, Y):
+        key = (R.buffer.ptr, Y.buffer.ptr)
+        d = self._solves.get(key)
+        if d is None:
+            F, mt = self.F, self.F.mt
+            D = [_square(self.Dinv.tile(k, 0)) for k in range(mt)]
+            ts = []
+            for k in range(mt):  # U^T Y = R: Y_k = U_kk^-T R_k, R_j -= U_kj^T Y_k
+                ts.append(make_task(BLAS.gemm, ("T", "N", 1.0, In(D[k]), In(R.tile(k, 0)), 0.0,
+                                                Out(Y.tile(k, 0)))))
+                for j in range(k + 1, mt):
+                    ts.append(make_task(BLAS.gemm, ("T", "N", -1.0, In(F.tile(k, j)),
+                                                    In(Y.tile(k, 0)), 1.0, InOut(R.tile(j, 0)))))
+            for k in reversed(range(mt)):  # U X = Y: X_k = U_kk^-1 Y_k, Y_i -= U_ik X_k
+                ts.append(make_task(BLAS.gemm, ("N", "N", 1.0, In(D[k]), In(Y.tile(k, 0)), 0.0,
+                                                Out(R.tile(k, 0)))))
+                for i in range(k):
+                    ts.append(make_task(BLAS.gemm, ("N", "N", -1.0, In(F.tile(i, k)),
+                                                    In(R.tile(k, 0)), 1.0, InOut(Y.tile(i, 0)))))
+            d = Dag(ts).plan(self.options)
+            self._solves[key] = d
+        return d
+
+    def solve(self, R, Y):
+        """R := F32^-1 R (Y: scratch of R's shape)."""
+        self._dag(R, Y).run(self.stream, sync=False)
 
 
 def _convert(dst, src, stream=None):
@@ -68,11 +128,17 @@ class MixedCholesky:
         self.F = F if F is not None else DArray(np.float32, A64.shape, A64.partitioning)
         if getattr(self.F, "_mixed_aux", None) is None:
             self.F._mixed_aux = dict(info=DeviceInt(1), ginfo=DeviceInt(1),
-                                     work=DeviceBuffer(8 * max(A64.m, 2)), norm=DeviceBuffer(16),
+                                     work=DeviceBuffer(8 * max(A64.m, 2) * (1 + -(-A64.n // bs))),
+                                     norm=DeviceBuffer(16),
                                      bufs={})
         aux = self.F._mixed_aux
         self.info, self.ginfo = aux["info"], aux["ginfo"]
         self._work, self._norm, self._bufs = aux["work"], aux["norm"], aux["bufs"]
+        if aux.get("isolve") is None:
+            aux["isolve"] = _DiagInverseSolve(self.F, self.options.replace(f32_update=0),
+                                              self.stream)
+        self._isolve = aux["isolve"]
+        self._isolve.stream = self.stream
         self.refactor()
 
     def refactor(self):
@@ -86,6 +152,7 @@ class MixedCholesky:
         info = int(self.info.get(s)[0])
         if info != 0:
             raise PosDefException(info)
+        self._isolve.invert()
         ta = self.A.tiled()
         call("dm_tiled_norm_inf", C.byref(ta), C.c_void_p(self._norm.ptr),
              C.c_void_p(self._work.ptr), s)
@@ -102,15 +169,15 @@ class MixedCholesky:
             b = dict(X=DArray(np.float64, (self.A.m, nrhs), blk),
                      R=DArray(np.float64, (self.A.m, nrhs), blk),
                      D=DArray(np.float32, (self.A.m, nrhs), blk),
+                     Y=DArray(np.float32, (self.A.m, nrhs), blk),
                      nrm=DeviceBuffer(8 * max(nrhs, 1)))
             self._bufs[nrhs] = b
         return b
 
     def _solve32(self, D):
-        """D := F32^-1 D (two FP32 tiled triangular solves)."""
-        tf, td = self.F.tiled(), D.tiled()
-        o = self.options.c()
-        call("dm_potrs_tiled", b"U", C.byref(tf), C.byref(td), C.byref(o), self.stream)
+        """D := F32^-1 D (FP32; D one of the per-nrhs buffers)."""
+        buf = self._buffers(D.n)
+        self._isolve.solve(D, buf["Y"])
 
     def residual(self, B, X, R):
         """R = B - A X in FP64."""
@@ -186,7 +253,7 @@ class MixedCholesky:
         blk = Blocks(self.nb, self.nb)
         v64 = DArray(np.float64, (n, 1), blk)
         w64 = DArray(np.float64, (n, 1), blk)
-        d32 = DArray(np.float32, (n, 1), blk)
+        d32 = self._buffers(1)["D"]
         x_all = X.collect(s)
         b_all = B.collect(s)
 

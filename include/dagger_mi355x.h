@@ -315,7 +315,8 @@ int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A,
 /* dst = (dst dtype) src, elementwise over the packed slab (same shape and
  * blocks, FP64 <-> FP32). */
 int dm_tiled_convert(const dm_tiled* dst, const dm_tiled* src, dm_stream_t s);
-/* *out_dev = ||A||_inf (max row sum, FP64); work_dev: m doubles. */
+/* *out_dev = ||A||_inf (max row sum, FP64); work_dev: m * (1 + ceil(n/nb))
+ * doubles. */
 int dm_tiled_norm_inf(const dm_tiled* A, double* out_dev, double* work_dev, dm_stream_t s);
 /* out_dev[j] = max_i |X_ij| for each column j (FP64). */
 int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s);
@@ -331,6 +332,10 @@ int dm_tiled_pack(const dm_tiled* dst, const void* src, int64_t ld, double diag_
 int dm_tiled_unpack(void* dst, int64_t ld, const dm_tiled* src, dm_stream_t s);
 /* A[i,i] += d[i] on a full column-major FP64 matrix (input generators). */
 int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_dev, dm_stream_t s);
+/* Every tile of a one-tile-column matrix D (m x nb, Blocks(nb, nb)) set to
+ * [I_r 0] with r its row count (the start of an inverse of diagonal tiles;
+ * no reference counterpart). */
+int dm_tiled_identity(const dm_tiled* D, dm_stream_t s);
 
 /* Drop every cached plan/graph (call before freeing matrices whose plans
  * were cached, or rely on dm_forget_matrix for one base pointer). */
