@@ -16,31 +16,31 @@
 //     SYRK kernel.
 // Every launch first checks the abort flag (and info), so after a failed
 // pivot the remaining launches are no-ops.
+#include <cstdlib>
 #include "dm_internal.h"
 
 namespace dm {
 
-constexpr int PANEL = 64;  // panel width: one column per lane of a wave
 
 // Factor the kb x kb diagonal block at (j0, j0), kb = min(64, n - j0), in one
 // wave.  Element (r, c), r <= c, of the block's upper view is A[j0+r, j0+c]
 // (U) or A[j0+c, j0+r] (L, i.e. L^T); lane c holds column c of that view in
 // registers.  On a failed pivot at local column j: info = j0 + j + 1 and
 // *abort = global_base + j0 + j + 1.
-template <typename T, bool UPPER>
+template <typename T, bool UPPER, int PW>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n, int lda, int j0,
                                                         int* __restrict__ info,
                                                         int* __restrict__ abort,
                                                         int global_base) {
-  __shared__ T rowb[PANEL];  // row of the current step
-  __shared__ T pv;           // next pivot
+  __shared__ T rowb[64];  // row of the current step (lanes >= PW write zeros)
+  __shared__ T pv;        // next pivot
   if (abort && *abort) return;
   if (info && *info) return;
   const int t = threadIdx.x;
-  const int kb = min(PANEL, n - j0);
-  T col[PANEL];
+  const int kb = min(PW, n - j0);
+  T col[PW];
 #pragma unroll
-  for (int r = 0; r < PANEL; ++r) {
+  for (int r = 0; r < PW; ++r) {
     const bool in = t < kb && r <= t;
     col[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
                 : T(0);
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n
   __builtin_amdgcn_wave_barrier();  // every lane has read pv before step 0 rewrites it
   T sd = sqrt(d), inv = T(1) / sd;
 #pragma unroll
-  for (int j = 0; j < PANEL; ++j) {
+  for (int j = 0; j < PW; ++j) {
     if (j < kb) {
       if (!(d > T(0))) {
         fail = j + 1;
@@ -69,22 +69,22 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n
         u = col[j] * inv;
         col[j] = u;
       }
-      if (j + 1 < PANEL && t == j + 1) pv = col[j + 1] - u * u;
+      if (j + 1 < PW && t == j + 1) pv = col[j + 1] - u * u;
       rowb[t] = u;  // row j of the factor (zero left of and on the diagonal)
       __builtin_amdgcn_wave_barrier();
-      if (j + 1 < PANEL) {
+      if (j + 1 < PW) {
         d = pv;
         sd = sqrt(d);
         inv = T(1) / sd;
       }
 #pragma unroll
-      for (int r = j + 1; r < PANEL; ++r) col[r] -= rowb[r] * u;  // rows r <= t are kept
+      for (int r = j + 1; r < PW; ++r) col[r] -= rowb[r] * u;  // rows r <= t are kept
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 #pragma unroll
-  for (int r = 0; r < PANEL; ++r) {
+  for (int r = 0; r < PW; ++r) {
     if (t < kb && r <= t) {
       if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = col[r];
       else A[(size_t)(j0 + r) * lda + j0 + t] = col[r];
@@ -99,82 +99,84 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n
 // Panel row beside a factored full 64 x 64 block: U12 := U11^-T A12 (U) or
 // A21 := A21 L11^-T (L).  One wave per 64 columns of A12 (rows of A21), one
 // column (row) per lane in registers; the factored block sits in LDS.
-template <typename T, bool UPPER>
+template <typename T, bool UPPER, int PW>
 __global__ __launch_bounds__(64) void potrf_row_kernel(T* __restrict__ A, int n, int lda, int j0,
                                                        const int* __restrict__ info,
                                                        const int* __restrict__ abort) {
-  __shared__ T Us[PANEL][PANEL + 1];  // upper view of the factored block
-  __shared__ T invd[PANEL];
+  __shared__ T Us[PW][PW + 1];  // upper view of the factored block
+  __shared__ T invd[PW];
   if (abort && *abort) return;
   if (info && *info) return;
   const int t = threadIdx.x;
   // coalesced copy of the block: U reads columns, L reads rows of L = L^T columns
-  for (int c = 0; c < PANEL; ++c) {
-    // U: element (t, c) = A[j0+t, j0+c] for t <= c; L: element (t, c) = A[j0+c, j0+t]
-    T v = T(0);
-    if (t <= c) v = UPPER ? A[(size_t)(j0 + c) * lda + j0 + t] : A[(size_t)(j0 + t) * lda + j0 + c];
-    Us[t][c] = v;
+  if (t < PW) {
+    for (int c = 0; c < PW; ++c) {
+      // U: element (t, c) = A[j0+t, j0+c] for t <= c; L: element (t, c) = A[j0+c, j0+t]
+      T v = T(0);
+      if (t <= c) v = UPPER ? A[(size_t)(j0 + c) * lda + j0 + t] : A[(size_t)(j0 + t) * lda + j0 + c];
+      Us[t][c] = v;
+    }
   }
   __syncthreads();
-  invd[t] = T(1) / Us[t][t];
+  if (t < PW) invd[t] = T(1) / Us[t][t];
   __syncthreads();
-  const int c_begin = j0 + PANEL, rest = n - c_begin;
+  const int c_begin = j0 + PW, rest = n - c_begin;
   const int c0 = blockIdx.x * 64;
   if (c0 + t >= rest) return;  // no barrier follows
   const int cc = c_begin + c0 + t;
   const size_t step = UPPER ? 1 : (size_t)lda;
   T* const base = UPPER ? A + (size_t)cc * lda + j0 : A + (size_t)j0 * lda + cc;
-  T x[PANEL];
+  T x[PW];
   const T* px = base;
 #pragma unroll
-  for (int r = 0; r < PANEL; ++r) {
+  for (int r = 0; r < PW; ++r) {
     x[r] = *px;
     px += step;
     if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
   }
   // U11^T x = b, forward over the rows of the block
 #pragma unroll
-  for (int p = 0; p < PANEL; ++p) {
+  for (int p = 0; p < PW; ++p) {
     x[p] *= invd[p];
 #pragma unroll
-    for (int r = p + 1; r < PANEL; ++r) x[r] -= Us[p][r] * x[p];
+    for (int r = p + 1; r < PW; ++r) x[r] -= Us[p][r] * x[p];
     __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of one step together
   }
   T* qx = base;
   DM_OPAQUE(qx);  // recompute the store addresses instead of keeping 64 live
 #pragma unroll
-  for (int r = 0; r < PANEL; ++r) {
+  for (int r = 0; r < PW; ++r) {
     *qx = x[r];
     qx += step;
     if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-template <typename T>
-int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset, T* ws,
-               hipStream_t s) {
-  (void)ws;
+namespace {
+template <typename T, int PW>
+int potrf_panels(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset,
+                 hipStream_t s) {
   // Launches after a failed pivot are no-ops: they watch the plan-level
   // abort flag when there is one, else info.
   const int* chk = abort ? abort : info;
-  for (int j0 = 0; j0 < n; j0 += PANEL) {
+  for (int j0 = 0; j0 < n; j0 += PW) {
     if (upper)
-      hipLaunchKernelGGL((potrf_diag_kernel<T, true>), dim3(1), dim3(64), 0, s, A, n, lda, j0, info,
-                         abort, global_offset);
+      hipLaunchKernelGGL((potrf_diag_kernel<T, true, PW>), dim3(1), dim3(64), 0, s, A, n, lda, j0,
+                         info, abort, global_offset);
     else
-      hipLaunchKernelGGL((potrf_diag_kernel<T, false>), dim3(1), dim3(64), 0, s, A, n, lda, j0, info,
-                         abort, global_offset);
+      hipLaunchKernelGGL((potrf_diag_kernel<T, false, PW>), dim3(1), dim3(64), 0, s, A, n, lda, j0,
+                         info, abort, global_offset);
     DM_LAUNCH_CHECK("potrf_diag_kernel");
-    const int kb = n - j0 < PANEL ? n - j0 : PANEL;
+    const int kb = n - j0 < PW ? n - j0 : PW;
     const int rest = n - j0 - kb;
-    if (rest <= 0) break;  // a block narrower than 64 is the last one
+    if (rest <= 0) break;  // a block narrower than PW is the last one
     const unsigned nwg = (unsigned)((rest + 63) / 64);
     if (upper)
-      hipLaunchKernelGGL((potrf_row_kernel<T, true>), dim3(nwg), dim3(64), 0, s, A, n, lda, j0, info,
-                         abort);
-    else
-      hipLaunchKernelGGL((potrf_row_kernel<T, false>), dim3(nwg), dim3(64), 0, s, A, n, lda, j0,
+      hipLaunchKernelGGL((potrf_row_kernel<T, true, PW>), dim3(nwg), dim3(64), 0, s, A, n, lda, j0,
                          info, abort);
+    else
+      hipLaunchKernelGGL((potrf_row_kernel<T, false, PW>), dim3(nwg), dim3(64), 0, s, A, n, lda,
+                         j0, info, abort);
     DM_LAUNCH_CHECK("potrf_row_kernel");
     int rc;
     if (upper) {
@@ -191,6 +193,25 @@ int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int glob
     if (rc) return rc;
   }
   return 0;
+}
+
+int potrf_panel_width() {
+  // environment DM_POTRF_PANEL: 32 or 64 (tuning); 64 by default
+  static const int w = [] {
+    const char* e = std::getenv("DM_POTRF_PANEL");
+    return (e && std::atoi(e) == 32) ? 32 : 64;
+  }();
+  return w;
+}
+}  // namespace
+
+template <typename T>
+int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset, T* ws,
+               hipStream_t s) {
+  (void)ws;
+  if (potrf_panel_width() == 32)
+    return potrf_panels<T, 32>(upper, n, A, lda, info, abort, global_offset, s);
+  return potrf_panels<T, 64>(upper, n, A, lda, info, abort, global_offset, s);
 }
 
 template int potrf_full<double>(bool, int, double*, int, int*, int*, int, double*, hipStream_t);
