@@ -23,6 +23,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -74,7 +75,13 @@ def gemm_roofline(dm, A0, DA, nb):
                             profile=True))
     dag.run()
     prof = dag.profile()
-    g = [p for p in prof if p["op"] == 4 and p["ms"] > 0]  # DM_OP_GEMM groups
+    g_all = [p for p in prof if p["op"] == 4 and p["ms"] > 0]  # DM_OP_GEMM groups
+    # the dominant kernel: the GEMM tile configuration with the most time
+    per_variant = {}
+    for p in g_all:
+        per_variant[p["variant"]] = per_variant.get(p["variant"], 0.0) + p["ms"]
+    variant = max(per_variant, key=per_variant.get)
+    g = [p for p in g_all if p["variant"] == variant]
     flops = sum(p["flops"] for p in g)
     ms = sum(p["ms"] for p in g)
     launches = len(g)
@@ -85,18 +92,38 @@ def gemm_roofline(dm, A0, DA, nb):
         d[0] += 1
         d[1] += p["ms"]
         d[2] += p["flops"]
+    bm, bn = (variant // 1000) % 1000, variant % 1000
     return {
         "bound": "mfma",
         "achieved": round(achieved, 2),
         "peak": FP64_MFMA_PEAK,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_MFMA_PEAK, 4),
-        "traffic": None,
-        "kernel": "gemm_kernel (batched FP64 MFMA GEMM/SYRK trailing update)",
+        "traffic": committed_traffic(variant),
+        "kernel": f"gemm_kernel FP64 {bm}x{bn} tile (batched MFMA GEMM/SYRK trailing update)",
         "launches": launches,
         "avg_launch_ms": round(ms / max(launches, 1), 4),
         "avg_launch_flops": flops / max(launches, 1),
+        "share_of_gemm_time": round(ms / sum(per_variant.values()), 3),
     }, {str(k): {"groups": v[0], "ms": round(v[1], 3), "flops": v[2]} for k, v in by_op.items()}
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_kernel_fp64_cholesky_n32768_pmc.txt")
+
+
+def committed_traffic(variant):
+    """HBM bytes per launch of the dominant GEMM configuration, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload
+    (None when the summary is absent or covers another configuration)."""
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    cfg = re.search(r"^variant: (\d+)", text, re.M)
+    val = re.search(r"^traffic_bytes_per_launch: (\d+)", text, re.M)
+    if not cfg or not val or int(cfg.group(1)) != variant:
+        return None
+    return int(val.group(1))
 
 
 def mixed_leg(dm, n, nb, kappa, nrhs, seed):

@@ -727,7 +727,7 @@ int dm_dag_profile(dm_dag_t dag, dm_group_profile* out, int* count) {
       p.level = G.level;
       p.flops = G.flops;
       p.ms = ms;
-      p.pad_ = 0;
+      p.variant = G.op == DM_OP_GEMM ? G.cfg.id * 1000000 + G.cfg.bm * 1000 + G.cfg.bn : -1;
     }
     ++n;
   }

@@ -17,78 +17,109 @@
 // Every launch first checks the abort flag (and info), so after a failed
 // pivot the remaining launches are no-ops.
 #include <cstdlib>
+#include <type_traits>
 #include "dm_internal.h"
 
 namespace dm {
 
 
-// Factor the kb x kb diagonal block at (j0, j0), kb = min(64, n - j0), in one
+// Both panel kernels keep their code small enough for the instruction
+// cache: a single wave runs them, so every instruction fetched from L2
+// costs the whole kernel.  Instead of unrolling the 64 steps (a triangle
+// of static register indices, tens of KB of code), each lane keeps a
+// window of its column that slides by one row per step (the update of
+// row i+1 lands in register i, so the row being eliminated is always in
+// register 0), and the step loop stays rolled.  The window shrinks in
+// phases of 16 steps, so the update length follows the triangle in
+// steps of 16 rows.
+
+// Factor the kb x kb diagonal block at (j0, j0), kb = min(PW, n - j0), in one
 // wave.  Element (r, c), r <= c, of the block's upper view is A[j0+r, j0+c]
-// (U) or A[j0+c, j0+r] (L, i.e. L^T); lane c holds column c of that view in
-// registers.  On a failed pivot at local column j: info = j0 + j + 1 and
-// *abort = global_base + j0 + j + 1.
+// (U) or A[j0+c, j0+r] (L, i.e. L^T); lane c holds rows j.. of column c of
+// that view at step j.  On a failed pivot at local column j: info =
+// j0 + j + 1 and *abort = global_base + j0 + j + 1.
+template <typename T, bool UPPER, int PW>
+struct PotrfDiag {
+  T c[PW];     // c[i] = element (j + i, t) of the upper view at step j
+  T d, sd, inv;  // pivot of the current step, its square root and reciprocal
+  T* rowb;     // LDS: row j of the factor (PW entries) then PW zeros
+  T* pv;       // LDS: next pivot
+  T* A;
+  int lda, j0, kb, t;
+
+  __device__ __forceinline__ void put(int r, T v) {
+    if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = v;
+    else A[(size_t)(j0 + r) * lda + j0 + t] = v;
+  }
+
+  // Steps j0s .. j0s+15 (those below kb) with a window of L live rows.
+  // Returns the failed local column + 1, or 0.
+  template <int L>
+  __device__ __forceinline__ int phase(int js) {
+    const int je = min(js + 16, kb);
+#pragma unroll 1
+    for (int j = js; j < je; ++j) {
+      if (!(d > T(0))) return j + 1;
+      T u = T(0), fin = c[0];
+      if (t == j) fin = sd;
+      else if (t > j) {
+        u = c[0] * inv;
+        fin = u;
+      }
+      if (t >= j && t < kb) put(j, fin);
+      if (t == j + 1) *pv = c[1] - u * u;
+      rowb[t] = u;  // zero for lanes <= j: their columns are left unchanged
+      __builtin_amdgcn_wave_barrier();
+      d = *pv;
+      sd = sqrt(d);
+      inv = T(1) / sd;
+      const T* rb = rowb + j + 1;  // reads past PW land in the zero half
+#pragma unroll
+      for (int i = 0; i < L - 1; ++i) c[i] = c[i + 1] - rb[i] * u;
+      c[L - 1] = T(0);
+      __builtin_amdgcn_wave_barrier();  // rowb/pv are rewritten by the next step
+    }
+    return 0;
+  }
+};
+
 template <typename T, bool UPPER, int PW>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n, int lda, int j0,
                                                         int* __restrict__ info,
                                                         int* __restrict__ abort,
                                                         int global_base) {
-  __shared__ T rowb[64];  // row of the current step (lanes >= PW write zeros)
-  __shared__ T pv;        // next pivot
+  static_assert(PW == 32 || PW == 64, "panel width");
+  __shared__ T rowb[2 * 64];
+  __shared__ T pv;
   if (abort && *abort) return;
   if (info && *info) return;
-  const int t = threadIdx.x;
-  const int kb = min(PW, n - j0);
-  T col[PW];
+  PotrfDiag<T, UPPER, PW> f;
+  f.t = threadIdx.x;
+  f.kb = min(PW, n - j0);
+  f.A = A;
+  f.lda = lda;
+  f.j0 = j0;
+  f.rowb = rowb;
+  f.pv = &pv;
+  const int t = f.t;
 #pragma unroll
   for (int r = 0; r < PW; ++r) {
-    const bool in = t < kb && r <= t;
-    col[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
+    const bool in = t < f.kb && r <= t;
+    f.c[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
                 : T(0);
   }
-  // One wave, so its LDS accesses execute in order: broadcasts through LDS
-  // need only compiler ordering (wave_barrier), no waits.  The pivot of step
-  // j+1 is lane j+1's diagonal after step j, which that lane updates with its
-  // own u: it is broadcast together with row j, so the next square root and
-  // reciprocal overlap this step's rank-1 update.
-  int fail = 0;
-  if (t == 0) pv = col[0];
+  rowb[64 + t] = T(0);
+  if (t == 0) pv = f.c[0];
   __builtin_amdgcn_wave_barrier();
-  T d = pv;
+  f.d = pv;
+  f.sd = sqrt(f.d);
+  f.inv = T(1) / f.sd;
   __builtin_amdgcn_wave_barrier();  // every lane has read pv before step 0 rewrites it
-  T sd = sqrt(d), inv = T(1) / sd;
-#pragma unroll
-  for (int j = 0; j < PW; ++j) {
-    if (j < kb) {
-      if (!(d > T(0))) {
-        fail = j + 1;
-        break;
-      }
-      T u = T(0);
-      if (t == j) col[j] = sd;
-      else if (t > j) {
-        u = col[j] * inv;
-        col[j] = u;
-      }
-      if (j + 1 < PW && t == j + 1) pv = col[j + 1] - u * u;
-      rowb[t] = u;  // row j of the factor (zero left of and on the diagonal)
-      __builtin_amdgcn_wave_barrier();
-      if (j + 1 < PW) {
-        d = pv;
-        sd = sqrt(d);
-        inv = T(1) / sd;
-      }
-#pragma unroll
-      for (int r = j + 1; r < PW; ++r) col[r] -= rowb[r] * u;  // rows r <= t are kept
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < PW; ++r) {
-    if (t < kb && r <= t) {
-      if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = col[r];
-      else A[(size_t)(j0 + r) * lda + j0 + t] = col[r];
-    }
+  int fail = f.template phase<PW>(0);
+  if (!fail && PW > 16) fail = f.template phase<PW - 16>(16);
+  if (PW == 64) {
+    if (!fail) fail = f.template phase<(PW > 32 ? PW - 32 : 16)>(32);
+    if (!fail) fail = f.template phase<(PW > 48 ? PW - 48 : 16)>(48);
   }
   if (t == 0 && fail) {
     if (info) *info = j0 + fail;
@@ -96,29 +127,32 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n
   }
 }
 
-// Panel row beside a factored full 64 x 64 block: U12 := U11^-T A12 (U) or
+// Panel row beside a factored full PW x PW block: U12 := U11^-T A12 (U) or
 // A21 := A21 L11^-T (L).  One wave per 64 columns of A12 (rows of A21), one
-// column (row) per lane in registers; the factored block sits in LDS.
+// column (row) per lane; the factored block sits in LDS as its upper view
+// with row stride PW, so reading row p past column PW-1 lands in the
+// (zero) strictly lower part of row p+1, followed by PW zeros.
 template <typename T, bool UPPER, int PW>
 __global__ __launch_bounds__(64) void potrf_row_kernel(T* __restrict__ A, int n, int lda, int j0,
                                                        const int* __restrict__ info,
                                                        const int* __restrict__ abort) {
-  __shared__ T Us[PW][PW + 1];  // upper view of the factored block
+  static_assert(PW == 32 || PW == 64, "panel width");
+  __shared__ T Us[PW * PW + PW];
   __shared__ T invd[PW];
   if (abort && *abort) return;
   if (info && *info) return;
   const int t = threadIdx.x;
-  // coalesced copy of the block: U reads columns, L reads rows of L = L^T columns
   if (t < PW) {
-    for (int c = 0; c < PW; ++c) {
-      // U: element (t, c) = A[j0+t, j0+c] for t <= c; L: element (t, c) = A[j0+c, j0+t]
+    // lane = column c; U: element (r, c) = A[j0+r, j0+c]; L: A[j0+c, j0+r]
+    for (int r = 0; r < PW; ++r) {
       T v = T(0);
-      if (t <= c) v = UPPER ? A[(size_t)(j0 + c) * lda + j0 + t] : A[(size_t)(j0 + t) * lda + j0 + c];
-      Us[t][c] = v;
+      if (r <= t) v = UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t];
+      Us[r * PW + t] = v;
     }
+    Us[PW * PW + t] = T(0);
   }
   __syncthreads();
-  if (t < PW) invd[t] = T(1) / Us[t][t];
+  if (t < PW) invd[t] = T(1) / Us[t * PW + t];
   __syncthreads();
   const int c_begin = j0 + PW, rest = n - c_begin;
   const int c0 = blockIdx.x * 64;
@@ -134,21 +168,24 @@ __global__ __launch_bounds__(64) void potrf_row_kernel(T* __restrict__ A, int n,
     px += step;
     if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
   }
-  // U11^T x = b, forward over the rows of the block
+  // U11^T x = b, forward over the rows of the block; x[i] holds row p + i
+  auto phase = [&](auto len, int ps) {
+    constexpr int L = decltype(len)::value;
+#pragma unroll 1
+    for (int p = ps; p < ps + 16; ++p) {
+      const T xp = x[0] * invd[p];
+      base[p * step] = xp;
+      const T* u = Us + p * PW + p + 1;
 #pragma unroll
-  for (int p = 0; p < PW; ++p) {
-    x[p] *= invd[p];
-#pragma unroll
-    for (int r = p + 1; r < PW; ++r) x[r] -= Us[p][r] * x[p];
-    __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of one step together
-  }
-  T* qx = base;
-  DM_OPAQUE(qx);  // recompute the store addresses instead of keeping 64 live
-#pragma unroll
-  for (int r = 0; r < PW; ++r) {
-    *qx = x[r];
-    qx += step;
-    if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+      for (int i = 0; i < L - 1; ++i) x[i] = x[i + 1] - u[i] * xp;
+      x[L - 1] = T(0);
+    }
+  };
+  phase(std::integral_constant<int, PW>{}, 0);
+  phase(std::integral_constant<int, PW - 16>{}, 16);
+  if constexpr (PW == 64) {
+    phase(std::integral_constant<int, 32>{}, 32);
+    phase(std::integral_constant<int, 16>{}, 48);
   }
 }
 

@@ -47,7 +47,7 @@ class dm_plan_opts(C.Structure):
 
 class dm_group_profile(C.Structure):
     _fields_ = [("op", i32), ("ntasks", i32), ("stream", i32), ("level", i32), ("flops", f64),
-                ("ms", f32), ("pad_", i32)]
+                ("ms", f32), ("variant", i32)]
 
 
 class dm_dag_stats(C.Structure):

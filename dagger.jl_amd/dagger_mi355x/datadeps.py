@@ -218,15 +218,16 @@ class Dag:
         return cls(handle=h)
 
     def profile(self):
-        """Per-group [op, ntasks, stream, level, flops, ms] of the last
-        profiled run."""
+        """Per-group [op, ntasks, stream, level, flops, ms, variant] of the
+        last profiled run (variant: GEMM tile configuration, id*1000000 +
+        BM*1000 + BN; -1 for other groups)."""
         cnt = C.c_int(0)
         call("dm_dag_profile", self.h, None, C.byref(cnt))
         buf = (_lib.dm_group_profile * max(cnt.value, 1))()
         cnt2 = C.c_int(cnt.value)
         call("dm_dag_profile", self.h, buf, C.byref(cnt2))
         return [dict(op=b.op, ntasks=b.ntasks, stream=b.stream, level=b.level, flops=b.flops,
-                     ms=b.ms) for b in buf[:cnt2.value]]
+                     ms=b.ms, variant=b.variant) for b in buf[:cnt2.value]]
 
     def edges(self):
         cnt = C.c_int(0)

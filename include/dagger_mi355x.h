@@ -227,7 +227,10 @@ typedef struct dm_group_profile {
     int level;
     double flops;
     float ms;
-    int pad_;
+    int variant;   /* GEMM groups: the tile configuration launched, encoded
+                      as id*1000000 + BM*1000 + BN (e.g. 4256128: FP64
+                      256x128; 3128128: FP32 via BF16x6 128x128); -1 for
+                      other groups */
 } dm_group_profile;
 
 typedef struct dm_dag_stats {
