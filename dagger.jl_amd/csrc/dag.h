@@ -55,6 +55,8 @@ struct Group {
     GemmCfgId cfg{0, 128, 128};
     size_t table_off = 0;
     void* table_dev = nullptr;
+    double flops = 0;  // algorithmic flops of the step's launch
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // profiled runs: around the launch
   };
   std::vector<Step> tsteps;
   // common

@@ -287,6 +287,11 @@ int Dag::create_streams(bool eager) {
     prof_stop.resize(groups.size());
     for (auto& e : prof_start) DM_HIP(hipEventCreate(&e));
     for (auto& e : prof_stop) DM_HIP(hipEventCreate(&e));
+    for (Group& G : groups)
+      for (Group::Step& st : G.tsteps) {
+        DM_HIP(hipEventCreate(&st.ev0));
+        DM_HIP(hipEventCreate(&st.ev1));
+      }
   }
   join_ev.resize(opts.num_streams);
   for (auto& e : join_ev) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -401,6 +406,7 @@ int Dag::build_device_tables() {
           if (st.leaf) {
             rec.njobs = (int)st.tjobs.size();
             rec.maxL = st.maxL;
+            for (const TrsmJob<T>& j : st.tjobs) rec.flops += (double)j.L * j.L * j.NV;
             rec.total_wg = st.total_wg;
             const unsigned char* p = (const unsigned char*)st.tjobs.data();
             blob.insert(blob.end(), p, p + st.tjobs.size() * sizeof(TrsmJob<T>));
@@ -410,6 +416,7 @@ int Dag::build_device_tables() {
               maxM = std::max(maxM, g.M); maxN = std::max(maxN, g.N); maxK = std::max(maxK, g.K);
             }
             rec.njobs = (int)st.gjobs.size();
+            for (const GemmJob<T>& g : st.gjobs) rec.flops += 2.0 * g.M * g.N * g.K;
             rec.ta = st.ta;
             rec.tb = st.tb;
             rec.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, rec.njobs);
@@ -495,8 +502,10 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       int rc = trinv_blocks_launch<T>(G.shape, (const T*)src.A, src.lda, src.L, (T*)src.dev, ab, s);
       if (rc) return rc;
     }
+    const bool prof = opts.profile && !capturing;
     for (const Group::Step& st : G.tsteps) {
       int rc;
+      if (prof && st.ev0) DM_HIP(hipEventRecord(st.ev0, s));
       if (st.leaf) {
         TrsmJob<T> dummy{};
         dummy.L = st.maxL;
@@ -509,6 +518,7 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
                             dummy, st.total_wg, ab, s, cap ? bulk_cus * gemm_wgs_per_cu(st.cfg) : 0);
       }
       if (rc) return rc;
+      if (prof && st.ev1) DM_HIP(hipEventRecord(st.ev1, s));
     }
     return 0;
   }
@@ -642,6 +652,11 @@ Dag::~Dag() {
   for (auto& e : join_ev) (void)hipEventDestroy(e);
   for (auto& e : prof_start) (void)hipEventDestroy(e);
   for (auto& e : prof_stop) (void)hipEventDestroy(e);
+  for (Group& G : groups)
+    for (Group::Step& st : G.tsteps) {
+      if (st.ev0) (void)hipEventDestroy(st.ev0);
+      if (st.ev1) (void)hipEventDestroy(st.ev1);
+    }
   if (!streams.empty() || origin) (void)hipDeviceSynchronize();
   for (auto& s : streams) (void)hipStreamDestroy(s);
   if (origin) (void)hipStreamDestroy(origin);
@@ -714,22 +729,40 @@ int dm_dag_profile(dm_dag_t dag, dm_group_profile* out, int* count) {
   if (rc) return rc;
   const int cap = *count;
   int n = 0;
-  for (int g : d.order) {
-    const Group& G = d.groups[g];
+  auto emit = [&](const Group& G, int op, int variant, double flops, hipEvent_t a,
+                  hipEvent_t b) -> int {
     if (out && n < cap) {
       float ms = 0.f;
-      DM_HIP(hipEventSynchronize(d.prof_stop[g]));
-      DM_HIP(hipEventElapsedTime(&ms, d.prof_start[g], d.prof_stop[g]));
+      DM_HIP(hipEventSynchronize(b));
+      DM_HIP(hipEventElapsedTime(&ms, a, b));
       dm_group_profile& p = out[n];
-      p.op = G.op;
+      p.op = op;
       p.ntasks = (int)G.tasks.size();
       p.stream = G.stream;
       p.level = G.level;
-      p.flops = G.flops;
+      p.flops = flops;
       p.ms = ms;
-      p.variant = G.op == DM_OP_GEMM ? G.cfg.id * 1000000 + G.cfg.bm * 1000 + G.cfg.bn : -1;
+      p.variant = variant;
     }
     ++n;
+    return 0;
+  };
+  for (int g : d.order) {
+    const Group& G = d.groups[g];
+    int rc;
+    if (G.op == DM_OP_TRSM && G.batched_trsm && !G.tsteps.empty() && G.tsteps[0].ev0) {
+      // one record per launch of the batched solve: leaf solves (TRSM) and
+      // the GEMM updates between them
+      for (const Group::Step& st : G.tsteps) {
+        const int variant = st.leaf ? -1 : st.cfg.id * 1000000 + st.cfg.bm * 1000 + st.cfg.bn;
+        rc = emit(G, st.leaf ? DM_OP_TRSM : DM_OP_GEMM, variant, st.flops, st.ev0, st.ev1);
+        if (rc) return rc;
+      }
+      continue;
+    }
+    const int variant = G.op == DM_OP_GEMM ? G.cfg.id * 1000000 + G.cfg.bm * 1000 + G.cfg.bn : -1;
+    rc = emit(G, G.op, variant, G.flops, d.prof_start[g], d.prof_stop[g]);
+    if (rc) return rc;
   }
   *count = n;
   return 0;

@@ -218,7 +218,9 @@ typedef struct dm_plan_opts {
 /* Per-launch-group timing of the last profiled run (dm_plan_opts.profile):
  * the hipEvent time of the group's launches on its stream and the
  * algorithmic flops of its tasks (GEMM 2mnk, SYRK n^2 k, TRSM m n ka,
- * POTRF n^3/3).  The event-based counterpart of Dagger's timespan logging
+ * POTRF n^3/3).  A batched TRSM group reports one record per launch: its
+ * leaf solves (op TRSM, L^2 flops per vector) and the GEMM updates between
+ * them (op GEMM, 2mnk, with their tile configuration).  The event-based counterpart of Dagger's timespan logging
  * (src/utils/logging.jl:26-143) for device work. */
 typedef struct dm_group_profile {
     int op;        /* DM_OP_* of the group's tasks (SYRK groups report GEMM) */
@@ -251,7 +253,7 @@ int dm_dag_plan(dm_dag_t dag, const dm_plan_opts* opts);
  * after PosDefException in the reference. */
 int dm_dag_run(dm_dag_t dag, dm_stream_t s);
 int dm_dag_get_stats(dm_dag_t dag, dm_dag_stats* st);
-/* Group timings of the last profiled run; *count: capacity in, groups out. */
+/* Records of the last profiled run; *count: capacity in, records out. */
 int dm_dag_profile(dm_dag_t dag, dm_group_profile* out, int* count);
 
 /* ------------------------------------------------- whole-matrix forms
