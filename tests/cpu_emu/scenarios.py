@@ -386,7 +386,73 @@ def capped():
               ("capped ldiv", uplo))
 
 
-GROUPS = dict(tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
+def quick():
+    """A few seconds of every kernel family: the CPU suite's default group."""
+    rng = np.random.default_rng(2)
+    for ta, tb, (m, n, k) in [("T", "N", (70, 40, 33)), ("N", "T", (5, 9, 3))]:
+        A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+        B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+        Cm = rng.standard_normal((m, n))
+        dA, dB, dC = Dev(A), Dev(B), Dev(Cm)
+        call("dm_dgemm", ta.encode(), tb.encode(), m, n, k, -1.0, dA.p, A.shape[0], dB.p,
+             B.shape[0], 0.5, dC.p, m, None)
+        ref = 0.5 * Cm - (A.T if ta == "T" else A) @ (B.T if tb == "T" else B)
+        check(relerr(dC.get(), ref) < 1e-13, ("dgemm", ta, tb))
+    F, _ = O.cholesky(gen.spd(70, 5), 70, "U")
+    B = rng.standard_normal((70, 20))
+    dA, dB = Dev(F), Dev(B)
+    call("dm_dtrsm", b"L", b"U", b"T", b"N", 70, 20, 1.0, dA.p, 70, dB.p, 70, None)
+    ref = B.copy(order="F")
+    O.tile_trsm("L", "U", "T", "N", 1.0, np.asfortranarray(F), ref)
+    check(relerr(dB.get(), ref) < 1e-12, "dtrsm")
+    for uplo in "UL":
+        n = 130
+        A = gen.spd(n, 900 + n)
+        dA = Dev(A)
+        info = dm.DeviceInt(1, value=-7)
+        call("dm_dpotrf", uplo.encode(), n, dA.p, n, C.c_void_p(info.ptr), None)
+        ref, _ = O.cholesky(A, n, uplo)
+        check(int(info.get()[0]) == 0 and O.rel_factor_diff(dA.get(), ref, uplo) < 1e-12,
+              ("dpotrf", uplo))
+    A = gen.spd(100, 5)
+    A[70, 70] = -1.0
+    dA, info = Dev(A), dm.DeviceInt(1)
+    call("dm_dpotrf", b"L", 100, dA.p, 100, C.c_void_p(info.ptr), None)
+    check(int(info.get()[0]) == 71, "dpotrf info")
+    A = np.zeros((36, 64), np.float32)
+    Bm = np.zeros((36, 48), np.float32)
+    A[:33] = rng.standard_normal((33, 64))
+    Bm[:33] = rng.standard_normal((33, 48))
+    Cm = rng.standard_normal((64, 48)).astype(np.float32)
+    dA, dB, dC = Dev(A), Dev(Bm), Dev(Cm)
+    call("dm_sgemm_bf16x", 6, b"T", b"N", 64, 48, 33, C.c_float(-1.0), dA.p, 36, dB.p, 36,
+         C.c_float(1.0), dC.p, 64, None)
+    ref = Cm.astype(np.float64) - A[:33].T.astype(np.float64) @ Bm[:33].astype(np.float64)
+    check(relerr(dC.get().astype(np.float64), ref) < 2e-6, "bf16x6")
+    n, nb = 200, 64  # ragged tiles
+    A = gen.spd(n, 6)
+    ref, _ = O.cholesky(A, nb, "U")
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    Fc = dm.cholesky_(DA, options=dm.PlanOptions(num_streams=2, batch=True, lookahead=1,
+                                                  use_graph=False))
+    check(O.rel_factor_diff(DA.collect(), ref, "U") < 1e-12, "tiled cholesky")
+    b = gen.randn(n * 3, 8).reshape(3, n).T.copy()
+    DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
+    dm.ldiv_(Fc, DB)
+    check(O.backward_error(A, DB.collect(), b) < 10 * n * np.finfo(float).eps, "ldiv")
+    from dagger_mi355x import batched as Bt
+    bs = Bt.BatchedSystems([40, 70, 33], [2, 1, 3], np.float64).generate(seed=100)
+    A0s = [bs.get_system(m) for m in range(bs.count)]
+    bs.posv(panels={})
+    check(np.all(bs.infos() == 0), "batched info")
+    for m in range(bs.count):
+        A0, B0 = A0s[m]
+        _, X = bs.get_system(m)
+        check(O.backward_error(A0, X, B0) <= 10 * A0.shape[0] * np.finfo(float).eps,
+              ("batched", m))
+
+
+GROUPS = dict(quick=quick, tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
               capped=capped)
 
 if __name__ == "__main__":
