@@ -276,7 +276,26 @@ int Dag::create_streams(bool eager) {
   // throughput than it saved on the critical path)
   crit_cus = 0;
   if (const char* e = std::getenv("DM_CRIT_CUS")) crit_cus = std::atoi(e);
-  for (auto& st : streams) DM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  // Environment DM_CU_RESERVE=R (tuning): with look-ahead, the bulk streams
+  // are created with a CU mask that leaves R CUs (R/8 per XCD, spread over
+  // the mask's XCD-interleaved bit order) to the critical stream.
+  std::vector<uint32_t> mask;
+  int reserve = 0;
+  if (const char* e = std::getenv("DM_CU_RESERVE")) reserve = std::atoi(e);
+  if (reserve > 0 && reserve < cus && opts.lookahead > 0 && opts.num_streams >= 2) {
+    mask.assign((size_t)(cus + 31) / 32, 0u);
+    for (int i = 0; i < cus; ++i) mask[i / 32] |= 1u << (i % 32);
+    for (int k = 0; k < reserve; ++k) {
+      const int i = cus - 1 - k;  // the last bits: k % 8 walks the 8 XCDs
+      mask[i / 32] &= ~(1u << (i % 32));
+    }
+  }
+  for (size_t i = 0; i < streams.size(); ++i) {
+    if (i > 0 && !mask.empty())
+      DM_HIP(hipExtStreamCreateWithCUMask(&streams[i], (uint32_t)mask.size(), mask.data()));
+    else
+      DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
+  }
   DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   events.resize(groups.size());
   for (auto& e : events) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));

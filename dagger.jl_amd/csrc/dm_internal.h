@@ -199,6 +199,11 @@ int trsm_full(char side, char uplo, char transa, char diag, int m, int n,
 // ------------------------------------------------------------ POTRF
 // info: LAPACK info of this call (local order); abort (may be null): set to
 // global_offset + local order on failure and checked by every launch.
+// Trailing SYRK of the 64-wide POTRF panel at j0 fused with the
+// factorization of the next diagonal block (gemm.hip).
+template <typename T>
+int potrf_update_launch(bool upper, int n, T* A, int lda, int j0, int* info, int* abort,
+                        int global_base, hipStream_t s);
 template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
                int global_offset, T* ws, hipStream_t s);

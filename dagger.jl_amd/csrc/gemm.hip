@@ -26,6 +26,7 @@
 //    masked.
 #include "dm_internal.h"
 #include "gemm_tiles.h"
+#include "potrf_diag.h"
 #include <type_traits>
 
 namespace dm {
@@ -283,6 +284,35 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __
   }
 }
 
+
+// Trailing update of one POTRF panel fused with the factorization of the
+// next diagonal block.  The SYRK's packed triangle tiles are 64 x 64, so
+// tile 0 is exactly the next 64 x 64 diagonal block: workgroup 0 updates it
+// and then its first wave factors it (potrf_diag.h) while the other
+// workgroups update the rest of the triangle -- one launch per panel
+// fewer on the critical path, and the block factorization overlaps the
+// update instead of following it.
+template <typename T, bool UPPER, int VEC>
+__global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __restrict__ A, int n,
+                                                           int lda, int j1,
+                                                           int* __restrict__ info,
+                                                           int* __restrict__ abort,
+                                                           int global_base) {
+  typedef GemmKernel<T, 64, 64, 16, 2, 2, UPPER, !UPPER, VEC> G;
+  __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
+  static_assert(2 * (G::A_ELEMS + G::B_ELEMS) >= 129, "LDS reuse for the block step");
+  if (abort && *abort) return;
+  if (info && *info) return;
+  const int t = blockIdx.x;
+  GemmJob<T> jb;
+  int tm, tn;
+  locate_tile<T>(nullptr, 0, job, t, jb, tm, tn, true);
+  G::run(jb, tm, tn, smem);
+  if (t != 0) return;
+  __syncthreads();  // the block's update is complete and visible to wave 0; LDS is free
+  potrf_diag_block<T, UPPER, 64>(A, n, lda, j1, info, abort, global_base, smem, smem + 128);
+}
+
 // ------------------------------------------------------------ host side
 // Configurations (FP64; measured on the k-contiguous Cholesky update shape):
 //   id 4: 256x128 tile, 16 waves of 64x32, BK 16 (110 KB LDS) -- big launches
@@ -431,5 +461,49 @@ template int gemm_single<double>(bool, bool, int, int, int, double, const double
                                  hipStream_t);
 template int gemm_single<float>(bool, bool, int, int, int, float, const float*, int, const float*,
                                 int, float, float*, int, int, const int*, hipStream_t);
+
+
+template <typename T>
+int potrf_update_launch(bool upper, int n, T* A, int lda, int j0, int* info, int* abort,
+                        int global_base, hipStream_t s) {
+  const int kb = 64, rest = n - j0 - kb;
+  if (rest <= 0) return 0;
+  GemmJob<T> j{};
+  if (upper) {  // A22 -= U12^T U12
+    j.A = j.B = A + (size_t)(j0 + kb) * lda + j0;
+    j.tri = 1;
+  } else {      // A22 -= A21 A21^T
+    j.A = j.B = A + (size_t)j0 * lda + j0 + kb;
+    j.tri = 2;
+  }
+  j.C = A + (size_t)(j0 + kb) * lda + j0 + kb;
+  j.M = j.N = rest;
+  j.K = kb;
+  j.lda = j.ldb = j.ldc = lda;
+  j.alpha = T(-1);
+  j.beta = T(1);
+  const long total = gemm_prepare_jobs<T>(&j, 1, GemmCfgId{1, 64, 64});
+  const bool vec = job_vec_ok(upper, !upper, j);
+  const int j1 = j0 + kb;
+  if (upper) {
+    if (vec)
+      hipLaunchKernelGGL((potrf_update_kernel<T, true, 2>), dim3((unsigned)total), dim3(256), 0, s, j,
+                         A, n, lda, j1, info, abort, global_base);
+    else
+      hipLaunchKernelGGL((potrf_update_kernel<T, true, 1>), dim3((unsigned)total), dim3(256), 0, s, j,
+                         A, n, lda, j1, info, abort, global_base);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((potrf_update_kernel<T, false, 2>), dim3((unsigned)total), dim3(256), 0, s,
+                         j, A, n, lda, j1, info, abort, global_base);
+    else
+      hipLaunchKernelGGL((potrf_update_kernel<T, false, 1>), dim3((unsigned)total), dim3(256), 0, s,
+                         j, A, n, lda, j1, info, abort, global_base);
+  }
+  DM_LAUNCH_CHECK("potrf_update_kernel");
+  return 0;
+}
+template int potrf_update_launch<double>(bool, int, double*, int, int, int*, int*, int, hipStream_t);
+template int potrf_update_launch<float>(bool, int, float*, int, int, int*, int*, int, hipStream_t);
 
 }  // namespace dm

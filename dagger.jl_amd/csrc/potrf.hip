@@ -4,127 +4,38 @@
 // 2-18, spawned at :39/:84).  info follows LAPACK: 0, or k > 0 when the
 // leading minor of order k is not positive definite (pivot <= 0 or NaN).
 //
-// Structure: blocked right-looking factorization with 64-wide panels, three
+// Structure: blocked right-looking factorization with 64-wide panels, two
 // launches per panel:
-//  1. potrf_diag_kernel (one wave): factors the 64 x 64 diagonal block in
-//     registers (lane c owns column c of U, or row c of L = column c of
-//     L^T), broadcasting the current row through LDS -- no barriers;
-//  2. potrf_row_kernel (one wave per 64 columns): the panel row
+//  1. potrf_row_kernel (one wave per 64 columns): the panel row
 //     U12 := U11^-T A12 (U) / A21 := A21 L11^-T (L), one column (row) per
 //     lane against the factored block held in LDS;
-//  3. the trailing update A22 -= U12^T U12 (A21 A21^T for L) on the MFMA
-//     SYRK kernel.
+//  2. the trailing update A22 -= U12^T U12 (A21 A21^T for L) on MFMA, fused
+//     with the factorization of the next 64 x 64 diagonal block
+//     (potrf_update_kernel, gemm.hip).
+// The first diagonal block is factored by potrf_diag_kernel (one wave, the
+// block in registers: lane c owns column c of U, or row c of L = column c
+// of L^T, broadcasting the current row through LDS -- no barriers).
 // Every launch first checks the abort flag (and info), so after a failed
 // pivot the remaining launches are no-ops.
 #include <cstdlib>
 #include <type_traits>
 #include "dm_internal.h"
+#include "potrf_diag.h"
 
 namespace dm {
 
 
-// Both panel kernels keep their code small enough for the instruction
-// cache: a single wave runs them, so every instruction fetched from L2
-// costs the whole kernel.  Instead of unrolling the 64 steps (a triangle
-// of static register indices, tens of KB of code), each lane keeps a
-// window of its column that slides by one row per step (the update of
-// row i+1 lands in register i, so the row being eliminated is always in
-// register 0), and the step loop stays rolled.  The window shrinks in
-// phases of 16 steps, so the update length follows the triangle in
-// steps of 16 rows.
-
-// Factor the kb x kb diagonal block at (j0, j0), kb = min(PW, n - j0), in one
-// wave.  Element (r, c), r <= c, of the block's upper view is A[j0+r, j0+c]
-// (U) or A[j0+c, j0+r] (L, i.e. L^T); lane c holds rows j.. of column c of
-// that view at step j.  On a failed pivot at local column j: info =
-// j0 + j + 1 and *abort = global_base + j0 + j + 1.
-template <typename T, bool UPPER, int PW>
-struct PotrfDiag {
-  T c[PW];     // c[i] = element (j + i, t) of the upper view at step j
-  T d, sd, inv;  // pivot of the current step, its square root and reciprocal
-  T* rowb;     // LDS: row j of the factor (PW entries) then PW zeros
-  T* pv;       // LDS: next pivot
-  T* A;
-  int lda, j0, kb, t;
-
-  __device__ __forceinline__ void put(int r, T v) {
-    if (UPPER) A[(size_t)(j0 + t) * lda + j0 + r] = v;
-    else A[(size_t)(j0 + r) * lda + j0 + t] = v;
-  }
-
-  // Steps j0s .. j0s+15 (those below kb) with a window of L live rows.
-  // Returns the failed local column + 1, or 0.
-  template <int L>
-  __device__ __forceinline__ int phase(int js) {
-    const int je = min(js + 16, kb);
-#pragma unroll 1
-    for (int j = js; j < je; ++j) {
-      if (!(d > T(0))) return j + 1;
-      T u = T(0), fin = c[0];
-      if (t == j) fin = sd;
-      else if (t > j) {
-        u = c[0] * inv;
-        fin = u;
-      }
-      if (t >= j && t < kb) put(j, fin);
-      if (t == j + 1) *pv = c[1] - u * u;
-      rowb[t] = u;  // zero for lanes <= j: their columns are left unchanged
-      __builtin_amdgcn_wave_barrier();
-      d = *pv;
-      sd = sqrt(d);
-      inv = T(1) / sd;
-      const T* rb = rowb + j + 1;  // reads past PW land in the zero half
-#pragma unroll
-      for (int i = 0; i < L - 1; ++i) c[i] = c[i + 1] - rb[i] * u;
-      c[L - 1] = T(0);
-      __builtin_amdgcn_wave_barrier();  // rowb/pv are rewritten by the next step
-    }
-    return 0;
-  }
-};
-
+// The diagonal block of panel j0 (see potrf_diag.h), one wave.
 template <typename T, bool UPPER, int PW>
 __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n, int lda, int j0,
                                                         int* __restrict__ info,
                                                         int* __restrict__ abort,
                                                         int global_base) {
-  static_assert(PW == 32 || PW == 64, "panel width");
   __shared__ T rowb[2 * 64];
   __shared__ T pv;
   if (abort && *abort) return;
   if (info && *info) return;
-  PotrfDiag<T, UPPER, PW> f;
-  f.t = threadIdx.x;
-  f.kb = min(PW, n - j0);
-  f.A = A;
-  f.lda = lda;
-  f.j0 = j0;
-  f.rowb = rowb;
-  f.pv = &pv;
-  const int t = f.t;
-#pragma unroll
-  for (int r = 0; r < PW; ++r) {
-    const bool in = t < f.kb && r <= t;
-    f.c[r] = in ? (UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t])
-                : T(0);
-  }
-  rowb[64 + t] = T(0);
-  if (t == 0) pv = f.c[0];
-  __builtin_amdgcn_wave_barrier();
-  f.d = pv;
-  f.sd = sqrt(f.d);
-  f.inv = T(1) / f.sd;
-  __builtin_amdgcn_wave_barrier();  // every lane has read pv before step 0 rewrites it
-  int fail = f.template phase<PW>(0);
-  if (!fail && PW > 16) fail = f.template phase<PW - 16>(16);
-  if (PW == 64) {
-    if (!fail) fail = f.template phase<(PW > 32 ? PW - 32 : 16)>(32);
-    if (!fail) fail = f.template phase<(PW > 48 ? PW - 48 : 16)>(48);
-  }
-  if (t == 0 && fail) {
-    if (info) *info = j0 + fail;
-    if (abort) *abort = global_base + j0 + fail;
-  }
+  potrf_diag_block<T, UPPER, PW>(A, n, lda, j0, info, abort, global_base, rowb, &pv);
 }
 
 // Panel row beside a factored full PW x PW block: U12 := U11^-T A12 (U) or
@@ -196,14 +107,18 @@ int potrf_panels(bool upper, int n, T* A, int lda, int* info, int* abort, int gl
   // Launches after a failed pivot are no-ops: they watch the plan-level
   // abort flag when there is one, else info.
   const int* chk = abort ? abort : info;
+  bool diag_done = false;  // the previous panel's fused update factored this block
   for (int j0 = 0; j0 < n; j0 += PW) {
-    if (upper)
-      hipLaunchKernelGGL((potrf_diag_kernel<T, true, PW>), dim3(1), dim3(64), 0, s, A, n, lda, j0,
-                         info, abort, global_offset);
-    else
-      hipLaunchKernelGGL((potrf_diag_kernel<T, false, PW>), dim3(1), dim3(64), 0, s, A, n, lda, j0,
-                         info, abort, global_offset);
-    DM_LAUNCH_CHECK("potrf_diag_kernel");
+    if (!diag_done) {
+      if (upper)
+        hipLaunchKernelGGL((potrf_diag_kernel<T, true, PW>), dim3(1), dim3(64), 0, s, A, n, lda,
+                           j0, info, abort, global_offset);
+      else
+        hipLaunchKernelGGL((potrf_diag_kernel<T, false, PW>), dim3(1), dim3(64), 0, s, A, n, lda,
+                           j0, info, abort, global_offset);
+      DM_LAUNCH_CHECK("potrf_diag_kernel");
+    }
+    diag_done = false;
     const int kb = n - j0 < PW ? n - j0 : PW;
     const int rest = n - j0 - kb;
     if (rest <= 0) break;  // a block narrower than PW is the last one
@@ -216,6 +131,12 @@ int potrf_panels(bool upper, int n, T* A, int lda, int* info, int* abort, int gl
                          j0, info, abort);
     DM_LAUNCH_CHECK("potrf_row_kernel");
     int rc;
+    if (PW == 64) {  // trailing update + next diagonal block in one launch
+      rc = potrf_update_launch<T>(upper, n, A, lda, j0, info, abort, global_offset, s);
+      if (rc) return rc;
+      diag_done = true;
+      continue;
+    }
     if (upper) {
       const T* A12 = A + (size_t)(j0 + kb) * lda + j0;
       T* A22 = A + (size_t)(j0 + kb) * lda + j0 + kb;

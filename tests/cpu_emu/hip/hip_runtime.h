@@ -141,6 +141,10 @@ inline int readlane(int v, int l) {
 #define DM_DYNAMIC_LDS(name) unsigned char* name = ::emu::dynamic_shared()
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+// Reciprocal square root estimates: the exact value perturbed by a relative
+// 2^-22 (FP64) / 2^-20 (FP32), so the callers' Newton refinement is exercised.
+#define __builtin_amdgcn_rsq(x) ((1.0 / std::sqrt((double)(x))) * (1.0 + 0x1p-22))
+#define __builtin_amdgcn_rsqf(x) ((1.0f / std::sqrt((float)(x))) * (1.0f + 0x1p-20f))
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) ::emu::mfma_f64_16x16x4((a), (b), (c))
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) ::emu::mfma_f32_16x16x4((a), (b), (c))
 #define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) \
