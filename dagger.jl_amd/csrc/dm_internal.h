@@ -204,6 +204,12 @@ int trsm_full(char side, char uplo, char transa, char diag, int m, int n,
 template <typename T>
 int potrf_update_launch(bool upper, int n, T* A, int lda, int j0, int* info, int* abort,
                         int global_base, hipStream_t s);
+// C = beta C + alpha A X on tiled FP64 operands where X and C are one block
+// column of at most 64 columns (mixed.hip): one launch, A read once.
+bool tiled_skinny_gemm_ok(char ta, char tb, const dm_tiled& A, const dm_tiled& X,
+                          const dm_tiled& Cm);
+int tiled_skinny_gemm_launch(const dm_tiled& A, const dm_tiled& X, const dm_tiled& Cm,
+                             double alpha, double beta, hipStream_t s);
 template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
                int global_offset, T* ws, hipStream_t s);

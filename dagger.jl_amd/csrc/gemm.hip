@@ -285,6 +285,105 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __
 }
 
 
+// Skinny products (N <= 64: solves with a few right-hand sides).  One
+// workgroup owns 32 rows x all N columns of C; its four waves split K
+// (wave w takes the 16-k blocks w, w+4, ...) and their partial sums are
+// added in wave order through LDS, so the result is deterministic.  Inside
+// a 16-k block, lane group l/16 feeds k = 4(l/16) + s to MFMA step s: both
+// operands use the same k slots, and each lane walks 4 consecutive k,
+// which is contiguous memory for A^T and for a column-major X.  A K-long
+// chain per workgroup shrinks by 4 against the 64 x 64 tile, which matters
+// for the latency-bound launches of a blocked substitution.
+template <typename T, bool TA, bool TB, int NB>
+__global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __restrict__ jobs,
+                                                          int njobs, GemmJob<T> single, int total,
+                                                          const int* __restrict__ abort) {
+  typedef typename Mfma<T>::v4 v4;
+  __shared__ T red[3][2 * NB * 4][64];
+  if (abort && *abort) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (TileRange rg = tile_range(total); rg.t < rg.end; rg.t += rg.step) {
+    GemmJob<T> job;
+    int tm, tn;
+    locate_tile(jobs, njobs, single, rg.t, job, tm, tn, false);
+    const int M = job.M, N = job.N, K = job.K;
+    const int m0 = tm * 32;
+    v4 acc[2][NB];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[rb][b] = v4{0, 0, 0, 0};
+    const int nkb = (K + 15) / 16;
+    for (int kb = w; kb < nkb; kb += 4) {
+      const int kbase = kb * 16 + 4 * lk;
+      T a[2][4], x[NB][4];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int i = m0 + 16 * rb + lr;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = kbase + s;
+          const bool ok = i < M && k < K;
+          a[rb][s] = ok ? (TA ? job.A[(size_t)i * job.lda + k] : job.A[(size_t)k * job.lda + i])
+                        : T(0);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int j = 16 * b + lr;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = kbase + s;
+          const bool ok = j < N && k < K;
+          x[b][s] = ok ? (TB ? job.B[(size_t)k * job.ldb + j] : job.B[(size_t)j * job.ldb + k])
+                       : T(0);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) acc[rb][b] = Mfma<T>::mac(a[rb][s], x[b][s], acc[rb][b]);
+    }
+    // partial sums of waves 1..3 through LDS, added to wave 0's in wave order
+    if (w > 0) {
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[w - 1][(rb * NB + b) * 4 + r][lane] = acc[rb][b][r];
+    }
+    __syncthreads();
+    if (w == 0) {
+      const T alpha = job.alpha, beta = job.beta;
+      const int tri = job.tri;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int q = (rb * NB + b) * 4 + r;
+            const T v = ((acc[rb][b][r] + red[0][q][lane]) + red[1][q][lane]) + red[2][q][lane];
+            // result lane l, item r: row Mfma<T>::row(l/16, r), column l%16 of the 16x16 block
+            const int m = m0 + 16 * rb + Mfma<T>::row(lk, r);
+            const int n = 16 * b + lr;
+            bool ok = m < M && n < N;
+            if (tri == 1) ok = ok && m <= n;
+            else if (tri == 2) ok = ok && m >= n;
+            if (ok) {
+              T* c = job.C + (size_t)n * job.ldc + m;
+              *c = (beta == T(0)) ? alpha * v : alpha * v + beta * *c;
+            }
+          }
+    }
+    __syncthreads();  // LDS is rewritten by the next tile
+  }
+}
+
 // Trailing update of one POTRF panel fused with the factorization of the
 // next diagonal block.  The SYRK's packed triangle tiles are 64 x 64, so
 // tile 0 is exactly the next 64 x 64 diagonal block: workgroup 0 updates it
@@ -323,12 +422,16 @@ __global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __
 // global latency that a single 64x64-per-wave workgroup leaves exposed.
 namespace {
 constexpr int GEMM_CFG_256x128 = 4;
+constexpr int GEMM_CFG_SKINNY = 5;  // 32 rows x bn (16..64) columns, K split over 4 waves
 }
 
 template <typename T>
 GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs) {
   (void)K;
   const long nj = njobs > 0 ? njobs : 1;
+  // few columns: the K-split skinny kernel (at 64 columns the 64 x 64 tile
+  // measured slightly faster)
+  if (N <= 48) return GemmCfgId{GEMM_CFG_SKINNY, 32, 16 * ((N + 15) / 16)};
   if (sizeof(T) == 8) {
     // Largest tile whose launch still puts about one workgroup on every CU.
     const long t256 = (long)((M + 255) / 256) * ((N + 127) / 128) * nj;
@@ -348,7 +451,7 @@ long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg) {
     j.tiles_n = (j.N + cfg.bn - 1) / cfg.bn;
     j.wg_begin = (int)total;
     long cnt;
-    if (j.tri == 0 || cfg.bm != cfg.bn) cnt = (long)j.tiles_m * j.tiles_n;
+    if (j.tri == 0 || cfg.bm != cfg.bn || cfg.id == GEMM_CFG_SKINNY) cnt = (long)j.tiles_m * j.tiles_n;
     else cnt = (long)j.tiles_m * (j.tiles_m + 1) / 2;  // square tiles of a square (M == N) C
     if (j.M <= 0 || j.N <= 0) cnt = 0;
     total += cnt;
@@ -384,6 +487,32 @@ int launch_cfg(int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& sing
     return launch_one<T, 64, 64, 16, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
   return arg_fail(0, "gemm: unknown configuration");
 }
+template <typename T, bool TA, bool TB>
+int launch_skinny(int bn, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total,
+                  long grid, const int* abort, hipStream_t s) {
+  switch (bn) {
+    case 16:
+      hipLaunchKernelGGL((gemm_skinny_kernel<T, TA, TB, 1>), dim3((unsigned)grid), dim3(256), 0, s,
+                         jobs, njobs, single, (int)total, abort);
+      break;
+    case 32:
+      hipLaunchKernelGGL((gemm_skinny_kernel<T, TA, TB, 2>), dim3((unsigned)grid), dim3(256), 0, s,
+                         jobs, njobs, single, (int)total, abort);
+      break;
+    case 48:
+      hipLaunchKernelGGL((gemm_skinny_kernel<T, TA, TB, 3>), dim3((unsigned)grid), dim3(256), 0, s,
+                         jobs, njobs, single, (int)total, abort);
+      break;
+    case 64:
+      hipLaunchKernelGGL((gemm_skinny_kernel<T, TA, TB, 4>), dim3((unsigned)grid), dim3(256), 0, s,
+                         jobs, njobs, single, (int)total, abort);
+      break;
+    default:
+      return arg_fail(0, "gemm: skinny width must be 16, 32, 48 or 64");
+  }
+  DM_LAUNCH_CHECK("gemm_skinny_kernel");
+  return 0;
+}
 template <typename T, int VEC>
 int launch_tt(bool ta, bool tb, int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single,
               long total, long grid, const int* abort, hipStream_t s) {
@@ -417,6 +546,12 @@ int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* job
       return gemm_split_launch(cfg.id == GEMM_CFG_BF16X6 ? 6 : 3, jobs_dev, njobs, single,
                                total_wg, grid, abort, s);
     return arg_fail(0, "gemm: BF16 split configurations are FP32 only");
+  }
+  if (cfg.id == GEMM_CFG_SKINNY) {
+    if (ta && tb) return launch_skinny<T, true, true>(cfg.bn, jobs_dev, njobs, single, total_wg, grid, abort, s);
+    if (ta) return launch_skinny<T, true, false>(cfg.bn, jobs_dev, njobs, single, total_wg, grid, abort, s);
+    if (tb) return launch_skinny<T, false, true>(cfg.bn, jobs_dev, njobs, single, total_wg, grid, abort, s);
+    return launch_skinny<T, false, false>(cfg.bn, jobs_dev, njobs, single, total_wg, grid, abort, s);
   }
   if (vec) return launch_tt<T, 2>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, grid, abort, s);
   return launch_tt<T, 1>(ta, tb, cfg.id, jobs_dev, njobs, single, total_wg, grid, abort, s);

@@ -137,6 +137,162 @@ int convert_slab(void* dst, int dst_dtype, const void* src, int src_dtype, size_
 void build_gemm_tiled(Dag& d, char ta, char tb, double alpha, const dm_tiled& A,
                       const dm_tiled& B, double beta, const dm_tiled& Cm);
 
+
+// ---------------------------------------------------------------- residual
+// C = beta C + alpha A X for a tiled FP64 A (m x k, square blocks mb with
+// mb % 128 == 0) and X, C of one block column (n <= 64): the FP64 residual
+// r = b - A x of iterative refinement in one launch.  The tile DAG form
+// accumulates C_i += A_ik X_k one k after another (a chain of mt launches);
+// here each workgroup owns 64 rows of C over the whole k range, so A is
+// streamed from HBM once (about 1 ms for N = 32768) at 2n flops per element.
+//  * 4 waves, each 16 rows x n columns on v_mfma_f64_16x16x4 (A operand:
+//    lane l holds A[row l%16][k l/16]; X operand: X[k l/16][col l%16];
+//    result lane l, item r = D[4r + l/16][l%16]);
+//  * X is staged through LDS in chunks of 128 k (double-buffered, row
+//    stride n_pad + 1), A is loaded straight into registers one chunk ahead.
+// Each output has one owner and a fixed summation order (deterministic).
+constexpr int SK_KC = 128;
+
+template <int NB16>
+struct Skinny {
+  static constexpr int NP = 16 * NB16, SX = NP + 1;
+  static constexpr int XPT = SK_KC * NP / 256;  // X elements per thread per chunk
+};
+
+template <int NB16>
+__global__ __launch_bounds__(256) void tiled_skinny_gemm_kernel(const double* __restrict__ A,
+                                                                const double* __restrict__ X,
+                                                                double* __restrict__ Cm,
+                                                                int64_t m, int64_t k, int n,
+                                                                int mb, double alpha,
+                                                                double beta) {
+  typedef Skinny<NB16> S;
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __shared__ double xs[2][SK_KC * S::SX];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int64_t ti = r0 / mb;
+  const int lrow0 = (int)(r0 - ti * mb);
+  const int rows_ti = (int)(m - ti * mb < mb ? m - ti * mb : mb);
+  const int arow = lrow0 + 16 * w + lr;  // this lane's A row (local to tile row ti)
+  const bool rowok = arow < rows_ti;
+  const int64_t nch = (k + SK_KC - 1) / SK_KC;
+
+  // chunk c: global k [c*KC, c*KC + KC) inside tile column tk = c*KC / mb
+  auto a_load = [&](int64_t c, double (&a)[SK_KC / 4]) {
+    const int64_t kg0 = c * SK_KC, tk = kg0 / mb;
+    const int kl0 = (int)(kg0 - tk * mb);
+    const int cols = (int)(k - tk * mb < mb ? k - tk * mb : mb);
+    const double* Ap = A + tk * (int64_t)mb * m + ti * (int64_t)mb * cols;
+#pragma unroll
+    for (int q = 0; q < SK_KC / 4; ++q) {
+      const int kl = kl0 + 4 * q + lk;
+      a[q] = (rowok && kl < cols) ? Ap[(int64_t)kl * rows_ti + arow] : 0.0;
+    }
+  };
+  auto x_load = [&](int64_t c, double (&xr)[S::XPT]) {
+    const int64_t kg0 = c * SK_KC, tk = kg0 / mb;
+    const int kl0 = (int)(kg0 - tk * mb);
+    const int ldx = (int)(k - tk * mb < mb ? k - tk * mb : mb);
+    const double* Xp = X + tk * (int64_t)mb * n;
+#pragma unroll
+    for (int q = 0; q < S::XPT; ++q) {
+      const int e = tid + 256 * q;
+      const int kl = e % SK_KC, j = e / SK_KC;
+      xr[q] = (kl0 + kl < ldx && j < n) ? Xp[(int64_t)j * ldx + kl0 + kl] : 0.0;
+    }
+  };
+  auto x_store = [&](double* dst, const double (&xr)[S::XPT]) {
+#pragma unroll
+    for (int q = 0; q < S::XPT; ++q) {
+      const int e = tid + 256 * q;
+      dst[(e % SK_KC) * S::SX + e / SK_KC] = xr[q];
+    }
+  };
+
+  v4 acc[NB16];
+#pragma unroll
+  for (int b = 0; b < NB16; ++b) acc[b] = v4{0, 0, 0, 0};
+  double an[SK_KC / 4], ac[SK_KC / 4], xr[S::XPT];
+  a_load(0, an);
+  x_load(0, xr);
+  x_store(xs[0], xr);
+  __syncthreads();
+  for (int64_t c = 0; c < nch; ++c) {
+#pragma unroll
+    for (int q = 0; q < SK_KC / 4; ++q) ac[q] = an[q];
+    const bool more = c + 1 < nch;
+    if (more) {
+      a_load(c + 1, an);
+      x_load(c + 1, xr);
+    }
+    const double* xb = xs[c & 1];
+#pragma unroll
+    for (int q = 0; q < SK_KC / 4; ++q) {
+#pragma unroll
+      for (int b = 0; b < NB16; ++b) {
+        const double xf = xb[(4 * q + lk) * S::SX + 16 * b + lr];
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[q], xf, acc[b], 0, 0, 0);
+      }
+    }
+    if (more) x_store(xs[(c + 1) & 1], xr);
+    __syncthreads();
+  }
+  double* Cp = Cm + ti * (int64_t)mb * n;  // C tile (ti, 0), ld = rows_ti
+#pragma unroll
+  for (int b = 0; b < NB16; ++b) {
+    const int col = 16 * b + lr;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = lrow0 + 16 * w + 4 * r + lk;
+      if (row < rows_ti && col < n) {
+        double* p = Cp + (int64_t)col * rows_ti + row;
+        const double v = alpha * acc[b][r];
+        *p = beta == 0.0 ? v : v + beta * *p;
+      }
+    }
+  }
+}
+
+bool tiled_skinny_gemm_ok(char ta, char tb, const dm_tiled& A, const dm_tiled& X,
+                          const dm_tiled& Cm) {
+  return ta == 'N' && tb == 'N' && A.dtype == DM_F64 && X.dtype == DM_F64 && Cm.dtype == DM_F64 &&
+         A.mb == A.nb && A.mb % SK_KC == 0 && X.mb == A.mb && X.nb == A.mb && Cm.mb == A.mb &&
+         X.n == Cm.n && Cm.n >= 1 && Cm.n <= 64 && X.n <= X.nb && Cm.n <= Cm.nb &&
+         A.m == Cm.m && A.n == X.m;
+}
+
+int tiled_skinny_gemm_launch(const dm_tiled& A, const dm_tiled& X, const dm_tiled& Cm,
+                             double alpha, double beta, hipStream_t s) {
+  const int n = (int)Cm.n;
+  const unsigned grid = (unsigned)((A.m + 63) / 64);
+  if (grid == 0) return 0;
+  const double* a = (const double*)A.base;
+  const double* x = (const double*)X.base;
+  double* c = (double*)Cm.base;
+  switch ((n + 15) / 16) {
+    case 1:
+      hipLaunchKernelGGL(tiled_skinny_gemm_kernel<1>, dim3(grid), dim3(256), 0, s, a, x, c, A.m,
+                         A.n, n, A.mb, alpha, beta);
+      break;
+    case 2:
+      hipLaunchKernelGGL(tiled_skinny_gemm_kernel<2>, dim3(grid), dim3(256), 0, s, a, x, c, A.m,
+                         A.n, n, A.mb, alpha, beta);
+      break;
+    case 3:
+      hipLaunchKernelGGL(tiled_skinny_gemm_kernel<3>, dim3(grid), dim3(256), 0, s, a, x, c, A.m,
+                         A.n, n, A.mb, alpha, beta);
+      break;
+    default:
+      hipLaunchKernelGGL(tiled_skinny_gemm_kernel<4>, dim3(grid), dim3(256), 0, s, a, x, c, A.m,
+                         A.n, n, A.mb, alpha, beta);
+      break;
+  }
+  DM_LAUNCH_CHECK("tiled_skinny_gemm_kernel");
+  return 0;
+}
+
 }  // namespace dm
 
 using namespace dm;

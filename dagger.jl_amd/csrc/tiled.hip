@@ -466,6 +466,10 @@ int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A, con
   if (Cm->m == 0 || Cm->n == 0) return 0;
   const dm_plan_opts o = opts_or_default(opts);
   const char ta = is_trans(transa) ? 'T' : 'N', tb = is_trans(transb) ? 'T' : 'N';
+  // A times a block column of at most 64 columns (the residual of
+  // refinement): one streaming launch instead of the k-ordered tile DAG
+  if (tiled_skinny_gemm_ok(ta, tb, *A, *B, *Cm))
+    return tiled_skinny_gemm_launch(*A, *B, *Cm, alpha, beta, stream_of(s));
   int64_t ab, bb;
   std::memcpy(&ab, &alpha, sizeof ab);
   std::memcpy(&bb, &beta, sizeof bb);

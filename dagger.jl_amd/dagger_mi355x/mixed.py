@@ -51,8 +51,8 @@ class _DiagInverseSolve:
     inverse's conditioning, which refinement absorbs like the FP32 factor's
     own error."""
 
-    def __init__(self, F, options, stream):
-        self.F, self.options, self.stream = F, options, stream
+    def __init__(self, F, options):
+        self.F, self.options = F, options
         nb = F.mb
         self.Dinv = DArray(np.float32, (F.m, nb), Blocks(nb, nb))
         tasks = [make_task(BLAS.trsm, ("L", "U", "N", "N", 1.0, In(F.tile(k, k)),
@@ -61,11 +61,11 @@ class _DiagInverseSolve:
         self._inv = Dag(tasks).plan(options.replace(use_graph=False))
         self._solves = {}
 
-    def invert(self):
+    def invert(self, stream):
         """Dinv_k := U_kk^-1 for every diagonal tile (after each refactor)."""
         t = self.Dinv.tiled()
-        call("dm_tiled_identity", C.byref(t), self.stream)
-        self._inv.run(self.stream, sync=False)
+        call("dm_tiled_identity", C.byref(t), stream)
+        self._inv.run(stream, sync=False)
 
     def _dag(self, R, Y):
         key = (R.buffer.ptr, Y.buffer.ptr)
@@ -90,9 +90,9 @@ class _DiagInverseSolve:
             self._solves[key] = d
         return d
 
-    def solve(self, R, Y):
-        """R := F32^-1 R (Y: scratch of R's shape)."""
-        self._dag(R, Y).run(self.stream, sync=False)
+    def solve(self, R, Y, stream):
+        """R := F32^-1 R on `stream` (Y: scratch of R's shape)."""
+        self._dag(R, Y).run(stream, sync=False)
 
 
 def _convert(dst, src, stream=None):
@@ -135,10 +135,8 @@ class MixedCholesky:
         self.info, self.ginfo = aux["info"], aux["ginfo"]
         self._work, self._norm, self._bufs = aux["work"], aux["norm"], aux["bufs"]
         if aux.get("isolve") is None:
-            aux["isolve"] = _DiagInverseSolve(self.F, self.options.replace(f32_update=0),
-                                              self.stream)
+            aux["isolve"] = _DiagInverseSolve(self.F, self.options.replace(f32_update=0))
         self._isolve = aux["isolve"]
-        self._isolve.stream = self.stream
         self.refactor()
 
     def refactor(self):
@@ -152,7 +150,7 @@ class MixedCholesky:
         info = int(self.info.get(s)[0])
         if info != 0:
             raise PosDefException(info)
-        self._isolve.invert()
+        self._isolve.invert(s)
         ta = self.A.tiled()
         call("dm_tiled_norm_inf", C.byref(ta), C.c_void_p(self._norm.ptr),
              C.c_void_p(self._work.ptr), s)
@@ -163,7 +161,11 @@ class MixedCholesky:
 
     # ---------------------------------------------------------------- pieces
     def _buffers(self, nrhs):
-        b = self._bufs.get(nrhs)
+        # per right-hand-side count and per stream: callers sharing one
+        # factor from different streams (the serving form) never share
+        # scratch
+        key = (nrhs, getattr(self.stream, "value", self.stream))
+        b = self._bufs.get(key)
         if b is None:
             blk = Blocks(self.nb, self.nb)
             b = dict(X=DArray(np.float64, (self.A.m, nrhs), blk),
@@ -171,13 +173,13 @@ class MixedCholesky:
                      D=DArray(np.float32, (self.A.m, nrhs), blk),
                      Y=DArray(np.float32, (self.A.m, nrhs), blk),
                      nrm=DeviceBuffer(8 * max(nrhs, 1)))
-            self._bufs[nrhs] = b
+            self._bufs[key] = b
         return b
 
     def _solve32(self, D):
         """D := F32^-1 D (FP32; D one of the per-nrhs buffers)."""
         buf = self._buffers(D.n)
-        self._isolve.solve(D, buf["Y"])
+        self._isolve.solve(D, buf["Y"], self.stream)
 
     def residual(self, B, X, R):
         """R = B - A X in FP64."""

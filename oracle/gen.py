@@ -9,7 +9,7 @@ TEST INFRASTRUCTURE ONLY (tests/, smoke(), bench.py's cpu_baseline leg).
 * randn: two 53-bit uniforms in (0,1) from the four words, Box-Muller
   z = sqrt(-2 ln u) cos(2 pi v).  Equal to the device up to the last-ulp
   differences of log/cos between libm implementations.
-* spd(n, seed): A = G G^T / n + I (the BASELINE generator for the plain
+* spd(n, seed): A = G G^T / n + I (the input generator of the plain
   Cholesky configurations).
 """
 import numpy as np

@@ -440,6 +440,18 @@ def quick():
     DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
     dm.ldiv_(Fc, DB)
     check(O.backward_error(A, DB.collect(), b) < 10 * n * np.finfo(float).eps, "ldiv")
+    for (m, nb, nrhs) in [(300, 128, 9), (256, 128, 40)]:  # streaming residual kernel
+        Am = rng.standard_normal((m, m))
+        X = rng.standard_normal((m, nrhs))
+        Y = rng.standard_normal((m, nrhs))
+        DA = dm.DArray.from_array(Am, dm.Blocks(nb, nb))
+        DX = dm.DArray.from_array(X, dm.Blocks(nb, nb))
+        DY = dm.DArray.from_array(Y, dm.Blocks(nb, nb))
+        ta, tx, ty = DA.tiled(), DX.tiled(), DY.tiled()
+        o = dm.PlanOptions().c()
+        call("dm_gemm_tiled", b"N", b"N", -1.0, C.byref(ta), C.byref(tx), 0.5, C.byref(ty),
+             C.byref(o), None)
+        check(relerr(DY.collect(), 0.5 * Y - Am @ X) < 1e-13, ("gemm block column", m, nrhs))
     from dagger_mi355x import batched as Bt
     bs = Bt.BatchedSystems([40, 70, 33], [2, 1, 3], np.float64).generate(seed=100)
     A0s = [bs.get_system(m) for m in range(bs.count)]

@@ -62,6 +62,16 @@ bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
   return ta && !tb && !((uintptr_t)j.A & 15) && !((uintptr_t)j.B & 15) && !(j.lda & 3) &&
          !(j.ldb & 3);
 }
+// the streaming block-column product is a device kernel: the planner tests
+// take the tile DAG path
+bool tiled_skinny_gemm_ok(char, char, const dm_tiled&, const dm_tiled&, const dm_tiled&) {
+  return false;
+}
+int tiled_skinny_gemm_launch(const dm_tiled&, const dm_tiled&, const dm_tiled&, double, double,
+                             hipStream_t) {
+  ++g_launch;
+  return 0;
+}
 int gemm_split_launch(int terms, const GemmJob<float>*, int, const GemmJob<float>&, long, long,
                       const int*, hipStream_t) {
   if (terms != 3 && terms != 6) abort();
