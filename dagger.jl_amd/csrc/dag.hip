@@ -416,7 +416,8 @@ int Dag::build_device_tables() {
         }
         G.batched_trsm = true;
         std::vector<TrsmStep<T>> steps;
-        trsm_plan_steps<T>(G.shape, jobs, steps);
+        const bool crit_group = opts.lookahead > 0 && opts.num_streams >= 2 && G.crit == 0;
+        trsm_plan_steps<T>(G.shape, jobs, steps, crit_group ? trsm_crit_leaf_len() : 0);
         for (TrsmStep<T>& st : steps) {
           Group::Step rec;
           rec.leaf = st.leaf;

@@ -185,9 +185,11 @@ struct TrsmStep {
   int maxL = 0;
   long total_wg = 0;
 };
+// leaf: recursion leaf length (0: trsm_leaf_len())
 template <typename T>
 void trsm_plan_steps(const TrsmShape& sh, const std::vector<TrsmJob<T>>& jobs,
-                     std::vector<TrsmStep<T>>& steps);
+                     std::vector<TrsmStep<T>>& steps, int leaf = 0);
+int trsm_crit_leaf_len();  // leaf length for critical-stream solves
 
 // Full BLAS trsm (any size; host recursion above TRSM_LEAF, inverse blocks
 // formed per leaf in ws).

@@ -40,9 +40,9 @@ TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
 // Batched planning: jobs of equal length L.
 template <typename T>
 static void plan_rec(const TrsmShape& sh, int L, const std::vector<TrsmJob<T>>& jobs,
-                     std::vector<TrsmStep<T>>& steps) {
+                     std::vector<TrsmStep<T>>& steps, int leaf) {
   if (jobs.empty() || L <= 0) return;
-  if (L <= trsm_leaf_len()) {
+  if (L <= leaf) {
     TrsmStep<T> st;
     st.leaf = true;
     st.maxL = L;
@@ -76,24 +76,37 @@ static void plan_rec(const TrsmShape& sh, int L, const std::vector<TrsmJob<T>>& 
     c.alpha = T(1);
     j2.push_back(c);
   }
-  plan_rec<T>(sh, sp.first_n, j1, steps);
+  plan_rec<T>(sh, sp.first_n, j1, steps, leaf);
   steps.push_back(std::move(up));
-  plan_rec<T>(sh, sp.second_n, j2, steps);
+  plan_rec<T>(sh, sp.second_n, j2, steps, leaf);
 }
 
 template <typename T>
 void trsm_plan_steps(const TrsmShape& sh, const std::vector<TrsmJob<T>>& jobs,
-                     std::vector<TrsmStep<T>>& steps) {
+                     std::vector<TrsmStep<T>>& steps, int leaf) {
+  if (leaf < TRSM_BS || leaf > TRSM_LEAF || leaf % TRSM_BS) leaf = trsm_leaf_len();
   // one recursion per distinct length (a group normally has one)
   std::map<int, std::vector<TrsmJob<T>>> byL;
   for (const TrsmJob<T>& j : jobs)
     if (j.L > 0 && j.NV > 0) byL[j.L].push_back(j);
-  for (auto& kv : byL) plan_rec<T>(sh, kv.first, kv.second, steps);
+  for (auto& kv : byL) plan_rec<T>(sh, kv.first, kv.second, steps, leaf);
+}
+
+int trsm_crit_leaf_len() {
+  // Leaf length of the solves on a look-ahead schedule's critical stream
+  // (environment DM_TRSM_LEAF_CRIT, a multiple of 32 in [64, TRSM_LEAF];
+  // by default the general leaf length).
+  static const int len = [] {
+    const char* e = std::getenv("DM_TRSM_LEAF_CRIT");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 64 && v <= TRSM_LEAF && v % TRSM_BS == 0) ? v : trsm_leaf_len();
+  }();
+  return len;
 }
 
 template void trsm_plan_steps<double>(const TrsmShape&, const std::vector<TrsmJob<double>>&,
-                                      std::vector<TrsmStep<double>>&);
+                                      std::vector<TrsmStep<double>>&, int);
 template void trsm_plan_steps<float>(const TrsmShape&, const std::vector<TrsmJob<float>>&,
-                                     std::vector<TrsmStep<float>>&);
+                                     std::vector<TrsmStep<float>>&, int);
 
 }  // namespace dm
