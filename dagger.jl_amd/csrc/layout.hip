@@ -128,45 +128,68 @@ extern "C" int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_d
 // max(||A_ij||_F, ||A_ji||_F) (is_cross_hermitian, src/array/linalg.jl:38-48).
 // One workgroup per tile pair; fixed-order reductions.
 namespace dm {
+// One workgroup (16 waves) per tile pair (i >= j), walking the pair in 64 x 64
+// sub-blocks: the sub-block of A_ij is read along its columns, the matching
+// sub-block of A_ji along its columns too and transposed through LDS, so
+// both reads are contiguous (A is streamed from HBM once).
 template <typename T>
-__global__ __launch_bounds__(256) void herm_pair_kernel(const T* __restrict__ A, int64_t m,
+__global__ __launch_bounds__(1024) void herm_pair_kernel(const T* __restrict__ A, int64_t m,
                                                         int mb, int mt,
                                                         int* __restrict__ ok_pairs) {
-  __shared__ double s0[256], s1[256], s2[256];
+  __shared__ T ts[64 * 65];
+  __shared__ double s0[1024], s1[1024], s2[1024];
   // decode pair index -> (i, j) with i >= j, column by column of the lower triangle
   int p = blockIdx.x, j = 0;
   while (p >= mt - j) { p -= mt - j; ++j; }
   const int i = j + p;
-  const int64_t ri = (i * (int64_t)mb + mb <= m) ? mb : m - i * (int64_t)mb;
-  const int64_t rj = (j * (int64_t)mb + mb <= m) ? mb : m - j * (int64_t)mb;
+  const int ri = (int)((i * (int64_t)mb + mb <= m) ? mb : m - i * (int64_t)mb);
+  const int rj = (int)((j * (int64_t)mb + mb <= m) ? mb : m - j * (int64_t)mb);
   // tile (r, c) origin for a square m x m Blocks(mb, mb) matrix
   auto tile = [&](int r, int c, int64_t cols) { return A + c * (int64_t)mb * m + r * (int64_t)mb * cols; };
   const T* Aij = tile(i, j, rj);  // ri x rj, ld = ri
   const T* Aji = tile(j, i, ri);  // rj x ri, ld = rj
+  const int tid = threadIdx.x, lo = tid & 63, hi = tid >> 6;
   double d2 = 0, n1 = 0, n2 = 0;
-  for (int64_t e = threadIdx.x; e < ri * rj; e += 256) {
-    const int64_t r = e % ri, c = e / ri;
-    const double a = (double)Aij[c * ri + r];
-    const double b = (double)Aji[r * rj + c];  // (A_ji^T)(r, c) = A_ji(c, r)
-    if (i == j) {
-      d2 += (a != b) ? 1.0 : 0.0;
-    } else {
-      d2 += (a - b) * (a - b);
-      n1 += a * a;
-      n2 += b * b;
+  for (int r0 = 0; r0 < ri; r0 += 64) {
+    for (int c0 = 0; c0 < rj; c0 += 64) {
+      // ts[rr * 65 + cc] = A_ji(c0 + cc, r0 + rr) = (A_ji^T)(r0 + rr, c0 + cc)
+      __syncthreads();  // the previous sub-block's reads are done
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = hi + 16 * q, cc = lo;
+        T v = T(0);
+        if (r0 + rr < ri && c0 + cc < rj) v = Aji[(int64_t)(r0 + rr) * rj + c0 + cc];
+        ts[rr * 65 + cc] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = lo, cc = hi + 16 * q;
+        if (r0 + rr < ri && c0 + cc < rj) {
+          const double a = (double)Aij[(int64_t)(c0 + cc) * ri + r0 + rr];
+          const double b = (double)ts[rr * 65 + cc];
+          if (i == j) {
+            d2 += (a != b) ? 1.0 : 0.0;
+          } else {
+            d2 += (a - b) * (a - b);
+            n1 += a * a;
+            n2 += b * b;
+          }
+        }
+      }
     }
   }
-  s0[threadIdx.x] = d2; s1[threadIdx.x] = n1; s2[threadIdx.x] = n2;
+  s0[tid] = d2; s1[tid] = n1; s2[tid] = n2;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s0[threadIdx.x] += s0[threadIdx.x + w];
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] += s2[threadIdx.x + w];
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) {
+      s0[tid] += s0[tid + w];
+      s1[tid] += s1[tid + w];
+      s2[tid] += s2[tid + w];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     int ok;
     if (i == j) ok = s0[0] == 0.0;
     else {
@@ -205,10 +228,10 @@ extern "C" int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* wor
   const int pairs = mt * (mt + 1) / 2;
   if (pairs == 0) return fill_i32_launch(result_dev, 1, 1, st);
   if (A->dtype == DM_F64)
-    hipLaunchKernelGGL((herm_pair_kernel<double>), dim3(pairs), dim3(256), 0, st,
+    hipLaunchKernelGGL((herm_pair_kernel<double>), dim3(pairs), dim3(1024), 0, st,
                        (const double*)A->base, A->m, A->mb, mt, work_dev);
   else
-    hipLaunchKernelGGL((herm_pair_kernel<float>), dim3(pairs), dim3(256), 0, st,
+    hipLaunchKernelGGL((herm_pair_kernel<float>), dim3(pairs), dim3(1024), 0, st,
                        (const float*)A->base, A->m, A->mb, mt, work_dev);
   DM_LAUNCH_CHECK("herm_pair_kernel");
   hipLaunchKernelGGL(all_ok_kernel, dim3(1), dim3(256), 0, st, work_dev, pairs, result_dev);
