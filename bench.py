@@ -177,6 +177,54 @@ def mixed_leg(dm, n, nb, kappa, nrhs, seed):
     return res
 
 
+def batched_leg(dm, count, seed, reps=3):
+    """Batching form: `count` uneven SPD systems A = G G^T/n + I with
+    n ~ LogUniform[32, 2048] and nrhs ~ U{1..8}, factored and solved by one
+    persistent launch (dm_batched_posv), FP64 and FP32, panel width per size
+    bucket from the autotuner.  Timed with HIP events, systems restored from
+    device copies between runs; reports systems/s and (sum n^3/3 + sum
+    2 n^2 nrhs) / time, plus the largest backward error of 8 sampled systems
+    (numpy on the host copies)."""
+    from dagger_mi355x import batched
+    call = dm._lib.call
+    timer = Timer(call)
+    sizes, nrhs = batched.log_uniform_sizes(count, 32, 2048, seed)
+    out = {"systems": count, "n_range": [32, 2048], "nrhs_range": [1, 8]}
+    for dt, name in ((np.float64, "fp64"), (np.float32, "fp32")):
+        bs = batched.BatchedSystems(sizes, nrhs, dt).generate(seed=seed + 1)
+        picks = [int(i) for i in np.linspace(0, count - 1, 8)]
+        originals = {m: bs.get_system(m) for m in picks}
+        panels, _ = batched.autotune(bs, sample=16, reps=1)
+        snap = bs.snapshot()
+        times = []
+        for rep in range(reps + 1):
+            bs.restore(snap)
+            call("dm_stream_sync", None)
+            timer.start()
+            bs.posv(panels=panels)
+            ms = timer.stop_ms()
+            if rep:
+                times.append(ms)
+        bad = int((bs.infos() != 0).sum())
+        berr = 0.0
+        for m in picks:
+            A0, B0 = originals[m]
+            _, X = bs.get_system(m)
+            if B0.shape[1]:
+                A64, X64, B64 = A0.astype(np.float64), X.astype(np.float64), B0.astype(np.float64)
+                r = B64 - A64 @ X64
+                den = np.abs(A64).sum(1).max() * np.abs(X64).max(0) + np.abs(B64).max(0)
+                berr = max(berr, float((np.abs(r).max(0) / den).max()))
+        ff, fs = bs.flops()
+        ms = sum(times) / len(times)
+        out[name] = {"ms": round(ms, 3), "systems_per_s": round(count / (ms * 1e-3), 1),
+                     "tflops": round((ff + fs) / (ms * 1e-3) / 1e12, 3), "failed": bad,
+                     "max_sampled_berr": berr, "panels": {str(k): v for k, v in panels.items()}}
+        for buf in (*snap, bs.A, bs.B):
+            buf.free()
+    return out
+
+
 def serving_leg(dm, rate, count, seed, workers=4):
     """Serving form vs one-at-a-time plain FP64 on the same Poisson trace."""
     from dagger_mi355x import serving
@@ -236,6 +284,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=120,
                     help="requests in the serving leg (0 disables it)")
     ap.add_argument("--serving-rate", type=float, default=1000.0)
+    ap.add_argument("--batched-count", type=int, default=4096,
+                    help="systems in the batched leg (0 disables it)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); this build has no "
@@ -308,6 +358,10 @@ def main():
     if a.serving_requests > 0:
         serving_res = serving_leg(dm, a.serving_rate, a.serving_requests, a.seed + 11)
         log(f"serving: {serving_res}")
+    batched_res = None
+    if a.batched_count > 0:
+        batched_res = batched_leg(dm, a.batched_count, a.seed + 13)
+        log(f"batched: {batched_res}")
     cpu = None
     if a.cpu_n > 0:
         cpu = cpu_baseline(dm, a.cpu_n, nb, a.seed)
@@ -340,6 +394,7 @@ def main():
         "groups_by_op": by_op,
         "mixed_ir": mixed_res,
         "serving": serving_res,
+        "batched": batched_res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
