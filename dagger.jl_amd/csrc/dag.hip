@@ -362,7 +362,7 @@ int Dag::build_device_tables() {
           if (opts.f32_update == 3 || opts.f32_update == 6) {
             bool split = true;
             for (auto& j : jobs) split = split && gemm_split_ok(ta, tb, j);
-            if (split) G.cfg = gemm_split_cfg(opts.f32_update);
+            if (split) G.cfg = gemm_split_cfg(opts.f32_update, maxM, maxN, G.njobs);
           }
         }
         G.total_wg = gemm_prepare_jobs<T>(jobs.data(), G.njobs, G.cfg);

@@ -114,14 +114,18 @@ int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A,
                 const int* abort, hipStream_t s);
 
 // FP32 updates on the BF16 matrix cores with operand splitting
-// (gemm_split.hip).  GemmCfgId ids for FP32 launches; jobs tile 128x128.
+// (gemm_split.hip).  GemmCfgId ids for FP32 launches; workgroup tiles of
+// 256x128 for launches of at least about one tile per CU, else 128x128.
 constexpr int GEMM_CFG_BF16X3 = 2, GEMM_CFG_BF16X6 = 3;
-inline GemmCfgId gemm_split_cfg(int terms) {
-  return GemmCfgId{terms == 6 ? GEMM_CFG_BF16X6 : GEMM_CFG_BF16X3, 128, 128};
+inline GemmCfgId gemm_split_cfg(int terms, int M, int N, int njobs) {
+  const int id = terms == 6 ? GEMM_CFG_BF16X6 : GEMM_CFG_BF16X3;
+  const long nj = njobs > 0 ? njobs : 1;
+  const long t256 = (long)((M + 255) / 256) * ((N + 127) / 128) * nj;
+  return t256 >= 256 ? GemmCfgId{id, 256, 128} : GemmCfgId{id, 128, 128};
 }
 // op(A) = A^T, op(B) = B (k-contiguous operands), 16-byte aligned rows.
 bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j);
-int gemm_split_launch(int terms, const GemmJob<float>* jobs_dev, int njobs,
+int gemm_split_launch(int terms, int bm, const GemmJob<float>* jobs_dev, int njobs,
                       const GemmJob<float>& single, long total_wg, long grid,
                       const int* abort, hipStream_t s);
 

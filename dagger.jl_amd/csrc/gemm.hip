@@ -543,8 +543,8 @@ int gemm_launch(bool ta, bool tb, bool vec, GemmCfgId cfg, const GemmJob<T>* job
   const long grid = (max_wg > 0 && max_wg < total_wg) ? max_wg : total_wg;
   if (cfg.id == GEMM_CFG_BF16X3 || cfg.id == GEMM_CFG_BF16X6) {
     if constexpr (std::is_same<T, float>::value)
-      return gemm_split_launch(cfg.id == GEMM_CFG_BF16X6 ? 6 : 3, jobs_dev, njobs, single,
-                               total_wg, grid, abort, s);
+      return gemm_split_launch(cfg.id == GEMM_CFG_BF16X6 ? 6 : 3, cfg.bm, jobs_dev, njobs,
+                               single, total_wg, grid, abort, s);
     return arg_fail(0, "gemm: BF16 split configurations are FP32 only");
   }
   if (cfg.id == GEMM_CFG_SKINNY) {
@@ -561,9 +561,10 @@ template <typename T>
 bool gemm_job_vec_ok(bool ta, bool tb, const GemmJob<T>& j) { return job_vec_ok(ta, tb, j); }
 
 int gemm_wgs_per_cu(GemmCfgId cfg) {
-  // the 128x128 and 256x128 workgroups fill a CU's LDS (or, FP32 128x128,
-  // its registers); 64x64 and the BF16-split kernel fit two
-  return (cfg.id == 0 || cfg.id == GEMM_CFG_256x128) ? 1 : 2;
+  // the 128x128 and 256x128 workgroups and the 16-wave BF16-split ones
+  // fill a CU's LDS (or, FP32 128x128, its registers); 64x64 fits two
+  return (cfg.id == 0 || cfg.id == GEMM_CFG_256x128 || cfg.id == GEMM_CFG_BF16X3 ||
+          cfg.id == GEMM_CFG_BF16X6) ? 1 : 2;
 }
 
 template <typename T>

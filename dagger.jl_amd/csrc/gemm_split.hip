@@ -16,13 +16,17 @@
 // are issued first.
 //
 // Kernel shape (wave64, v_mfma_f32_16x16x32_bf16):
-//  * 128x128 C tile per workgroup, 2x2 waves of 64x64, BK = 32 (one MFMA k).
+//  * 16 waves (4 x 4) over a 256x128 (or 128x128) C tile, BK = 32 (one MFMA
+//    k); four waves per SIMD hide LDS and global latency.
 //  * Operands must be contiguous along k (C = alpha op(A) op(B) + beta C with
 //    op(A) = A^T and op(B) = B, the shapes of the upper tiled Cholesky's
-//    SYRK('U','T') and GEMM('T','N')): FP32 rows are staged global ->
-//    registers -> LDS ([mn][BK+4] floats, double buffered) and split into
-//    BF16 terms when the MFMA fragments are read, so LDS holds plain FP32
-//    (72 KB, two workgroups per CU).
+//    SYRK('U','T') and GEMM('T','N')).  FP32 rows go global -> registers,
+//    are split into their BF16 terms once per workgroup, and are stored as
+//    BF16 planes in LDS (double buffered, 144 KB for 256x128 and three
+//    terms), 16-byte chunks XOR-swizzled by row so that the fragment reads
+//    of a quarter-wave spread over the banks.  Splitting when the tile is
+//    staged (not when each wave reads a fragment) does the conversion once
+//    instead of once per wave that consumes the row.
 //  * MFMA operand layout (measured on gfx950): lane l holds row l%16,
 //    k = 8*(l/16) .. 8*(l/16)+7 of both operands; the 16x16 result D[i][j]
 //    sits in lane j + 16*(i/4), item i%4.  The MFMA computes C^T (operand
@@ -35,23 +39,28 @@ namespace dm {
 namespace {
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <int TERMS>
+template <int TERMS, int BM_, int BN_>
 struct SplitGemm {
-  static constexpr int BM = 128, BN = 128, BK = 32, WM = 2, WN = 2;
+  static constexpr int BM = BM_, BN = BN_, BK = 32, WM = 4, WN = 4;
   static constexpr int NT = 64 * WM * WN;
   static constexpr int WTM = BM / WM, WTN = BN / WN;
   static constexpr int FM = WTM / 16, FN = WTN / 16;
-  static constexpr int SK = BK + 4;  // floats per LDS row (144 B: conflict-free b128 reads)
-  static constexpr int A_ELEMS = BM * SK, B_ELEMS = BN * SK;
-  static constexpr int STAGE = A_ELEMS + B_ELEMS;
+  static constexpr int P = TERMS == 6 ? 3 : 2;                // BF16 terms per operand
+  static constexpr int A_PLANE = BM * BK, B_PLANE = BN * BK;  // BF16 elements per plane
+  static constexpr int STAGE = P * (A_PLANE + B_PLANE);
   static constexpr int LA = BM * BK / (4 * NT);  // float4 loads per thread
   static constexpr int LB = BN * BK / (4 * NT);
-  static constexpr int P = TERMS == 6 ? 3 : 2;   // BF16 terms per operand
   static_assert(LA * 4 * NT == BM * BK && LB * 4 * NT == BN * BK, "tile split");
 
-  // Row r (of op(X) with k contiguous: X[k + r*ld]) chunk k0 + 4c .. +3.
+  // 16-byte chunk c (k = 8c .. 8c+7) of row `row` inside a plane
+  static __device__ __forceinline__ int chunk_off(int row, int c) {
+    return row * BK + ((c ^ ((row >> 1) & 3)) << 3);
+  }
+
+  // Row r (of op(X) with k contiguous: X[k + r*ld]) chunk k .. k+3.
   static __device__ __forceinline__ f4 load4(const float* __restrict__ X, int ld, int r, int k,
                                              int R, int K) {
     if (r >= R) return f4{0, 0, 0, 0};
@@ -78,41 +87,52 @@ struct SplitGemm {
     }
   }
 
-  static __device__ __forceinline__ void store(float* __restrict__ st, const f4 (&ra)[LA],
+  // Split 4 consecutive k (k = 4 kq ..) of row `row` into the P planes.
+  static __device__ __forceinline__ void put(__bf16* __restrict__ planes, int plane, int row,
+                                             int kq, f4 x) {
+    bf4 h, m, l;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const __bf16 hh = (__bf16)x[e];
+      float r = x[e] - (float)hh;
+      const __bf16 mm = (__bf16)r;
+      h[e] = hh;
+      m[e] = mm;
+      if (P == 3) {
+        r -= (float)mm;
+        l[e] = (__bf16)r;
+      }
+    }
+    const int off = chunk_off(row, kq >> 1) + (kq & 1) * 4;
+    *(bf4*)(planes + off) = h;
+    *(bf4*)(planes + plane + off) = m;
+    if (P == 3) *(bf4*)(planes + 2 * plane + off) = l;
+  }
+
+  static __device__ __forceinline__ void store(__bf16* __restrict__ st, const f4 (&ra)[LA],
                                                const f4 (&rb)[LB]) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int e = threadIdx.x + i * NT;
-      *(f4*)&st[(e / (BK / 4)) * SK + (e % (BK / 4)) * 4] = ra[i];
+      put(st, A_PLANE, e / (BK / 4), e % (BK / 4), ra[i]);
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
       const int e = threadIdx.x + i * NT;
-      *(f4*)&st[A_ELEMS + (e / (BK / 4)) * SK + (e % (BK / 4)) * 4] = rb[i];
+      put(st + P * A_PLANE, B_PLANE, e / (BK / 4), e % (BK / 4), rb[i]);
     }
   }
 
-  // 8 consecutive k of one row from LDS, split into P BF16 terms.
-  static __device__ __forceinline__ void frag(const float* __restrict__ row, bf8 (&t)[P]) {
-    const f4 lo = *(const f4*)row;
-    const f4 hi = *(const f4*)(row + 4);
+  // 8 consecutive k of one row, all P terms.
+  static __device__ __forceinline__ void frag(const __bf16* __restrict__ planes, int plane, int row,
+                                              int lk, bf8 (&t)[P]) {
+    const int off = chunk_off(row, lk);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const float x = e < 4 ? lo[e] : hi[e - 4];
-      const __bf16 h = (__bf16)x;
-      float r = x - (float)h;
-      const __bf16 m = (__bf16)r;
-      t[0][e] = h;
-      t[1][e] = m;
-      if (P == 3) {
-        r -= (float)m;
-        t[P - 1][e] = (__bf16)r;
-      }
-    }
+    for (int p = 0; p < P; ++p) t[p] = *(const bf8*)(planes + p * plane + off);
   }
 
   static __device__ __forceinline__ void run(const GemmJob<float>& job, int tm, int tn,
-                                             float* smem) {
+                                             __bf16* smem) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
     const int m0 = tm * BM, n0 = tn * BN;
@@ -132,20 +152,19 @@ struct SplitGemm {
     }
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-      const float* st = smem + (kt & 1) * STAGE;
+      const __bf16* st = smem + (kt & 1) * STAGE;
       const bool more = kt + 1 < nk;
       if (more) load(job, m0, n0, (kt + 1) * BK, ra, rb);
       bf8 af[FM][P], bf[FN][P];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) frag(st + (wm0 + i * 16 + lr) * SK + 8 * lk, af[i]);
+      for (int i = 0; i < FM; ++i) frag(st, A_PLANE, wm0 + i * 16 + lr, lk, af[i]);
 #pragma unroll
-      for (int j = 0; j < FN; ++j) frag(st + A_ELEMS + (wn0 + j * 16 + lr) * SK + 8 * lk, bf[j]);
+      for (int j = 0; j < FN; ++j) frag(st + P * A_PLANE, B_PLANE, wn0 + j * 16 + lr, lk, bf[j]);
       // (A term, B term) products, smallest first
-      constexpr int NPROD = TERMS;
       constexpr int pa6[6] = {2, 0, 1, 1, 0, 0}, pb6[6] = {0, 2, 1, 0, 1, 0};
       constexpr int pa3[3] = {1, 0, 0}, pb3[3] = {0, 1, 0};
 #pragma unroll
-      for (int q = 0; q < NPROD; ++q) {
+      for (int q = 0; q < TERMS; ++q) {
         const int pa = TERMS == 6 ? pa6[q] : pa3[q];
         const int pb = TERMS == 6 ? pb6[q] : pb3[q];
 #pragma unroll
@@ -184,18 +203,23 @@ struct SplitGemm {
   }
 };
 
-template <int TERMS>
-__global__ __launch_bounds__(256) void gemm_split_kernel(const GemmJob<float>* __restrict__ jobs,
-                                                         int njobs, GemmJob<float> single,
-                                                         int total,
-                                                         const int* __restrict__ abort) {
-  typedef SplitGemm<TERMS> G;
-  __shared__ float smem[2 * G::STAGE];
+template <int TERMS, int BM>
+__global__ __launch_bounds__(1024) void gemm_split_kernel(const GemmJob<float>* __restrict__ jobs,
+                                                          int njobs, GemmJob<float> single,
+                                                          int total,
+                                                          const int* __restrict__ abort) {
+  typedef SplitGemm<TERMS, BM, 128> G;
+  __shared__ __attribute__((aligned(16))) __bf16 smem[2 * G::STAGE];
   if (abort && *abort) return;
+  constexpr bool packed = BM == 128;  // SYRK jobs enumerate only triangle tiles
   for (TileRange r = tile_range(total); r.t < r.end; r.t += r.step) {
     GemmJob<float> job;
     int tm, tn;
-    locate_tile(jobs, njobs, single, r.t, job, tm, tn);
+    locate_tile(jobs, njobs, single, r.t, job, tm, tn, packed);
+    if (!packed && job.tri) {  // rectangular tiles: skip those outside the triangle
+      const int m0 = tm * BM, n0 = tn * 128;
+      if (job.tri == 1 ? m0 > n0 + 128 - 1 : n0 > m0 + BM - 1) continue;
+    }
     G::run(job, tm, tn, smem);
     __syncthreads();  // the next tile's prologue rewrites LDS
   }
@@ -211,20 +235,30 @@ bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
   return true;
 }
 
-int gemm_split_launch(int terms, const GemmJob<float>* jobs_dev, int njobs,
+int gemm_split_launch(int terms, int bm, const GemmJob<float>* jobs_dev, int njobs,
                       const GemmJob<float>& single, long total_wg, long grid, const int* abort,
                       hipStream_t s) {
   if (total_wg <= 0) return 0;
   if (total_wg > 0x7fffffffL) return arg_fail(0, "gemm: too many workgroups");
   if (grid <= 0 || grid > total_wg) grid = total_wg;
-  if (terms == 6)
-    hipLaunchKernelGGL(gemm_split_kernel<6>, dim3((unsigned)grid), dim3(256), 0, s, jobs_dev,
-                       njobs, single, (int)total_wg, abort);
-  else if (terms == 3)
-    hipLaunchKernelGGL(gemm_split_kernel<3>, dim3((unsigned)grid), dim3(256), 0, s, jobs_dev,
-                       njobs, single, (int)total_wg, abort);
-  else
-    return arg_fail(1, "split GEMM terms must be 3 or 6");
+  if (bm != 128 && bm != 256) return arg_fail(2, "split GEMM tile must be 128 or 256 rows");
+  if (terms != 3 && terms != 6) return arg_fail(1, "split GEMM terms must be 3 or 6");
+  const dim3 g((unsigned)grid), b(1024);
+  if (terms == 6) {
+    if (bm == 256)
+      hipLaunchKernelGGL((gemm_split_kernel<6, 256>), g, b, 0, s, jobs_dev, njobs, single,
+                         (int)total_wg, abort);
+    else
+      hipLaunchKernelGGL((gemm_split_kernel<6, 128>), g, b, 0, s, jobs_dev, njobs, single,
+                         (int)total_wg, abort);
+  } else {
+    if (bm == 256)
+      hipLaunchKernelGGL((gemm_split_kernel<3, 256>), g, b, 0, s, jobs_dev, njobs, single,
+                         (int)total_wg, abort);
+    else
+      hipLaunchKernelGGL((gemm_split_kernel<3, 128>), g, b, 0, s, jobs_dev, njobs, single,
+                         (int)total_wg, abort);
+  }
   DM_LAUNCH_CHECK("gemm_split_kernel");
   return 0;
 }

@@ -555,8 +555,9 @@ extern "C" int dm_sgemm_bf16x(int terms, char transa, char transb, int m, int n,
   j.lda = lda; j.ldb = ldb; j.ldc = ldc;
   j.alpha = alpha; j.beta = beta; j.tri = 0;
   if (k == 0 || alpha == 0.0f) { j.alpha = 0.0f; j.K = 0; }
-  const long total = gemm_prepare_jobs<float>(&j, 1, gemm_split_cfg(terms));
-  return gemm_split_launch(terms, nullptr, 0, j, total, total, nullptr, stream_of(s));
+  const GemmCfgId cfg = gemm_split_cfg(terms, m, n, 1);
+  const long total = gemm_prepare_jobs<float>(&j, 1, cfg);
+  return gemm_split_launch(terms, cfg.bm, nullptr, 0, j, total, total, nullptr, stream_of(s));
 }
 
 extern "C" int dm_dtrsv(char uplo, char trans, char diag, int n, const double* A, int lda, double* x,

@@ -72,8 +72,8 @@ int tiled_skinny_gemm_launch(const dm_tiled&, const dm_tiled&, const dm_tiled&, 
   ++g_launch;
   return 0;
 }
-int gemm_split_launch(int terms, const GemmJob<float>*, int, const GemmJob<float>&, long, long,
-                      const int*, hipStream_t) {
+int gemm_split_launch(int terms, int, const GemmJob<float>*, int, const GemmJob<float>&, long,
+                      long, const int*, hipStream_t) {
   if (terms != 3 && terms != 6) abort();
   ++g_launch;
   return 0;

@@ -68,6 +68,27 @@ def test_split_update_factor_close_to_fp32(dm):
     assert errs[3] < 1e-4, errs
 
 
+def test_split_update_large_tiles_match_small_tiles(dm):
+    """Launches of at least 256 workgroup tiles use 256x128 tiles (SYRK jobs
+    as rectangles with the tiles outside the triangle skipped): the factor
+    is bitwise the one of 128x128 tiles (same per-element product order),
+    checked here against the same update arithmetic on FP32 MFMA."""
+    from dagger_mi355x import mixed
+    n, nb = 8192, 512  # step 0: 120 trailing tasks = 960 tiles of 256x128
+    A = gen.spd(n, 41)
+    f = {}
+    for update in (0, 6):
+        DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+        F = mixed.MixedCholesky(DA, options=mixed._OPTS.replace(f32_update=update))
+        f[update] = np.triu(F.F.collect().astype(np.float64))
+    assert np.linalg.norm(f[6] - f[0]) / np.linalg.norm(f[0]) < 1e-6
+    # the strictly lower triangle of the FP32 copy keeps A (nothing written there)
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    F = mixed.MixedCholesky(DA, options=mixed._OPTS.replace(f32_update=6))
+    low = np.tril(F.F.collect(), -1)
+    assert np.array_equal(low, np.tril(A.astype(np.float32), -1))
+
+
 def test_mixed_matches_plain_fp64_solution(dm):
     from dagger_mi355x import mixed
     n, nb = 1024, 256
