@@ -440,6 +440,15 @@ int Dag::build_device_tables() {
             rec.ta = st.ta;
             rec.tb = st.tb;
             rec.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, rec.njobs);
+            if constexpr (std::is_same<T, float>::value) {
+              // the solve's updates take the same BF16-split arithmetic as
+              // the factorization's when their operands allow it
+              if ((opts.f32_update == 3 || opts.f32_update == 6) && maxN > 48) {
+                bool split = true;
+                for (const GemmJob<T>& g : st.gjobs) split = split && gemm_split_ok(st.ta, st.tb, g);
+                if (split) rec.cfg = gemm_split_cfg(opts.f32_update, maxM, maxN, rec.njobs);
+              }
+            }
             rec.total_wg = gemm_prepare_jobs<T>(st.gjobs.data(), rec.njobs, rec.cfg);
             rec.vec = true;
             for (const GemmJob<T>& g : st.gjobs) rec.vec = rec.vec && gemm_job_vec_ok<T>(st.ta, st.tb, g);
@@ -473,6 +482,7 @@ int Dag::build_device_tables() {
   // device memory: [abort int (256 B)] [inverse blocks] [workspaces] [job tables]
   const size_t tab_base = 256 + inv_bytes + ws_bytes;
   const size_t total = tab_base + blob.size();
+  std::shared_lock<std::shared_mutex> cl(capture_mutex());
   DM_HIP(hipMalloc(&dev_mem, total));
   abort_dev = (int*)dev_mem;
   unsigned char* base = (unsigned char*)dev_mem;
@@ -497,8 +507,12 @@ int Dag::build_device_tables() {
     for (InvSrc& s : G.inv_src) s.dev = base + 256 + s.off;
     for (Group::Step& st : G.tsteps) st.table_dev = base + tab_base + st.table_off;
   }
-  if (!blob.empty()) DM_HIP(hipMemcpy(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice));
-  DM_HIP(hipMemset(dev_mem, 0, 256));
+  // the library's own non-blocking stream, not the legacy default stream
+  hipStream_t ds = stream_of(nullptr);
+  if (!blob.empty())
+    DM_HIP(hipMemcpyAsync(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice, ds));
+  DM_HIP(hipMemsetAsync(dev_mem, 0, 256, ds));
+  DM_HIP(hipStreamSynchronize(ds));
   return 0;
 }
 
@@ -620,6 +634,7 @@ int Dag::issue(hipStream_t o) {
 int Dag::capture() {
   // Capture, instantiate (and later launch) on the big-stack helper thread:
   // the runtime's graph walk recurses along dependency chains.
+  std::unique_lock<std::shared_mutex> cl(capture_mutex());
   return run_big_stack([this]() -> int {
     DM_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
     capturing = true;
@@ -677,7 +692,9 @@ Dag::~Dag() {
       if (st.ev0) (void)hipEventDestroy(st.ev0);
       if (st.ev1) (void)hipEventDestroy(st.ev1);
     }
-  if (!streams.empty() || origin) (void)hipDeviceSynchronize();
+  std::shared_lock<std::shared_mutex> cl(capture_mutex());
+  for (auto& s : streams) (void)hipStreamSynchronize(s);
+  if (origin) (void)hipStreamSynchronize(origin);
   for (auto& s : streams) (void)hipStreamDestroy(s);
   if (origin) (void)hipStreamDestroy(origin);
   if (dev_mem) (void)hipFree(dev_mem);

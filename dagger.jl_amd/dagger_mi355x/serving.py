@@ -11,8 +11,8 @@ Per request (worker stream s):
     padded size n_pad (a ladder of sizes; [A 0; 0 I] keeps x exact), then
     the FP32 tiled factor (MixedCholesky) on s;
   * solve with FP64 iterative refinement to the request's tolerance.
-Pooled buffers keep their device addresses, so the cached plan of each
-(buffer, size) is reused across requests.  Every computation depends
+Pooled buffers keep their device addresses, so the cached plan + hipGraph of
+each (buffer, size) is reused across requests.  Every computation depends
 only on its own request, so a result is the same alone and in the mix.
 
 Latency = completion - arrival (wall clock); solves/s = completed requests
@@ -110,11 +110,11 @@ class Server:
         self.nworkers = workers
         self.budget = float(budget_bytes)
         self.cache_enabled = cache
-        # Eager issue: a hipGraph capture on one worker's stream fails when
-        # another worker issues a synchronous runtime call meanwhile, so
-        # concurrent workers never capture.
+        # Plans are captured into hipGraphs on first use; the library holds
+        # other workers' allocating or synchronizing calls while a capture
+        # is in progress.
         self.options = options or PlanOptions(num_streams=2, batch=True, lookahead=1,
-                                              use_graph=False)
+                                              use_graph=True)
         self.streams = []
         for _ in range(workers):
             h = C.c_void_p()

@@ -415,6 +415,11 @@ hipError_t hipMemset(void* d, int v, size_t n) {
   std::memset(d, v, n);
   return hipSuccess;
 }
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+  if (emu::g_capture) return hipErrorInvalidValue;  // no memset nodes in captured graphs
+  std::memset(d, v, n);
+  return hipSuccess;
+}
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipGetLastError() { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "emulated HIP error"; }

@@ -87,6 +87,7 @@ hipError_t hipMemcpyAsync(void*, const void*, size_t, hipMemcpyKind, hipStream_t
 hipError_t hipMemcpy2DAsync(void*, size_t, const void*, size_t, size_t, size_t, hipMemcpyKind,
                             hipStream_t);
 hipError_t hipMemset(void*, int, size_t);
+hipError_t hipMemsetAsync(void*, int, size_t, hipStream_t);
 hipError_t hipDeviceSynchronize();
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t);

@@ -38,6 +38,9 @@ hipError_t hipMalloc(void**, size_t);
 hipError_t hipFree(void*);
 hipError_t hipMemcpy(void*, const void*, size_t, hipMemcpyKind);
 hipError_t hipMemset(void*, int, size_t);
+hipError_t hipMemcpyAsync(void*, const void*, size_t, hipMemcpyKind, hipStream_t);
+hipError_t hipMemsetAsync(void*, int, size_t, hipStream_t);
+hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipDeviceSynchronize();
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t);

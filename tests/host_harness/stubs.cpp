@@ -33,6 +33,9 @@ hipError_t hipMalloc(void** p, size_t n) { *p = malloc(n ? n : 1); return 0; }
 hipError_t hipFree(void* p) { free(p); return 0; }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { memcpy(d, s, n); return 0; }
 hipError_t hipMemset(void* d, int v, size_t n) { memset(d, v, n); return 0; }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { memcpy(d, s, n); return 0; }
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { memset(d, v, n); return 0; }
+hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
 hipError_t hipDeviceSynchronize() { return 0; }
 hipError_t hipGetLastError() { return 0; }
 const char* hipGetErrorString(hipError_t) { return "stub"; }
@@ -43,6 +46,10 @@ int hip_fail(hipError_t e, const char* w) { fprintf(stderr, "hip_fail %s\n", w);
 int arg_fail(int a, const char* w) { fprintf(stderr, "arg_fail %d %s\n", a, w); return a > 0 ? -a : -99; }
 int ensure_init() { return 0; }
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : (hipStream_t)0x42; }
+std::shared_mutex& capture_mutex() {
+  static std::shared_mutex m;
+  return m;
+}
 void* stream_workspace(hipStream_t, size_t) { return nullptr; }
 int trsm_init_attributes() { return 0; }
 int run_big_stack(std::function<int()> fn) { return fn(); }
