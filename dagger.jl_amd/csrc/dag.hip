@@ -482,7 +482,6 @@ int Dag::build_device_tables() {
   // device memory: [abort int (256 B)] [inverse blocks] [workspaces] [job tables]
   const size_t tab_base = 256 + inv_bytes + ws_bytes;
   const size_t total = tab_base + blob.size();
-  std::shared_lock<std::shared_mutex> cl(capture_mutex());
   DM_HIP(hipMalloc(&dev_mem, total));
   abort_dev = (int*)dev_mem;
   unsigned char* base = (unsigned char*)dev_mem;
@@ -507,12 +506,8 @@ int Dag::build_device_tables() {
     for (InvSrc& s : G.inv_src) s.dev = base + 256 + s.off;
     for (Group::Step& st : G.tsteps) st.table_dev = base + tab_base + st.table_off;
   }
-  // the library's own non-blocking stream, not the legacy default stream
-  hipStream_t ds = stream_of(nullptr);
-  if (!blob.empty())
-    DM_HIP(hipMemcpyAsync(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice, ds));
-  DM_HIP(hipMemsetAsync(dev_mem, 0, 256, ds));
-  DM_HIP(hipStreamSynchronize(ds));
+  if (!blob.empty()) DM_HIP(hipMemcpy(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  DM_HIP(hipMemset(dev_mem, 0, 256));
   return 0;
 }
 
@@ -634,7 +629,6 @@ int Dag::issue(hipStream_t o) {
 int Dag::capture() {
   // Capture, instantiate (and later launch) on the big-stack helper thread:
   // the runtime's graph walk recurses along dependency chains.
-  std::unique_lock<std::shared_mutex> cl(capture_mutex());
   return run_big_stack([this]() -> int {
     DM_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
     capturing = true;
@@ -692,9 +686,7 @@ Dag::~Dag() {
       if (st.ev0) (void)hipEventDestroy(st.ev0);
       if (st.ev1) (void)hipEventDestroy(st.ev1);
     }
-  std::shared_lock<std::shared_mutex> cl(capture_mutex());
-  for (auto& s : streams) (void)hipStreamSynchronize(s);
-  if (origin) (void)hipStreamSynchronize(origin);
+  if (!streams.empty() || origin) (void)hipDeviceSynchronize();
   for (auto& s : streams) (void)hipStreamDestroy(s);
   if (origin) (void)hipStreamDestroy(origin);
   if (dev_mem) (void)hipFree(dev_mem);

@@ -6,8 +6,6 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
-#include <mutex>
-#include <shared_mutex>
 #include <string>
 #include <vector>
 #include "../../include/dagger_mi355x.h"
@@ -57,11 +55,6 @@ int run_big_stack(std::function<int()> fn);
 // Per-stream scratch buffer (grown on demand, never shrunk while the
 // library is loaded).  Calls ordered on one stream may reuse it.
 void* stream_workspace(hipStream_t s, size_t bytes);
-// Graph captures hold this exclusively; calls that allocate, free or
-// synchronize the device hold it shared, so they never run while another
-// thread captures (the runtime invalidates a capture in progress when such
-// a call makes the default stream depend on capturing work).
-std::shared_mutex& capture_mutex();
 
 inline bool is_trans(char t) { return t == 'T' || t == 't' || t == 'C' || t == 'c'; }
 inline bool is_notrans(char t) { return t == 'N' || t == 'n'; }

@@ -83,11 +83,6 @@ int ensure_init() {
 
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : g_default_stream; }
 
-std::shared_mutex& capture_mutex() {
-  static std::shared_mutex m;
-  return m;
-}
-
 // ----------------------------------------------------------- big-stack runner
 // The HIP runtime walks a captured graph recursively (recursion depth grows
 // with the longest dependency chain of the graph), which overflows a default
@@ -172,7 +167,6 @@ std::unordered_map<hipStream_t, WsEntry> g_ws;
 
 void* stream_workspace(hipStream_t s, size_t bytes) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  std::shared_lock<std::shared_mutex> cl(capture_mutex());
   WsEntry& e = g_ws[s];
   if (e.bytes >= bytes) return e.p;
   // Growing: the old buffer may still be in use by queued work on s, so
@@ -234,7 +228,6 @@ int dm_malloc(void** ptr, size_t bytes) {
   DM_INIT();
   *ptr = nullptr;
   if (bytes == 0) return 0;
-  std::shared_lock<std::shared_mutex> lk(capture_mutex());
   DM_HIP(hipMalloc(ptr, bytes));
   return 0;
 }
@@ -242,7 +235,6 @@ int dm_malloc(void** ptr, size_t bytes) {
 int dm_free(void* ptr) {
   if (!ptr) return 0;
   DM_INIT();
-  std::shared_lock<std::shared_mutex> lk(capture_mutex());
   DM_HIP(hipFree(ptr));
   return 0;
 }
@@ -332,7 +324,6 @@ int dm_stream_sync(dm_stream_t s) {
 
 int dm_device_sync(void) {
   DM_INIT();
-  std::shared_lock<std::shared_mutex> lk(capture_mutex());
   DM_HIP(hipDeviceSynchronize());
   return 0;
 }

@@ -11,8 +11,8 @@ Per request (worker stream s):
     padded size n_pad (a ladder of sizes; [A 0; 0 I] keeps x exact), then
     the FP32 tiled factor (MixedCholesky) on s;
   * solve with FP64 iterative refinement to the request's tolerance.
-Pooled buffers keep their device addresses, so the cached plan + hipGraph of
-each (buffer, size) is reused across requests.  Every computation depends
+Pooled buffers keep their device addresses, so the cached plan of each
+(buffer, size) is reused across requests.  Every computation depends
 only on its own request, so a result is the same alone and in the mix.
 
 Latency = completion - arrival (wall clock); solves/s = completed requests
@@ -110,11 +110,11 @@ class Server:
         self.nworkers = workers
         self.budget = float(budget_bytes)
         self.cache_enabled = cache
-        # Plans are captured into hipGraphs on first use; the library holds
-        # other workers' allocating or synchronizing calls while a capture
-        # is in progress.
+        # Eager issue: a hipGraph capture on one worker's stream is
+        # invalidated when another worker makes a synchronous runtime call
+        # meanwhile, so concurrent workers never capture.
         self.options = options or PlanOptions(num_streams=2, batch=True, lookahead=1,
-                                              use_graph=True)
+                                              use_graph=False)
         self.streams = []
         for _ in range(workers):
             h = C.c_void_p()
@@ -165,9 +165,10 @@ class Server:
     def _admit(self, req):
         """Reserve the request's factor (cached or new).  Returns the entry,
         or None when memory does not allow it yet.  Caller holds the lock."""
-        e = self.cache.get(req.mid) if self.cache_enabled else None
+        key = (req.mid, req.n, req.kappa)  # a matrix is identified by all three
+        e = self.cache.get(key) if self.cache_enabled else None
         if e is not None:
-            self.cache.move_to_end(req.mid)
+            self.cache.move_to_end(key)
             e.users += 1
             return e, False
         npad = pad_size(req.n)
@@ -180,7 +181,7 @@ class Server:
         if not pooled:
             self.used += need
         if self.cache_enabled:
-            self.cache[req.mid] = e
+            self.cache[key] = e
         return e, True
 
     def _run(self, req, e, fresh, stream):
@@ -236,7 +237,12 @@ class Server:
                     self.lock.notify_all()
 
     def run(self, trace, time_scale=1.0):
-        """Replay `trace` (arrival times relative to start) and serve it."""
+        """Replay `trace` (arrival times relative to start) and serve it.
+        A server may run several traces one after another; pooled buffers,
+        cached factors and plans carry over."""
+        with self.lock:
+            self.stop = False
+            self.done = []
         threads = [threading.Thread(target=self._worker, args=(w,), daemon=True)
                    for w in range(self.nworkers)]
         for t in threads:

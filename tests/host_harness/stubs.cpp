@@ -46,10 +46,6 @@ int hip_fail(hipError_t e, const char* w) { fprintf(stderr, "hip_fail %s\n", w);
 int arg_fail(int a, const char* w) { fprintf(stderr, "arg_fail %d %s\n", a, w); return a > 0 ? -a : -99; }
 int ensure_init() { return 0; }
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : (hipStream_t)0x42; }
-std::shared_mutex& capture_mutex() {
-  static std::shared_mutex m;
-  return m;
-}
 void* stream_workspace(hipStream_t, size_t) { return nullptr; }
 int trsm_init_attributes() { return 0; }
 int run_big_stack(std::function<int()> fn) { return fn(); }
