@@ -225,7 +225,7 @@ def batched_leg(dm, count, seed, reps=3):
     return out
 
 
-def serving_leg(dm, rate, count, seed, workers=4):
+def serving_leg(dm, rate, count, seed, workers=6):
     """Serving form vs one-at-a-time plain FP64 on the same Poisson trace."""
     from dagger_mi355x import serving
     tr1 = serving.make_trace(rate=rate, count=count, seed=seed)

@@ -71,6 +71,59 @@ class DeviceBuffer:
         self.ptr = 0
 
 
+class Scratch:
+    """Named device buffers reused across calls on one stream.  A caller
+    that stages many transfers or fills through one Scratch allocates and
+    frees nothing once warm; per-call temporaries would not do for
+    concurrent streams, because hipFree waits for the whole device.  A
+    buffer grows to the next power of two (at least 1 MiB) when a larger one
+    is asked for; the one it replaces is kept until free(), so growing never
+    waits for the device either."""
+
+    def __init__(self):
+        self._bufs = {}
+        self._retired = []
+
+    def get(self, name, nbytes):
+        b = self._bufs.get(name)
+        if b is None or b.nbytes < nbytes:
+            if b is not None:
+                self._retired.append(b)  # queued work on the stream may still use it
+            b = DeviceBuffer(1 << max(20, (max(int(nbytes), 1) - 1).bit_length()))
+            self._bufs[name] = b
+        return b
+
+    def nbytes(self):
+        return sum(b.nbytes for b in self._bufs.values()) + sum(b.nbytes for b in self._retired)
+
+    def free(self):
+        """Release every buffer (the caller has synchronized the stream)."""
+        for b in list(self._bufs.values()) + self._retired:
+            b.free()
+        self._bufs.clear()
+        self._retired = []
+
+
+class _Temp:
+    """A temporary device buffer: from `scratch` when given (kept), else
+    allocated here and freed on exit."""
+
+    def __init__(self, scratch, name, nbytes):
+        self.scratch, self.name, self.nbytes = scratch, name, nbytes
+
+    def __enter__(self):
+        if self.scratch is not None:
+            self.buf = self.scratch.get(self.name, self.nbytes)
+        else:
+            self.buf = DeviceBuffer(self.nbytes)
+        return self.buf
+
+    def __exit__(self, *exc):
+        if self.scratch is None:
+            self.buf.free()
+        return False
+
+
 class DeviceInt:
     """A device int32 array (LAPACK info / flags)."""
 
@@ -85,6 +138,9 @@ class DeviceInt:
 
     def fill(self, value, stream=None):
         call("dm_fill_i32", C.c_void_p(self.ptr), int(value), C.c_size_t(self.count), stream)
+
+    def free(self):
+        self.buf.free()
 
     def get(self, stream=None):
         out = np.zeros(self.count, dtype=np.int32)
@@ -192,33 +248,33 @@ class DArray:
         D.copy_from_host(A, stream)
         return D
 
-    def copy_from_host(self, A, stream=None):
-        """`copyto!(DA, A)` (src/array/copy.jl:146-151): one H2D + pack kernel."""
+    def copy_from_host(self, A, stream=None, scratch=None):
+        """`copyto!(DA, A)` (src/array/copy.jl:146-151): one H2D + pack kernel.
+        `scratch` (a Scratch) provides the staging buffer."""
         A = np.asfortranarray(np.asarray(A, dtype=self.dtype).reshape(self.m, self.n, order="F")
                               if self.ndim == 1 else np.asarray(A, dtype=self.dtype))
         if A.shape != (self.m, self.n):
             raise ValueError("shape mismatch")
         if self.m * self.n == 0:
             return self
-        tmp = DeviceBuffer(A.nbytes)
-        call("dm_memcpy_h2d", C.c_void_p(tmp.ptr), A.ctypes.data_as(C.c_void_p),
-             C.c_size_t(A.nbytes), stream)
-        t = self.tiled()
-        call("dm_tiled_pack", C.byref(t), C.c_void_p(tmp.ptr), self.m, 0.0, stream)
-        call("dm_stream_sync", stream)
-        tmp.free()
+        with _Temp(scratch, "stage", A.nbytes) as tmp:
+            call("dm_memcpy_h2d", C.c_void_p(tmp.ptr), A.ctypes.data_as(C.c_void_p),
+                 C.c_size_t(A.nbytes), stream)
+            t = self.tiled()
+            call("dm_tiled_pack", C.byref(t), C.c_void_p(tmp.ptr), self.m, 0.0, stream)
+            call("dm_stream_sync", stream)
         return self
 
-    def collect(self, stream=None):
-        """`collect(DA)` (src/array/darray.jl:236-271): unpack kernel + D2H."""
+    def collect(self, stream=None, scratch=None):
+        """`collect(DA)` (src/array/darray.jl:236-271): unpack kernel + D2H.
+        `scratch` (a Scratch) provides the staging buffer."""
         out = np.empty((self.m, self.n), dtype=self.dtype, order="F")
         if self.m * self.n:
-            tmp = DeviceBuffer(out.nbytes)
-            t = self.tiled()
-            call("dm_tiled_unpack", C.c_void_p(tmp.ptr), self.m, C.byref(t), stream)
-            call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(tmp.ptr),
-                 C.c_size_t(out.nbytes), stream)
-            tmp.free()
+            with _Temp(scratch, "stage", out.nbytes) as tmp:
+                t = self.tiled()
+                call("dm_tiled_unpack", C.c_void_p(tmp.ptr), self.m, C.byref(t), stream)
+                call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(tmp.ptr),
+                     C.c_size_t(out.nbytes), stream)
         return out.reshape(self.m) if self.ndim == 1 else out
 
     def copy(self, stream=None):

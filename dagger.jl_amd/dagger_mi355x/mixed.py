@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call
-from .darray import DArray, Blocks, DeviceBuffer, DeviceInt, Tile
+from .darray import DArray, Blocks, DeviceBuffer, DeviceInt, Tile, _Temp
 from .datadeps import PlanOptions, Dag, make_task, BLAS, In, Out, InOut
 from .linalg import PosDefException
 
@@ -107,6 +107,30 @@ def _colnorms(X, buf, stream=None):
     call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
          C.c_size_t(8 * X.n), stream)
     return out
+
+
+def release_factor_buffers(F):
+    """Free the refinement buffers a MixedCholesky attached to its FP32
+    factor array F (info flags, norm scratch, per-right-hand-side buffers,
+    inverted diagonal tiles).  They reference F, so without this they are
+    reclaimed only by Python's cyclic collector, whose hipFree calls then
+    wait for the whole device at an arbitrary later time."""
+    aux = getattr(F, "_mixed_aux", None)
+    if aux is None:
+        return
+    for key in ("info", "ginfo", "work", "norm"):
+        aux[key].free()
+    for b in aux["bufs"].values():
+        for name in ("X", "R", "D", "Y"):
+            b[name].unsafe_free()
+        b["nrm"].free()
+    iso = aux.get("isolve")
+    if iso is not None:
+        iso.Dinv.unsafe_free()
+        iso._solves.clear()
+        iso._inv = None
+    aux.clear()
+    F._mixed_aux = None
 
 
 class MixedCholesky:
@@ -336,15 +360,19 @@ class HouseholderSpec:
       A = S - Q V^T - V Q^T = S - U W^T,  U = [Q V], W = [V Q],
       Y = S V T^T, Z = T V^T Y, Q = Y - V Z / 2."""
 
-    def __init__(self, n, kappa, seed, nrefl=64, stream=None):
+    def __init__(self, n, kappa, seed, nrefl=64, stream=None, scratch=None):
         self.n, self.kappa, self.r = n, float(kappa), nrefl
-        V = _philox_randn(n * nrefl, seed, stream).reshape(nrefl, n).T.copy()
-        tau = 2.0 / np.einsum("ij,ij->j", V, V)
-        T = np.zeros((nrefl, nrefl))
-        for i in range(nrefl):
-            T[i, i] = tau[i]
-            if i:
-                T[:i, i] = -tau[i] * (T[:i, :i] @ (V[:, :i].T @ V[:, i]))
+        V = _philox_randn(n * nrefl, seed, stream, scratch).reshape(nrefl, n).T.copy()
+        # T from the UT transform (Joffrain, Low, Quintana-Orti, van de Geijn,
+        # Van Zee, "Accumulating Householder transformations, revisited",
+        # ACM TOMS 32(2), 2006): with tau_i = 2 / v_i^T v_i,
+        # T^-1 = striu(V^T V) + diag(V^T V) / 2 -- the same T as dlarft's
+        # column recurrence, from one product and one inverse (LU of an upper
+        # triangular matrix needs no row exchange, so T comes out upper
+        # triangular).
+        G = V.T @ V
+        Tinv = np.triu(G, 1) + np.diag(0.5 * np.diag(G))
+        T = np.triu(np.linalg.inv(Tinv))
         self.V, self.T = V, T
         self.s = self.kappa ** (-np.arange(n) / max(n - 1, 1))
         Y = self.s[:, None] * (V @ T.T)
@@ -358,10 +386,11 @@ class HouseholderSpec:
         return (self.s[:, None] * x2 - self.U @ (self.W.T @ x2)).reshape(x.shape)
 
 
-def householder_fill(D, spec, stream=None):
+def householder_fill(D, spec, stream=None, scratch=None):
     """Write A = H diag(s) H^T into the resident DArray D (n_pad >= n):
     D = [A 0; 0 I] (zero rows appended to U, W and ones to s), via one FP64
-    GEMM n_pad x n_pad x 2r, the diagonal add and the tile pack."""
+    GEMM n_pad x n_pad x 2r, the diagonal add and the tile pack.  `scratch`
+    (a Scratch) provides the device temporaries."""
     n, npad = spec.n, D.m
     r2 = spec.U.shape[1]
     Up = np.zeros((npad, r2), order="F")
@@ -370,24 +399,20 @@ def householder_fill(D, spec, stream=None):
     Wp[:n] = spec.W
     sp = np.ones(npad)
     sp[:n] = spec.s
-    U = DeviceBuffer(npad * r2 * 8)
-    W = DeviceBuffer(npad * r2 * 8)
-    sd = DeviceBuffer(npad * 8)
-    Fb = DeviceBuffer(npad * npad * 8)
-    call("dm_memcpy_h2d", C.c_void_p(U.ptr), Up.ctypes.data_as(C.c_void_p),
-         C.c_size_t(Up.nbytes), stream)
-    call("dm_memcpy_h2d", C.c_void_p(W.ptr), Wp.ctypes.data_as(C.c_void_p),
-         C.c_size_t(Wp.nbytes), stream)
-    call("dm_memcpy_h2d", C.c_void_p(sd.ptr), sp.ctypes.data_as(C.c_void_p),
-         C.c_size_t(8 * npad), stream)
-    call("dm_dgemm", b"N", b"T", npad, npad, r2, -1.0, C.c_void_p(U.ptr), npad, C.c_void_p(W.ptr),
-         npad, 0.0, C.c_void_p(Fb.ptr), npad, stream)
-    call("dm_add_diag_d", C.c_void_p(Fb.ptr), npad, npad, C.c_void_p(sd.ptr), stream)
-    t = D.tiled()
-    call("dm_tiled_pack", C.byref(t), C.c_void_p(Fb.ptr), npad, 0.0, stream)
-    call("dm_stream_sync", stream)
-    for b in (U, W, sd, Fb):
-        b.free()
+    with _Temp(scratch, "hh_u", npad * r2 * 8) as U, _Temp(scratch, "hh_w", npad * r2 * 8) as W, \
+            _Temp(scratch, "hh_s", npad * 8) as sd, _Temp(scratch, "hh_full", npad * npad * 8) as Fb:
+        call("dm_memcpy_h2d", C.c_void_p(U.ptr), Up.ctypes.data_as(C.c_void_p),
+             C.c_size_t(Up.nbytes), stream)
+        call("dm_memcpy_h2d", C.c_void_p(W.ptr), Wp.ctypes.data_as(C.c_void_p),
+             C.c_size_t(Wp.nbytes), stream)
+        call("dm_memcpy_h2d", C.c_void_p(sd.ptr), sp.ctypes.data_as(C.c_void_p),
+             C.c_size_t(8 * npad), stream)
+        call("dm_dgemm", b"N", b"T", npad, npad, r2, -1.0, C.c_void_p(U.ptr), npad,
+             C.c_void_p(W.ptr), npad, 0.0, C.c_void_p(Fb.ptr), npad, stream)
+        call("dm_add_diag_d", C.c_void_p(Fb.ptr), npad, npad, C.c_void_p(sd.ptr), stream)
+        t = D.tiled()
+        call("dm_tiled_pack", C.byref(t), C.c_void_p(Fb.ptr), npad, 0.0, stream)
+        call("dm_stream_sync", stream)
     return D
 
 
@@ -399,13 +424,12 @@ def householder_generate(n, blocks, kappa, seed, nrefl=64, stream=None):
     return D, spec
 
 
-def _philox_randn(count, seed, stream=None):
+def _philox_randn(count, seed, stream=None, scratch=None):
     """The device Philox/Box-Muller stream (count values, offset 0)."""
-    buf = DeviceBuffer(8 * count)
-    call("dm_fill_randn_d", C.c_void_p(buf.ptr), C.c_size_t(count), C.c_uint64(seed),
-         C.c_uint64(0), stream)
     out = np.empty(count)
-    call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
-         C.c_size_t(8 * count), stream)
-    buf.free()
+    with _Temp(scratch, "randn", 8 * count) as buf:
+        call("dm_fill_randn_d", C.c_void_p(buf.ptr), C.c_size_t(count), C.c_uint64(seed),
+             C.c_uint64(0), stream)
+        call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr),
+             C.c_size_t(8 * count), stream)
     return out

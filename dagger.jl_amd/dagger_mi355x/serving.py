@@ -8,17 +8,30 @@ factors on the GPU so a request reusing an earlier matrix only solves.
 
 Per request (worker stream s):
   * new matrix: A = H diag(s) H^T written into a pooled resident DArray of
-    padded size n_pad (a ladder of sizes; [A 0; 0 I] keeps x exact), then
-    the FP32 tiled factor (MixedCholesky) on s;
-  * solve with FP64 iterative refinement to the request's tolerance.
+    padded size n_pad (a ladder of sizes; [A 0; 0 I] keeps x exact), then,
+    for n_pad >= mixed_min_n, the FP32 tiled factor (MixedCholesky) on s,
+    else the FP64 tiled factor in place;
+  * solve: FP64 iterative refinement to the request's tolerance on the
+    FP32 factor, or one FP64 potrs on the FP64 factor (its backward error
+    checked against the request's tolerance like the refined ones).
 Pooled buffers keep their device addresses, so the cached plan of each
-(buffer, size) is reused across requests.  Every computation depends
+(buffer, size) is reused across requests.  Each worker also keeps its own
+staging buffers and right-hand-side arrays (by padded size and column
+count), so a warm worker allocates and frees nothing per request: hipFree
+waits for the whole device, which would serialize the workers.  These
+per-worker buffers (at most the largest padded n_pad^2 FP64 plus small
+vectors per worker) sit outside the HBM budget.  Every computation depends
 only on its own request, so a result is the same alone and in the mix.
 
 Latency = completion - arrival (wall clock); solves/s = completed requests
-/ (last completion - first arrival).
+/ (last completion - first arrival).  Both the server and the one-at-a-time
+baseline replay a trace with Python's cyclic garbage collector paused
+(after a full collection): a finalizer of an unrelated object running
+inside the loop would free device memory, and hipFree waits for the whole
+device.
 """
 import ctypes as C
+import gc
 import threading
 import time
 from collections import OrderedDict, deque
@@ -26,10 +39,10 @@ from collections import OrderedDict, deque
 import numpy as np
 
 from ._lib import call
-from .darray import DArray, Blocks
+from .darray import DArray, Blocks, DeviceInt, Scratch
 from .datadeps import PlanOptions
-from .mixed import MixedCholesky, HouseholderSpec, householder_fill
-from .linalg import cholesky_, ldiv_
+from .mixed import MixedCholesky, HouseholderSpec, householder_fill, release_factor_buffers
+from .linalg import cholesky_, ldiv_, PosDefException
 
 LADDER = (64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192)
 
@@ -90,54 +103,79 @@ def make_trace(rate, count, seed, n_lo=64, n_hi=8192, k_lo=1e2, k_hi=1e6, reuse=
 
 
 class _Entry:
-    """A cached factor: the FP64 A, its FP32 factor, readiness."""
+    """A cached factor: the FP64 A with its FP32 factor F (mixed), or the
+    FP64 factor alone (direct), and readiness."""
 
-    def __init__(self, npad):
-        self.npad = npad
+    def __init__(self, npad, mixed):
+        self.npad, self.mixed = npad, mixed
         self.A = None
         self.F = None
-        self.solver = None
+        self.solver = None   # mixed: the MixedCholesky
+        self.spec = None     # direct: the generator's host algebra (A x)
+        self.normA = None    # direct: ||A||_inf
         self.ready = threading.Event()
         self.users = 0
         self.error = None
 
-    def nbytes(self):
-        return self.npad * self.npad * 12
+
+def _entry_bytes(npad, mixed):
+    return npad * npad * (12 if mixed else 8)
 
 
 class Server:
-    def __init__(self, workers=4, budget_bytes=200e9, cache=True, options=None):
+    """Continuous-batching server.  Requests whose padded size is at least
+    `mixed_min_n` get an FP32 factor refined in FP64 (iterative refinement
+    to the request's tolerance); smaller ones an FP64 factor and one direct
+    solve, where the factor costs too little for the FP32 saving to pay for
+    the refinement's extra launches and synchronizations."""
+
+    def __init__(self, workers=4, budget_bytes=200e9, cache=True, options=None,
+                 mixed_min_n=4096):
         self.nworkers = workers
         self.budget = float(budget_bytes)
         self.cache_enabled = cache
+        self.mixed_min_n = mixed_min_n
         # Eager issue: a hipGraph capture on one worker's stream is
         # invalidated when another worker makes a synchronous runtime call
         # meanwhile, so concurrent workers never capture.
         self.options = options or PlanOptions(num_streams=2, batch=True, lookahead=1,
                                               use_graph=False)
+        self.scratch = [Scratch() for _ in range(workers)]
+        self.rhs = [dict() for _ in range(workers)]  # (npad, nrhs) -> DArray
         self.streams = []
         for _ in range(workers):
             h = C.c_void_p()
             call("dm_stream_create", C.byref(h))
             self.streams.append(h)
         self.lock = threading.Condition()
-        self.cache = OrderedDict()       # mid -> _Entry (LRU order)
-        self.pool = {}                   # npad -> [(A, F)] free buffers
+        self.cache = OrderedDict()       # (mid, n, kappa) -> _Entry (LRU order)
+        self.pool = {}                   # (npad, mixed) -> [(A, F)] free buffers
         self.used = 0.0                  # bytes of entries + pooled buffers
         self.queue = deque()
         self.done = []
         self.stop = False
 
     # ---------------------------------------------------------- memory
+    @staticmethod
+    def _free_pair(A, F):
+        info = getattr(A, "_srv_info", None)
+        if info is not None:
+            for d in info:
+                d.free()
+            A._srv_info = None
+        if F is not None:
+            release_factor_buffers(F)
+            F.unsafe_free()
+        A.unsafe_free()
+
     def _evict_for(self, need):
         """Free pooled buffers, then unused LRU cache entries, until `need`
         bytes fit in the budget.  Caller holds the lock."""
-        for npad in list(self.pool):
-            while self.pool[npad] and self.used + need > self.budget:
-                A, F = self.pool[npad].pop()
-                A.unsafe_free()
-                F.unsafe_free()
-                self.used -= npad * npad * 12
+        for key in list(self.pool):
+            while self.pool[key] and self.used + need > self.budget:
+                A, F = self.pool[key].pop()
+                self._free_pair(A, F)
+                self.used -= _entry_bytes(*key)
         for mid in list(self.cache):
             if self.used + need <= self.budget:
                 break
@@ -149,16 +187,16 @@ class Server:
 
     def _release(self, e):
         if e.A is not None:
-            self.pool.setdefault(e.npad, []).append((e.A, e.F))
-            e.A = e.F = e.solver = None
+            self.pool.setdefault((e.npad, e.mixed), []).append((e.A, e.F))
+            e.A = e.F = e.solver = e.spec = None
 
-    def _take_buffers(self, npad):
-        lst = self.pool.get(npad)
+    def _take_buffers(self, npad, mixed):
+        lst = self.pool.get((npad, mixed))
         if lst:
             return lst.pop()
         nb = tile_for(npad)
         A = DArray(np.float64, (npad, npad), Blocks(nb, nb))
-        F = DArray(np.float32, (npad, npad), Blocks(nb, nb))
+        F = DArray(np.float32, (npad, npad), Blocks(nb, nb)) if mixed else None
         return A, F
 
     # ---------------------------------------------------------- work
@@ -172,11 +210,12 @@ class Server:
             e.users += 1
             return e, False
         npad = pad_size(req.n)
-        need = npad * npad * 12
-        pooled = bool(self.pool.get(npad))
+        mixed = npad >= self.mixed_min_n
+        need = _entry_bytes(npad, mixed)
+        pooled = bool(self.pool.get((npad, mixed)))
         if not pooled and not self._evict_for(need):
             return None, False
-        e = _Entry(npad)
+        e = _Entry(npad, mixed)
         e.users = 1
         if not pooled:
             self.used += need
@@ -184,15 +223,49 @@ class Server:
             self.cache[key] = e
         return e, True
 
-    def _run(self, req, e, fresh, stream):
+    def _rhs_buffer(self, w, npad, nrhs):
+        key = (npad, nrhs)
+        DB = self.rhs[w].get(key)
+        if DB is None:
+            nb = tile_for(npad)
+            DB = DArray(np.float64, (npad, nrhs), Blocks(nb, nb))
+            self.rhs[w][key] = DB
+        return DB
+
+    def _factor_direct(self, e, A, stream, scratch):
+        """||A||_inf, then the FP64 tiled factor in place (cholesky!)."""
+        t = A.tiled()
+        work = scratch.get("norm_work", 8 * max(A.m, 2) * (1 + A.nt))
+        nrm = scratch.get("norm", 16)
+        call("dm_tiled_norm_inf", C.byref(t), C.c_void_p(nrm.ptr), C.c_void_p(work.ptr), stream)
+        out = np.zeros(1)
+        call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(nrm.ptr),
+             C.c_size_t(8), stream)
+        e.normA = float(out[0])
+        if getattr(A, "_srv_info", None) is None:
+            A._srv_info = (DeviceInt(1), DeviceInt(1))
+        info, ginfo = A._srv_info
+        o = self.options.replace(f32_update=0).c()
+        call("dm_cholesky_tiled", b"U", C.byref(t), C.c_void_p(info.ptr), C.c_void_p(ginfo.ptr),
+             C.byref(o), stream)
+        loc = int(info.get(stream)[0])
+        if loc != 0:
+            raise PosDefException(loc)
+
+    def _run(self, req, e, fresh, w):
+        stream, scratch = self.streams[w], self.scratch[w]
         if fresh:
             try:
                 with self.lock:
-                    A, F = self._take_buffers(e.npad)
-                spec = HouseholderSpec(req.n, req.kappa, 1_000_003 + req.mid, 64, stream)
-                householder_fill(A, spec, stream)
+                    A, F = self._take_buffers(e.npad, e.mixed)
                 e.A, e.F = A, F
-                e.solver = MixedCholesky(A, options=self.options, stream=stream, F=F)
+                spec = HouseholderSpec(req.n, req.kappa, 1_000_003 + req.mid, 64, stream, scratch)
+                householder_fill(A, spec, stream, scratch)
+                if e.mixed:
+                    e.solver = MixedCholesky(A, options=self.options, stream=stream, F=F)
+                else:
+                    e.spec = spec
+                    self._factor_direct(e, A, stream, scratch)
             except Exception as ex:  # surfaced per request
                 e.error = ex
             e.ready.set()
@@ -202,21 +275,32 @@ class Server:
             raise e.error
         npad = e.npad
         B = np.zeros((npad, req.nrhs))
-        B[: req.n] = req.rhs()
-        nb = tile_for(npad)
-        DB = DArray(np.float64, (npad, req.nrhs), Blocks(nb, nb))
-        DB.copy_from_host(B, stream)
-        solver = MixedCholesky.__new__(MixedCholesky)
-        solver.__dict__.update(e.solver.__dict__)
-        solver.stream = stream
-        X, info = solver.solve(DB, tol=req.tol, max_iter=30)
-        req.x = X.collect(stream)[: req.n]
-        req.berr = float(np.max(info["berr"]))
-        req.iters = info["iterations"]
-        DB.unsafe_free()
+        b = req.rhs()
+        B[: req.n] = b
+        DB = self._rhs_buffer(w, npad, req.nrhs)
+        DB.copy_from_host(B, stream, scratch)
+        if e.mixed:
+            solver = MixedCholesky.__new__(MixedCholesky)
+            solver.__dict__.update(e.solver.__dict__)
+            solver.stream = stream
+            X, info = solver.solve(DB, tol=req.tol, max_iter=30)
+            req.x = X.collect(stream, scratch)[: req.n]
+            req.berr = float(np.max(info["berr"]))
+            req.iters = info["iterations"]
+        else:
+            ta, tb = e.A.tiled(), DB.tiled()
+            o = self.options.replace(f32_update=0).c()
+            call("dm_potrs_tiled", b"U", C.byref(ta), C.byref(tb), C.byref(o), stream)
+            x = DB.collect(stream, scratch)[: req.n]
+            # backward error of the delivered x against A's host algebra
+            # (A x = s x - U (W^T x), O(n r nrhs)), the refinement's measure
+            r = b - e.spec.matvec(x)
+            den = e.normA * np.abs(x).max(0) + np.abs(b).max(0)
+            req.x = x
+            req.berr = float((np.abs(r).max(0) / den).max())
+            req.iters = 0
 
     def _worker(self, w):
-        stream = self.streams[w]
         while True:
             with self.lock:
                 while not self.queue and not self.stop:
@@ -226,7 +310,7 @@ class Server:
                 req, e, fresh = self.queue.popleft()
             req.t_start = time.perf_counter()
             try:
-                self._run(req, e, fresh, stream)
+                self._run(req, e, fresh, w)
             finally:
                 req.t_done = time.perf_counter()
                 with self.lock:
@@ -243,6 +327,10 @@ class Server:
         with self.lock:
             self.stop = False
             self.done = []
+        with _gc_paused():
+            return self._replay(trace, time_scale)
+
+    def _replay(self, trace, time_scale):
         threads = [threading.Thread(target=self._worker, args=(w,), daemon=True)
                    for w in range(self.nworkers)]
         for t in threads:
@@ -277,14 +365,52 @@ class Server:
         return summarize(self.done)
 
     def close(self):
+        """Free every device buffer the server holds (cached factors, pooled
+        buffers, per-worker scratch) and its streams."""
+        for h in self.streams:
+            call("dm_stream_sync", h)
+        with self.lock:
+            for e in self.cache.values():
+                self._release(e)
+            self.cache.clear()
+            for lst in self.pool.values():
+                for A, F in lst:
+                    self._free_pair(A, F)
+            self.pool.clear()
+            self.used = 0.0
+        for sc in self.scratch:
+            sc.free()
+        for pool in self.rhs:
+            for DB in pool.values():
+                DB.unsafe_free()
+            pool.clear()
         for h in self.streams:
             call("dm_stream_destroy", h)
         self.streams = []
 
 
+class _gc_paused:
+    """Full collection, then the cyclic collector off until exit."""
+
+    def __enter__(self):
+        self.was = gc.isenabled()
+        gc.collect()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        if self.was:
+            gc.enable()
+        return False
+
+
 def serve_one_at_a_time(trace, options=None):
     """Baseline: requests strictly one at a time in arrival order, each a
     plain FP64 tiled cholesky! + ldiv! on its own freshly built matrix."""
+    with _gc_paused():
+        return _one_at_a_time(trace, options)
+
+
+def _one_at_a_time(trace, options):
     options = options or PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True)
     t0 = time.perf_counter()
     done = []

@@ -357,7 +357,7 @@ int dm_forget_matrix(const void* base);
  * num_items systems; order_host: the `count` systems to process, as indices
  * into items_host (each in [0, num_items), processing order: largest first
  * is best).  info_dev[m] (num_items ints): LAPACK info of processed system m.
- * workgroups: persistent grid (0 = two per CU). */
+ * workgroups: persistent grid (0 = one 512-thread workgroup per CU). */
 typedef struct dm_batch_item {
     int64_t a_off, b_off;
     int n, nrhs, panel, pad_;

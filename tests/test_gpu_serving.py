@@ -12,11 +12,14 @@ def _trace(dm, count, rate, seed, n_hi=1500):
                               k_lo=1e2, k_hi=1e6, reuse=0.3, tols=(1e-8, 1e-13), nrhs_max=8)
 
 
-def test_serving_mix_equals_alone_and_meets_tolerance(dm):
+@pytest.mark.parametrize("mixed_min_n", [0, 4096])
+def test_serving_mix_equals_alone_and_meets_tolerance(dm, mixed_min_n):
+    """mixed_min_n 0: every request on the FP32 factor + refinement; 4096:
+    every request of this trace (n <= 1500) on the FP64 direct path."""
     from dagger_mi355x import serving
     from oracle import gen
     trace = _trace(dm, 24, rate=400.0, seed=3)
-    srv = serving.Server(workers=3, budget_bytes=8e9)
+    srv = serving.Server(workers=3, budget_bytes=8e9, mixed_min_n=mixed_min_n)
     stats = srv.run(trace)
     srv.close()
     assert stats["completed"] == len(trace)
@@ -25,7 +28,7 @@ def test_serving_mix_equals_alone_and_meets_tolerance(dm):
     # the same requests alone, one server per request: bitwise identical x
     for r in trace[:6]:
         alone = serving.Request(r.rid, r.mid, r.n, r.kappa, r.nrhs, r.tol, 0.0, False, r.seed)
-        s1 = serving.Server(workers=1, budget_bytes=8e9)
+        s1 = serving.Server(workers=1, budget_bytes=8e9, mixed_min_n=mixed_min_n)
         s1.run([alone])
         s1.close()
         assert np.array_equal(alone.x, r.x), r.rid
@@ -48,3 +51,17 @@ def test_serving_budget_limits_resident_bytes(dm):
     srv.close()
     assert stats["completed"] == len(trace)
     assert all(r.berr <= r.tol for r in trace)
+
+
+def test_server_runs_several_traces_and_keys_cache_by_matrix(dm):
+    """A server serves traces one after another; matrix ids repeat across
+    traces with other sizes, so cached factors are keyed by (id, n, kappa)."""
+    from dagger_mi355x import serving
+    srv = serving.Server(workers=2, budget_bytes=8e9)
+    t1 = _trace(dm, 8, rate=500.0, seed=5, n_hi=700)
+    t2 = _trace(dm, 8, rate=500.0, seed=6, n_hi=700)
+    s1 = srv.run(t1)
+    s2 = srv.run(t2)
+    srv.close()
+    assert s1["completed"] == len(t1) and s2["completed"] == len(t2)
+    assert all(r.berr <= r.tol for r in t1 + t2)
