@@ -266,8 +266,6 @@ int Dag::plan(const dm_plan_opts& o) {
 }
 
 int Dag::create_streams(bool eager) {
-  streams.resize(opts.num_streams);
-  (void)eager;
   int dev = 0;
   DM_HIP(hipGetDevice(&dev));
   DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -290,13 +288,20 @@ int Dag::create_streams(bool eager) {
       mask[i / 32] &= ~(1u << (i % 32));
     }
   }
-  for (size_t i = 0; i < streams.size(); ++i) {
-    if (i > 0 && !mask.empty())
-      DM_HIP(hipExtStreamCreateWithCUMask(&streams[i], (uint32_t)mask.size(), mask.data()));
-    else
-      DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
+  // An eagerly issued plan runs on the executor streams of its caller's
+  // stream (shared by all plans issued from it); a plan captured into a
+  // hipGraph, or one with CU-masked streams, owns its streams and the
+  // origin stream the capture runs on.
+  if (!eager || !mask.empty()) {
+    streams.resize(opts.num_streams);
+    for (size_t i = 0; i < streams.size(); ++i) {
+      if (i > 0 && !mask.empty())
+        DM_HIP(hipExtStreamCreateWithCUMask(&streams[i], (uint32_t)mask.size(), mask.data()));
+      else
+        DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
+    }
+    DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   }
-  DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   events.resize(groups.size());
   for (auto& e : events) DM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   fork_ev.resize(1);
@@ -601,11 +606,24 @@ int Dag::issue(hipStream_t o) {
   // reset the abort flag, fork, run groups, join
   int rc = fill_i32_launch(abort_dev, 0, 1, o);
   if (rc) return rc;
+  // worker streams: the plan's own, else the caller's shared executor
+  // streams; a one-stream eager plan runs on the caller's stream itself
+  hipStream_t ss[8];
+  const int ns = opts.num_streams;
+  if (!streams.empty()) {
+    for (int i = 0; i < ns; ++i) ss[i] = streams[i];
+  } else if (ns == 1) {
+    ss[0] = o;
+  } else {
+    rc = exec_streams(o, ns, ss);
+    if (rc) return rc;
+  }
   DM_HIP(hipEventRecord(fork_ev[0], o));
-  for (hipStream_t s : streams) DM_HIP(hipStreamWaitEvent(s, fork_ev[0], 0));
+  for (int i = 0; i < ns; ++i)
+    if (ss[i] != o) DM_HIP(hipStreamWaitEvent(ss[i], fork_ev[0], 0));
   for (int g : order) {
     const Group& G = groups[g];
-    hipStream_t s = streams[G.stream];
+    hipStream_t s = ss[G.stream];
     for (int w : G.waits) DM_HIP(hipStreamWaitEvent(s, events[w], 0));
     const bool prof = opts.profile && !capturing;
     if (prof) DM_HIP(hipEventRecord(prof_start[g], s));
@@ -614,8 +632,9 @@ int Dag::issue(hipStream_t o) {
     if (prof) DM_HIP(hipEventRecord(prof_stop[g], s));
     DM_HIP(hipEventRecord(events[g], s));
   }
-  for (size_t i = 0; i < streams.size(); ++i) {
-    DM_HIP(hipEventRecord(join_ev[i], streams[i]));
+  for (int i = 0; i < ns; ++i) {
+    if (ss[i] == o) continue;
+    DM_HIP(hipEventRecord(join_ev[i], ss[i]));
     DM_HIP(hipStreamWaitEvent(o, join_ev[i], 0));
   }
   if (post_hook) {
@@ -686,7 +705,9 @@ Dag::~Dag() {
       if (st.ev0) (void)hipEventDestroy(st.ev0);
       if (st.ev1) (void)hipEventDestroy(st.ev1);
     }
-  if (!streams.empty() || origin) (void)hipDeviceSynchronize();
+  // queued launches of this plan (on its own or on shared streams) may
+  // still read its tables
+  (void)hipDeviceSynchronize();
   for (auto& s : streams) (void)hipStreamDestroy(s);
   if (origin) (void)hipStreamDestroy(origin);
   if (dev_mem) (void)hipFree(dev_mem);

@@ -56,6 +56,15 @@ int run_big_stack(std::function<int()> fn);
 // library is loaded).  Calls ordered on one stream may reuse it.
 void* stream_workspace(hipStream_t s, size_t bytes);
 
+// Executor streams of eagerly issued plans, shared by every plan issued
+// from one caller stream: the first `count` streams of that caller's pool
+// (created on first use; at most 8).  Plans therefore add no streams of
+// their own, and a process's streams stay few beside the hardware queues
+// they are multiplexed onto.  Released with the caller stream
+// (dm_stream_destroy); the default stream's pool lives until unload.
+int exec_streams(hipStream_t caller, int count, hipStream_t* out);
+void release_exec_streams(hipStream_t caller);
+
 inline bool is_trans(char t) { return t == 'T' || t == 't' || t == 'C' || t == 'c'; }
 inline bool is_notrans(char t) { return t == 'N' || t == 'n'; }
 inline bool is_upper(char u) { return u == 'U' || u == 'u'; }

@@ -186,6 +186,39 @@ void* stream_workspace(hipStream_t s, size_t bytes) {
   return e.p;
 }
 
+// ----------------------------------------------------- executor streams
+namespace {
+std::mutex g_exec_mu;
+std::unordered_map<hipStream_t, std::vector<hipStream_t>> g_exec;
+}  // namespace
+
+int exec_streams(hipStream_t caller, int count, hipStream_t* out) {
+  std::lock_guard<std::mutex> lk(g_exec_mu);
+  std::vector<hipStream_t>& v = g_exec[caller];
+  while ((int)v.size() < count) {
+    hipStream_t h = nullptr;
+    DM_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+    v.push_back(h);
+  }
+  for (int i = 0; i < count; ++i) out[i] = v[i];
+  return 0;
+}
+
+void release_exec_streams(hipStream_t caller) {
+  std::vector<hipStream_t> v;
+  {
+    std::lock_guard<std::mutex> lk(g_exec_mu);
+    auto it = g_exec.find(caller);
+    if (it == g_exec.end()) return;
+    v.swap(it->second);
+    g_exec.erase(it);
+  }
+  for (hipStream_t h : v) {
+    (void)hipStreamSynchronize(h);
+    (void)hipStreamDestroy(h);
+  }
+}
+
 }  // namespace dm
 
 using namespace dm;
@@ -310,6 +343,7 @@ int dm_stream_create(dm_stream_t* s) {
 int dm_stream_destroy(dm_stream_t s) {
   if (!s) return 0;
   DM_INIT();
+  release_exec_streams(s->h);
   hipError_t e = hipStreamDestroy(s->h);
   delete s;
   if (e != hipSuccess) return hip_fail(e, "hipStreamDestroy");

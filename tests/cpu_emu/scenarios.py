@@ -220,7 +220,12 @@ def tiled():
             f = DA.collect()
             check(O.rel_factor_diff(f, ref, uplo) < 1e-12, ("cholesky", uplo))
             outs.setdefault(uplo, f)
-            check(np.array_equal(f, outs[uplo]), ("schedules differ", uplo))
+            # Bit-identical across schedules except for GEMMs on tiles of at
+            # most 48 columns (here the ragged 44-column edge): alone they run
+            # the K-split skinny kernel, batched with 64-column tiles the
+            # regular one, so the two agree to rounding.
+            check(np.abs(f - outs[uplo]).max() <= 4 * np.finfo(float).eps * np.abs(f).max(),
+                  ("schedules differ", uplo))
         DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
         Fc = dm.cholesky_(DA, uplo=uplo)
         for nrhs in (5, 1):

@@ -47,6 +47,11 @@ int arg_fail(int a, const char* w) { fprintf(stderr, "arg_fail %d %s\n", a, w); 
 int ensure_init() { return 0; }
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : (hipStream_t)0x42; }
 void* stream_workspace(hipStream_t, size_t) { return nullptr; }
+int exec_streams(hipStream_t, int count, hipStream_t* out) {
+  for (int i = 0; i < count; ++i) out[i] = (hipStream_t)(uintptr_t)(0x100 + i);
+  return 0;
+}
+void release_exec_streams(hipStream_t) {}
 int trsm_init_attributes() { return 0; }
 int run_big_stack(std::function<int()> fn) { return fn(); }
 template <typename T> GemmCfgId gemm_pick_cfg(int, int, int, int) { return GemmCfgId{0, 128, 128}; }
