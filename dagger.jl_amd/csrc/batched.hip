@@ -175,24 +175,57 @@ __global__ __launch_bounds__(BNT) void batched_posv_kernel(T* __restrict__ Aall,
         int ti, tj;
         btri_decode(t, ti, tj);
         const int i0 = j0 + ti * 32, jj0 = j0 + tj * 32;
+        // The tile's current values are read first and U12's k-strip in
+        // chunks of four k-steps, the next chunk loaded while the current
+        // one feeds the MFMAs: sixteen loads in flight instead of four.
+        // The k order of the accumulation is unchanged.
+        T cold[2][2][4];
+#pragma unroll
+        for (int hi = 0; hi < 2; ++hi)
+#pragma unroll
+          for (int hj = 0; hj < 2; ++hj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = i0 + hi * 16 + lr;
+              const int j = jj0 + hj * 16 + BMfma<T>::row(lk, r);
+              cold[hi][hj][r] = (i <= j && j < n) ? A[i + (size_t)j * ld] : T(0);
+            }
         v4 acc[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) acc[a][0] = acc[a][1] = v4{0, 0, 0, 0};
-        for (int p = 0; p < kb; p += 4) {
-          const bool pok = p + lk < kb;
-          const size_t pr = (size_t)(k0 + p + lk);
-          T xi[2], xj[2];
+        T xa[4][2][2], xb[4][2][2];  // [k-step][i/j][h]: two chunk buffers
+        auto load = [&](T (&x)[4][2][2], int p0) {
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int ci = i0 + h * 16 + lr, cj = jj0 + h * 16 + lr;
-            xi[h] = (pok && ci < n) ? A[pr + (size_t)ci * ld] : T(0);
-            xj[h] = (pok && cj < n) ? A[pr + (size_t)cj * ld] : T(0);
+          for (int q = 0; q < 4; ++q) {
+            const int pp = p0 + 4 * q + lk;
+            const bool pok = pp < kb;
+            const size_t pr = (size_t)(k0 + pp);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int ci = i0 + h * 16 + lr, cj = jj0 + h * 16 + lr;
+              x[q][0][h] = (pok && ci < n) ? A[pr + (size_t)ci * ld] : T(0);
+              x[q][1][h] = (pok && cj < n) ? A[pr + (size_t)cj * ld] : T(0);
+            }
           }
-          // D'[a][b] = sum_k X(k, j a) X(k, i b): lane holds j = .. + row(lk,r), i = .. + lr
+        };
+        // D'[a][b] = sum_k X(k, j a) X(k, i b): lane holds j = .. + row(lk,r), i = .. + lr
+        auto mma = [&](const T (&x)[4][2][2], int p0) {
 #pragma unroll
-          for (int hi = 0; hi < 2; ++hi)
+          for (int q = 0; q < 4; ++q) {
+            if (p0 + 4 * q >= kb) break;  // uniform
 #pragma unroll
-            for (int hj = 0; hj < 2; ++hj) acc[hi][hj] = BMfma<T>::mac(xj[hj], xi[hi], acc[hi][hj]);
+            for (int hi = 0; hi < 2; ++hi)
+#pragma unroll
+              for (int hj = 0; hj < 2; ++hj)
+                acc[hi][hj] = BMfma<T>::mac(x[q][1][hj], x[q][0][hi], acc[hi][hj]);
+          }
+        };
+        load(xa, 0);
+        for (int p0 = 0; p0 < kb; p0 += 32) {
+          if (p0 + 16 < kb) load(xb, p0 + 16);
+          mma(xa, p0);
+          if (p0 + 32 < kb) load(xa, p0 + 32);
+          if (p0 + 16 < kb) mma(xb, p0 + 16);
         }
 #pragma unroll
         for (int hi = 0; hi < 2; ++hi)
@@ -202,7 +235,7 @@ __global__ __launch_bounds__(BNT) void batched_posv_kernel(T* __restrict__ Aall,
             for (int r = 0; r < 4; ++r) {
               const int i = i0 + hi * 16 + lr;
               const int j = jj0 + hj * 16 + BMfma<T>::row(lk, r);
-              if (i <= j && j < n) A[i + (size_t)j * ld] -= acc[hi][hj][r];
+              if (i <= j && j < n) A[i + (size_t)j * ld] = cold[hi][hj][r] - acc[hi][hj][r];
             }
       }
       __syncthreads();
