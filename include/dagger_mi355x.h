@@ -86,6 +86,8 @@ int dm_fill_i32(int* dst, int value, size_t count, dm_stream_t s);
 /* Streams, events, graphs (ROCExt keeps one HIP stream per device,
  * ext/ROCExt.jl:93-97,532-553; this backend uses several). */
 int dm_stream_create(dm_stream_t* s);
+/* Also releases the executor streams that plans issued from s shared
+ * (eager plans fork onto a pool of streams kept per caller stream). */
 int dm_stream_destroy(dm_stream_t s);
 int dm_stream_sync(dm_stream_t s);
 int dm_device_sync(void);
