@@ -300,3 +300,32 @@ def test_tiled_gemm_block_column(dm, m, nb, nrhs):
     ref = 0.5 * Y - A @ X
     err = np.abs(DY.collect() - ref).max() / np.abs(ref).max()
     assert err < 1e-13
+
+
+def test_plans_on_user_streams_and_stream_release(dm):
+    """Eager plans fork onto executor streams shared per caller stream and
+    released with it: the same cached plan run from one user stream, then
+    (after that stream is destroyed) from a new one, and with one stream
+    directly on the caller's stream, gives the default stream's bits."""
+    import ctypes as C
+    from dagger_mi355x._lib import call
+    n, nb = 1536, 256
+    A = gen.spd(n, 21)
+    _, want = factor_gpu(dm, A, nb, options=dm.PlanOptions(num_streams=2, batch=True,
+                                                            lookahead=1, use_graph=False))
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    info = dm.DeviceInt(2)
+    for ns in (2, 3, 1):
+        dag = dm.Dag.cholesky(DA, "U", info)
+        dag.plan(dm.PlanOptions(num_streams=ns, batch=True, lookahead=1 if ns > 1 else 0,
+                                use_graph=False))
+        assert dag.stats()["graph"] == 0
+        for rep in range(2):
+            h = C.c_void_p()
+            call("dm_stream_create", C.byref(h))
+            DA.copy_from_host(A, h)
+            dag.run(h)
+            got = DA.collect(h)
+            call("dm_stream_destroy", h)
+            assert int(info.get()[0]) == 0
+            assert np.array_equal(got, want), (ns, rep)
