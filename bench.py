@@ -58,6 +58,41 @@ class Timer:
         return ms.value
 
 
+def norm2_leg(dm, A, timer, reps=5):
+    """Device norm2 + chkfinite pass (dm_tiled_norm2: one HBM read of A, sum
+    of squares in FP64 plus the non-finite flag) over the FP64 matrix: mean
+    time of `reps` passes (HIP events on the launching stream) and achieved
+    GB/s = 8 N^2 bytes / time, against MI355X's 8 TB/s."""
+    from dagger_mi355x.darray import DeviceBuffer, DeviceInt
+    call = dm._lib.call
+    work = DeviceBuffer(8 * 8 * A.mt * A.nt)
+    out = DeviceBuffer(8)
+    flag = DeviceInt(1)
+    t = A.tiled()
+
+    def once():
+        call("dm_tiled_norm2", C.byref(t), 0, C.c_void_p(out.ptr), C.c_void_p(flag.ptr),
+             C.c_void_p(work.ptr), None)
+
+    once()
+    call("dm_device_sync")
+    times = []
+    for _ in range(reps):
+        timer.start()
+        once()
+        times.append(timer.stop_ms())
+    val = np.zeros(1)
+    call("dm_memcpy_d2h", val.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr), C.c_size_t(8), None)
+    finite = not bool(flag.get()[0])
+    for b in (work, out, flag):
+        b.free()
+    ms = sum(times) / len(times)
+    gbs = A.m * A.n * 8 / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 3), "gb_per_s": round(gbs, 1), "peak_gb_per_s": 8000.0,
+            "frac": round(gbs / 8000.0, 3), "bytes": A.m * A.n * 8, "norm2": float(val[0]),
+            "finite": finite}
+
+
 def restore(call, dst, src):
     call("dm_memcpy_d2d", C.c_void_p(dst.buffer.ptr), C.c_void_p(src.buffer.ptr),
          C.c_size_t(src.m * src.n * 8), None)
@@ -348,6 +383,8 @@ def main():
     if not a.no_roofline:
         roof, by_op = gemm_roofline(dm, A0, DA, nb)
         log(f"roofline: {roof}")
+    norm_res = norm2_leg(dm, A0, timer)
+    log(f"norm2 pass: {norm_res}")
     mixed_res = None
     if a.mixed_n > 0:
         DA.unsafe_free()
@@ -392,6 +429,7 @@ def main():
         "first_call_s": round(first_s, 3),
         "roofline": roof,
         "groups_by_op": by_op,
+        "norm2_pass": norm_res,
         "mixed_ir": mixed_res,
         "serving": serving_res,
         "batched": batched_res,

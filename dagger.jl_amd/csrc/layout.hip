@@ -238,3 +238,135 @@ extern "C" int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* wor
   DM_LAUNCH_CHECK("all_ok_kernel");
   return 0;
 }
+
+// ------------------------------------------------ norm2 and chkfinite
+// One pass over the stored tiles: per-workgroup partial sums of squares
+// (accumulated in FP64, fixed order) of the elements in the requested part
+// (0 whole, 1 upper triangle with the diagonal, 2 lower triangle with the
+// diagonal), and a flag set when any element of the part is not finite.
+// The reference sums norm_sqr per chunk and then over chunks
+// (src/array/linalg.jl:1-34); chkfinite is all(isfinite) per chunk
+// (src/array/linalg.jl:127-148).
+namespace dm {
+
+constexpr int kSumsqThreads = 256;
+constexpr int kSumsqMaxPerTile = 8;
+
+template <typename T>
+__global__ __launch_bounds__(kSumsqThreads) void tiled_sumsq_kernel(
+    const T* __restrict__ base, int64_t m, int64_t n, int mb, int nb, int part, int bpt,
+    double* __restrict__ partial, int* __restrict__ nonfinite) {
+  __shared__ double red[kSumsqThreads];
+  const int64_t mt = (m + mb - 1) / mb;
+  const int64_t tile = blockIdx.x / bpt;
+  const int sub = (int)(blockIdx.x - tile * bpt);
+  const int64_t ti = tile % mt, tj = tile / mt;
+  const int64_t r0 = ti * mb, c0 = tj * nb;
+  const int h = (int)min((int64_t)mb, m - r0), w = (int)min((int64_t)nb, n - c0);
+  const T* __restrict__ t = base + tj * (int64_t)nb * m + ti * (int64_t)mb * w;
+  // tiles wholly outside the part contribute nothing
+  const bool skip = (part == 1 && r0 > c0 + w - 1) || (part == 2 && c0 > r0 + h - 1);
+  const bool whole = part == 0 || (part == 1 && r0 + h - 1 <= c0) ||
+                     (part == 2 && c0 + w - 1 <= r0);
+  double acc = 0.0;
+  bool bad = false;
+  if (!skip && whole) {
+    // four independent loads in flight per thread, four partial sums
+    // combined in a fixed order
+    const int count = h * w, stride = bpt * kSumsqThreads;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int e = sub * kSumsqThreads + threadIdx.x;
+    for (; e + 3 * stride < count; e += 4 * stride) {
+      const double x0 = (double)t[e], x1 = (double)t[e + stride];
+      const double x2 = (double)t[e + 2 * stride], x3 = (double)t[e + 3 * stride];
+      bad = bad || !isfinite(x0) || !isfinite(x1) || !isfinite(x2) || !isfinite(x3);
+      a0 += x0 * x0;
+      a1 += x1 * x1;
+      a2 += x2 * x2;
+      a3 += x3 * x3;
+    }
+    for (; e < count; e += stride) {
+      const double x = (double)t[e];
+      bad = bad || !isfinite(x);
+      a0 += x * x;
+    }
+    acc = (a0 + a1) + (a2 + a3);
+  } else if (!skip) {
+    const int count = h * w;
+    for (int e = sub * kSumsqThreads + threadIdx.x; e < count; e += bpt * kSumsqThreads) {
+      bool in = whole;
+      if (!in) {
+        const int cl = e / h, rl = e - cl * h;
+        in = part == 1 ? (r0 + rl <= c0 + cl) : (r0 + rl >= c0 + cl);
+      }
+      if (in) {
+        const double x = (double)t[e];
+        bad = bad || !isfinite(x);
+        acc += x * x;
+      }
+    }
+  }
+  if (bad) *nonfinite = 1;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kSumsqThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kSumsqThreads) void sum_sqrt_kernel(const double* __restrict__ partial,
+                                                                 int64_t count,
+                                                                 double* __restrict__ out) {
+  __shared__ double red[kSumsqThreads];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += kSumsqThreads) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kSumsqThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sqrt(red[0]);
+}
+
+}  // namespace dm
+
+// *out_dev = sqrt(sum of squares over the part), *nonfinite_dev = 1 if an
+// element of the part is Inf or NaN, else 0.  work_dev: 8 * mt * nt doubles.
+extern "C" int dm_tiled_norm2(const dm_tiled* A, int part, double* out_dev, int* nonfinite_dev,
+                              double* work_dev, dm_stream_t s) {
+  if (!A) return arg_fail(1, "A is NULL");
+  if (A->m < 0 || A->n < 0 || A->mb < 1 || A->nb < 1) return arg_fail(1, "bad tiled matrix");
+  if (A->dtype != DM_F64 && A->dtype != DM_F32) return arg_fail(1, "dtype must be FP64 or FP32");
+  if (part < 0 || part > 2) return arg_fail(2, "part must be 0 (whole), 1 (upper) or 2 (lower)");
+  if (!out_dev || !nonfinite_dev || !work_dev) return arg_fail(3, "NULL output/flag/work");
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = stream_of(s);
+  rc = fill_i32_launch(nonfinite_dev, 0, 1, st);
+  if (rc) return rc;
+  const int64_t mt = (A->m + A->mb - 1) / A->mb, nt = (A->n + A->nb - 1) / A->nb;
+  const int64_t tiles = mt * nt;
+  if (tiles == 0) return fill_i32_launch((int*)out_dev, 0, 2, st);
+  if ((int64_t)A->mb * A->nb > (int64_t)1 << 30) return arg_fail(1, "tiles above 2^30 elements");
+  if (tiles * kSumsqMaxPerTile > ((int64_t)1 << 31) - 1) return arg_fail(1, "too many tiles");
+  // about 2048 workgroups in all, at most 8 per tile
+  int bpt = (int)((2048 + tiles - 1) / tiles);
+  bpt = bpt < 1 ? 1 : (bpt > kSumsqMaxPerTile ? kSumsqMaxPerTile : bpt);
+  const dim3 grid((unsigned)(tiles * bpt));
+  if (A->dtype == DM_F64)
+    hipLaunchKernelGGL((tiled_sumsq_kernel<double>), grid, dim3(kSumsqThreads), 0, st,
+                       (const double*)A->base, A->m, A->n, A->mb, A->nb, part, bpt, work_dev,
+                       nonfinite_dev);
+  else
+    hipLaunchKernelGGL((tiled_sumsq_kernel<float>), grid, dim3(kSumsqThreads), 0, st,
+                       (const float*)A->base, A->m, A->n, A->mb, A->nb, part, bpt, work_dev,
+                       nonfinite_dev);
+  DM_LAUNCH_CHECK("tiled_sumsq_kernel");
+  hipLaunchKernelGGL(sum_sqrt_kernel, dim3(1), dim3(kSumsqThreads), 0, st, (const double*)work_dev,
+                     tiles * bpt, out_dev);
+  DM_LAUNCH_CHECK("sum_sqrt_kernel");
+  return 0;
+}

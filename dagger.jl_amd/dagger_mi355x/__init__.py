@@ -17,13 +17,14 @@ from .darray import Blocks, DArray, DeviceBuffer, DeviceInt, Tile, zeros, randn,
 from .datadeps import (In, Out, InOut, BLAS, LAPACK, potrf_checked, spawn, spawn_datadeps,
                        PlanOptions, SEQUENTIAL, Dag)
 from .linalg import (Cholesky, PosDefException, cholesky, cholesky_, ldiv_, trsm_, trsv_,
-                     ishermitian, DEFAULT_OPTIONS)
+                     ishermitian, norm2, chkfinite, DEFAULT_OPTIONS)
 
 __all__ = [
     "Blocks", "DArray", "DeviceBuffer", "DeviceInt", "Tile", "zeros", "randn", "spd_generate",
     "In", "Out", "InOut", "BLAS", "LAPACK", "potrf_checked", "spawn", "spawn_datadeps",
     "PlanOptions", "SEQUENTIAL", "Dag", "Cholesky", "PosDefException", "cholesky",
-    "cholesky_", "ldiv_", "trsm_", "trsv_", "ishermitian", "DEFAULT_OPTIONS", "DMError",
+    "cholesky_", "ldiv_", "trsm_", "trsv_", "ishermitian", "norm2", "chkfinite",
+    "DEFAULT_OPTIONS", "DMError",
     "load_library",
     "init",
 ]

@@ -24,7 +24,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call
-from .darray import DArray, Blocks, DeviceInt
+from .darray import DArray, Blocks, DeviceBuffer, DeviceInt
 from .datadeps import (spawn, spawn_datadeps, In, Out, InOut, BLAS, potrf_checked,
                        PlanOptions)
 
@@ -151,6 +151,49 @@ def ishermitian(A):
     if B is not A:
         B.unsafe_free()
     return ok
+
+
+_PARTS = {"full": 0, "upper": 1, "lower": 2}
+
+
+def _norm2_pass(A, part, stream=None):
+    """One dm_tiled_norm2 pass: (sqrt of the sum of squares, any non-finite)."""
+    if A.ndim == 1:
+        A = DArray(A.dtype, (A.m, 1), Blocks(A.mb, 1), _buffer=A.buffer)
+    work = DeviceBuffer(8 * 8 * max(A.mt * A.nt, 1))
+    out = DeviceBuffer(8)
+    flag = DeviceInt(1)
+    t = A.tiled()
+    call("dm_tiled_norm2", C.byref(t), _PARTS[part], C.c_void_p(out.ptr), C.c_void_p(flag.ptr),
+         C.c_void_p(work.ptr), stream)
+    val = np.zeros(1)
+    call("dm_memcpy_d2h", val.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr), C.c_size_t(8),
+         stream)
+    bad = bool(flag.get(stream)[0])
+    for b in (work, out, flag):
+        b.free()
+    return float(val[0]), bad
+
+
+def norm2(A, part="full"):
+    """`LinearAlgebra.norm2(::DArray)` (src/array/linalg.jl:1-5) and of its
+    UpperTriangular / LowerTriangular views (part "upper" / "lower",
+    src/array/linalg.jl:7-33): sqrt of the sum of squares over the stored
+    tiles, one device pass (FP64 accumulation)."""
+    if part not in _PARTS:
+        raise ValueError("part must be 'full', 'upper' or 'lower'")
+    if part != "full" and (A.ndim != 2 or A.partitioning != _square_blocks(A)):
+        raise ValueError("triangular norm2 needs a matrix with square blocks")
+    return _norm2_pass(A, part)[0]
+
+
+def chkfinite(A):
+    """`LAPACK.chkfinite(::DArray)` (src/array/linalg.jl:127-148): True when
+    every element is finite, else raises ValueError("matrix has Inf or NaN")
+    (the reference's ArgumentError), from one device pass."""
+    if _norm2_pass(A, "full")[1]:
+        raise ValueError("matrix has Inf or NaN")
+    return True
 
 
 def cholesky_(A, uplo="U", check=True, via="native", options=None, check_hermitian=True):

@@ -289,6 +289,16 @@ int dm_cholesky_tiled(char uplo, const dm_tiled* A, int* info_dev,
  * work_dev: mt*(mt+1)/2 ints. */
 int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* work_dev, dm_stream_t s);
 
+/* LinearAlgebra.norm2 of a DArray (src/array/linalg.jl:1-5), of its
+ * UpperTriangular view (:7-19) or LowerTriangular view (:21-33), selected by
+ * part = 0 / 1 / 2; and the finiteness test of LAPACK.chkfinite(::DArray)
+ * (src/array/linalg.jl:127-148) over the same elements, in one HBM pass.
+ * *out_dev = sqrt of the sum of squares (accumulated in FP64 in a fixed
+ * order), *nonfinite_dev = 1 when an element is Inf or NaN, else 0.
+ * FP64 or FP32.  work_dev: 8 * mt * nt doubles. */
+int dm_tiled_norm2(const dm_tiled* A, int part, double* out_dev, int* nonfinite_dev,
+                   double* work_dev, dm_stream_t s);
+
 /* ldiv!(C::Cholesky, B) for a matrix right-hand side (src/array/linalg.jl:
  * 385-401): two tiled trsm! passes (src/array/trsm.jl:59-192) over the
  * resident factor.  B is tiled with Blocks(nb, nb) rows matching A. */

@@ -16,6 +16,9 @@ What it restates (paths inside JuliaParallel/Dagger.jl):
   * ldiv!(::Cholesky{T,<:DMatrix}, B)   src/array/linalg.jl:352-405
   * datadeps dependency rules           src/datadeps/queue.jl:44-54,
                                         src/datadeps/aliasing.jl:740-773
+  * norm2 of a DArray and of its UpperTriangular / LowerTriangular views
+                                        src/array/linalg.jl:1-33
+  * LAPACK.chkfinite(::DArray)          src/array/linalg.jl:127-148
 The tile kernels are the same LAPACK/BLAS routines the reference calls on
 its CPU path (LAPACK.potrf!, BLAS.trsm!/syrk!/gemm!/gemv!/trsv!), here
 through scipy's bundled OpenBLAS.
@@ -511,3 +514,35 @@ def backward_error(A, X, B):
     return float(np.abs(R).sum(axis=1).max() /
                  (np.abs(A).sum(axis=1).max() * np.abs(X2).sum(axis=1).max() +
                   np.abs(B2).sum(axis=1).max()))
+
+
+# ---------------------------------------------------------------- norms
+def norm2(A, nb, part="full"):
+    """norm2 (src/array/linalg.jl:1-33): per chunk mapreduce(norm_sqr, +),
+    then the sum over chunks, then sqrt.  part "upper" takes the chunks
+    right of the diagonal chunk plus UpperTriangular of each diagonal chunk
+    (:7-19); "lower" the mirror image (:21-33).  Square blocks."""
+    T = TiledMatrix(A, nb)
+    sq = lambda X: float(np.sum(np.abs(X.astype(np.float64)) ** 2))
+    if part == "full":
+        return float(np.sqrt(sum(sq(T.chunks[i, j]) for i in range(T.mt) for j in range(T.nt))))
+    off, diag = 0.0, 0.0
+    for i in range(T.mt):
+        if part == "upper":
+            off += sum(sq(T.chunks[i, j]) for j in range(i + 1, T.nt))
+            diag += sq(np.triu(T.chunks[i, i]))
+        else:
+            off += sum(sq(T.chunks[j, i]) for j in range(i + 1, T.mt))
+            diag += sq(np.tril(T.chunks[i, i]))
+    return float(np.sqrt(off + diag))
+
+
+def chkfinite(A, nb):
+    """LAPACK.chkfinite(::DArray) (src/array/linalg.jl:127-148): every
+    chunk all-finite, else ArgumentError (here ValueError)."""
+    T = TiledMatrix(A, nb)
+    for i in range(T.mt):
+        for j in range(T.nt):
+            if not np.all(np.isfinite(T.chunks[i, j])):
+                raise ValueError("matrix has Inf or NaN")
+    return True

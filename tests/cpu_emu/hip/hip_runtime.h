@@ -19,6 +19,7 @@
 #pragma once
 #include <algorithm>
 #include <cmath>
+using std::isfinite;  // a global device function under HIP
 #include <cstddef>
 #include <cstdint>
 #include <cstring>

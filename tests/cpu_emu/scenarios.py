@@ -394,6 +394,34 @@ def capped():
 def quick():
     """A few seconds of every kernel family: the CPU suite's default group."""
     rng = np.random.default_rng(2)
+    # norm2 / chkfinite pass over ragged tiles, whole and triangular parts
+    for dt in (np.float64, np.float32):
+        A = rng.standard_normal((75, 75)).astype(dt)
+        DA = dm.DArray.from_array(A, dm.Blocks(16, 16))
+        A64 = A.astype(np.float64)
+        for part, ref in (("full", A64), ("upper", np.triu(A64)), ("lower", np.tril(A64))):
+            want = np.linalg.norm(ref)
+            check(abs(dm.norm2(DA, part) - want) <= 1e-13 * want, ("norm2", part, dt))
+        check(dm.chkfinite(DA), "chkfinite")
+        # large tiles: the four-way unrolled loop of the whole-tile path
+        L = rng.standard_normal((600, 500)).astype(dt)
+        DL = dm.DArray.from_array(L, dm.Blocks(300, 300))
+        want = np.linalg.norm(L.astype(np.float64))
+        check(abs(dm.norm2(DL) - want) <= 1e-13 * want, ("norm2 large tiles", dt))
+        L[599, 499] = np.inf
+        DL.copy_from_host(L)
+        try:
+            dm.chkfinite(DL)
+            check(False, "chkfinite missed an Inf")
+        except ValueError:
+            pass
+        A[74, 74] = np.nan
+        DA.copy_from_host(A)
+        try:
+            dm.chkfinite(DA)
+            check(False, "chkfinite missed a NaN")
+        except ValueError:
+            pass
     for ta, tb, (m, n, k) in [("T", "N", (70, 40, 33)), ("N", "T", (5, 9, 3))]:
         A = rng.standard_normal((k, m) if ta == "T" else (m, k))
         B = rng.standard_normal((n, k) if tb == "T" else (k, n))

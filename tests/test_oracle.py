@@ -220,3 +220,31 @@ def test_randn_moments_and_determinism():
     A = gen.spd(64, 1)
     assert np.array_equal(A, A.T)
     assert np.linalg.eigvalsh(A).min() > 0.9
+
+
+@pytest.mark.parametrize("m,n,nb", [(300, 300, 64), (257, 129, 32), (64, 64, 64), (1, 1, 4)])
+def test_norm2_closed_forms(m, n, nb):
+    """The chunked norm2 restatement against the dense Frobenius norm, its
+    triangular views against triu/tril, and exact values on constant and
+    identity matrices (a dropped diagonal or chunk would fail these)."""
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.standard_normal((m, n))
+    assert abs(O.norm2(A, nb) - np.linalg.norm(A)) <= 1e-13 * np.linalg.norm(A)
+    assert O.norm2(np.ones((m, n)), nb) == pytest.approx(np.sqrt(m * n), rel=1e-15)
+    if m == n:
+        up, lo = O.norm2(A, nb, "upper"), O.norm2(A, nb, "lower")
+        assert up == pytest.approx(np.linalg.norm(np.triu(A)), rel=1e-13)
+        assert lo == pytest.approx(np.linalg.norm(np.tril(A)), rel=1e-13)
+        d2 = float(np.sum(np.diag(A) ** 2))
+        assert up ** 2 + lo ** 2 - d2 == pytest.approx(np.linalg.norm(A) ** 2, rel=1e-12)
+        assert O.norm2(np.eye(n), nb, "upper") == pytest.approx(np.sqrt(n), rel=1e-15)
+
+
+def test_chkfinite_restatement():
+    A = np.ones((130, 70))
+    assert O.chkfinite(A, 32)
+    for bad in (np.nan, np.inf, -np.inf):
+        B = A.copy()
+        B[129, 69] = bad  # last element of the ragged corner chunk
+        with pytest.raises(ValueError, match="Inf or NaN"):
+            O.chkfinite(B, 32)
