@@ -5,6 +5,8 @@ Host mirror of Dagger's tiled linear algebra for this path:
                          (src/array/cholesky.jl:19-108)
   * `cholesky(A)`        out of place via cholcopy = copy (src/array/cholesky.jl:1)
   * `ldiv_(C, B)`        ldiv!(::Cholesky{T,<:DMatrix}, B) (src/array/linalg.jl:352-405)
+  * `ldiv(C, B)`         C \\ B staged in the factor's blocking (src/array/linalg.jl:208-246)
+  * `ldiv_triangular_`   ldiv!(UpperTriangular(A), B) etc. (src/array/linalg.jl:310-335)
   * `trsm_`, `trsv_`     Dagger.trsm!/trsv! (src/array/trsm.jl:1-192)
 Block re-partitioning follows maybe_copy_buffered (src/array/copy.jl:3-10):
 arguments whose partitioning differs from the requested square Blocks are
@@ -352,4 +354,61 @@ def ldiv_(F, B, via="native", options=None):
             _trsm_datadeps("L", "L", "T", "N", 1.0, pa.chunks, pb.chunks, options)
 
     _maybe_copy_buffered([(pA, sq), (B, partB)], mat)
+    return B
+
+
+def ldiv(F, B, via="native", options=None):
+    """`F \\ B` for `F::Cholesky{T,<:DMatrix}`: a new DArray X with A X = B;
+    B (a host array or a DArray) is left as it was.
+
+    The stdlib `ldiv(F, B)` stages B into a plain array whose automatic
+    blocking almost never matches the factor's, so every solve would re-block
+    the whole factor (src/array/linalg.jl:208-220). As the reference's LU
+    override does (src/array/linalg.jl:221-246), B is staged once into a
+    DArray with the factor's square blocking, so `ldiv!`
+    (src/array/linalg.jl:352-405) runs without any copy of the factor."""
+    A = F.factors
+    if A.m != A.n:
+        raise ValueError("the factor must be square")
+    if not isinstance(B, DArray):
+        B = np.asarray(B)
+    B_shape = B.shape
+    if len(B_shape) not in (1, 2):
+        raise ValueError("B must be a vector or a matrix")
+    if B_shape[0] != A.m:
+        raise ValueError("arguments must have the same number of rows")
+    if np.result_type(B.dtype, A.dtype) != A.dtype:
+        raise NotImplementedError(
+            f"promoting a {A.dtype} factor to the right-hand side's {B.dtype} is not supported")
+    bs = min(A.partitioning.blocksize)
+    part = Blocks(bs) if len(B_shape) == 1 else Blocks(bs, bs)
+    if isinstance(B, DArray):
+        if B.dtype != A.dtype:
+            raise NotImplementedError("a DArray right-hand side must have the factor's type")
+        X = B.copy() if B.partitioning == part else B.reblock(part)
+    else:
+        X = DArray.from_array(np.asarray(B, dtype=A.dtype), part)
+    call("dm_stream_sync", None)
+    return ldiv_(F, X, via=via, options=options)
+
+
+def ldiv_triangular_(uplo, diag, A, B, options=None):
+    """`ldiv!(T(A), B)` for T in {Upper,UnitUpper,Lower,UnitLower}Triangular
+    of a DMatrix (src/array/linalg.jl:310-335): B overwritten by T(A) \\ B,
+    through `trsv!` for a vector and `trsm!('L', uplo, 'N', diag, 1, ...)` for
+    a matrix, both on A and B re-blocked to A's smallest block size."""
+    if uplo not in ("U", "L"):
+        raise ValueError("uplo must be 'U' or 'L'")
+    if diag not in ("N", "U"):
+        raise ValueError("diag must be 'N' or 'U'")
+    if A.m != A.n or B.m != A.m:
+        raise ValueError("A must be square with as many rows as B")
+    bsa = min(A.partitioning.blocksize)
+    if B.ndim == 1:
+        bs = min(bsa, B.partitioning.blocksize[0])
+        _maybe_copy_buffered([(A, Blocks(bs, bs)), (B, Blocks(bs))],
+                             lambda pa, pb: trsv_(uplo, "N", diag, 1.0, pa, pb, options))
+        return B
+    _maybe_copy_buffered([(A, Blocks(bsa, bsa)), (B, Blocks(bsa, bsa))],
+                         lambda pa, pb: trsm_("L", uplo, "N", diag, 1.0, pa, pb, options=options))
     return B

@@ -453,6 +453,15 @@ def trsv_region(uplo, trans, diag, TA, TB):
     return R
 
 
+def trsv(uplo, trans, diag, A, b, nb):
+    """Dagger.trsv!(uplo, trans, diag, 1, A, b) (src/array/trsm.jl:1-57) on
+    A tiled nb x nb and b tiled nb; returns x."""
+    TA = TiledMatrix(np.array(A, copy=True), nb, nb)
+    TB = TiledMatrix(np.array(b, copy=True), nb)
+    run_sequential(trsv_region(uplo, trans, diag, TA, TB))
+    return TB.collect().ravel()
+
+
 def ldiv(F, uplo, B, nb, threaded=False, workers=None):
     """ldiv!(Cholesky(F, uplo), B) (src/array/linalg.jl:352-405): the two
     triangular-solve regions over the tiled factor; returns X."""

@@ -422,6 +422,27 @@ def quick():
             check(False, "chkfinite missed a NaN")
         except ValueError:
             pass
+    # F \\ B staged in the factor's blocking; triangular ldiv! (vector and
+    # matrix right-hand sides, unit diagonal, B re-blocked and copied back)
+    A = gen.spd(70, 9)
+    DA = dm.DArray.from_array(A, dm.Blocks(32, 32))
+    F = dm.cholesky_(DA, uplo="L")
+    for B in (rng.standard_normal(70), rng.standard_normal((70, 3))):
+        X = dm.ldiv(F, B).collect()
+        check(relerr(X, np.linalg.solve(A, B)) < 1e-11, ("backslash", B.ndim))
+    T = 0.1 * rng.standard_normal((70, 70)) + np.diag(np.full(70, 2.0))
+    DT = dm.DArray.from_array(T, dm.Blocks(32, 32))
+    for uplo in ("U", "L"):
+        Tu = np.triu(T, 1) if uplo == "U" else np.tril(T, -1)
+        Tu = Tu + np.eye(70)
+        b = rng.standard_normal(70)
+        Db = dm.DArray.from_array(b, dm.Blocks(32))
+        dm.ldiv_triangular_(uplo, "U", DT, Db)
+        check(relerr(Db.collect(), np.linalg.solve(Tu, b)) < 1e-12, ("trsv unit", uplo))
+        Bm = rng.standard_normal((70, 5))
+        DBm = dm.DArray.from_array(Bm, dm.Blocks(20, 2))
+        dm.ldiv_triangular_(uplo, "U", DT, DBm)
+        check(relerr(DBm.collect(), np.linalg.solve(Tu, Bm)) < 1e-12, ("trsm unit", uplo))
     for ta, tb, (m, n, k) in [("T", "N", (70, 40, 33)), ("N", "T", (5, 9, 3))]:
         A = rng.standard_normal((k, m) if ta == "T" else (m, k))
         B = rng.standard_normal((n, k) if tb == "T" else (k, n))

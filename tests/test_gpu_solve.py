@@ -1,0 +1,102 @@
+"""GPU parity of `F \\ B` (staged in the factor's blocking) and of the
+triangular `ldiv!` methods against the oracle's restatement of
+src/array/linalg.jl:208-246, :310-335 and :352-405 (trsm!/trsv! regions of
+src/array/trsm.jl).
+
+Tolerances: the GPU and the oracle run the same tile DAG with different
+FP64 kernels (different summation orders), so solutions agree to a relative
+1e-12 in norm on these well-conditioned systems; backward errors stay below
+10 n eps."""
+import numpy as np
+import pytest
+
+from oracle import dagger_tiled as O
+from oracle import gen
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def rel(x, ref):
+    return np.linalg.norm(x - ref) / np.linalg.norm(ref)
+
+
+def factor(dm, n, nb, uplo, seed):
+    A = gen.spd(n, seed)
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    return A, DA, dm.cholesky_(DA, uplo=uplo)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("n,nb,nrhs", [(1000, 128, 0), (1000, 128, 7), (768, 256, 32)])
+def test_backslash_host_rhs(dm, uplo, n, nb, nrhs):
+    A, DA, F = factor(dm, n, nb, uplo, 40 + n)
+    B = gen.randn(n * max(nrhs, 1), 41).reshape((n,) if nrhs == 0 else (nrhs, n)).T.copy()
+    B0 = B.copy()
+    X = dm.ldiv(F, B)
+    assert X.partitioning == (dm.Blocks(nb) if nrhs == 0 else dm.Blocks(nb, nb))
+    x = X.collect()
+    assert np.array_equal(B, B0)
+    ref = O.ldiv(DA.collect(), uplo, B, nb)
+    assert rel(x, ref) <= 1e-12
+    assert O.backward_error(A, x, B) <= 10 * n * EPS
+
+
+def test_backslash_darray_rhs_other_blocking(dm):
+    # B blocked differently from the factor: staged by one re-block of B, the
+    # factor is not copied, and B itself is left unchanged
+    n, nb = 900, 256
+    A, DA, F = factor(dm, n, nb, "U", 50)
+    B = gen.randn(n * 5, 51).reshape(5, n).T.copy()
+    DB = dm.DArray.from_array(B, dm.Blocks(100, 3))
+    base = F.factors.buffer.ptr
+    X = dm.ldiv(F, DB)
+    assert F.factors.buffer.ptr == base
+    assert X.partitioning == dm.Blocks(nb, nb)
+    assert np.array_equal(DB.collect(), B)
+    assert rel(X.collect(), O.ldiv(DA.collect(), "U", B, nb)) <= 1e-12
+    with pytest.raises(ValueError):
+        dm.ldiv(F, np.zeros(n + 1))
+
+
+def tri_system(n, uplo, diag, seed):
+    rng = np.random.default_rng(seed)
+    A = 0.5 * rng.standard_normal((n, n)) / np.sqrt(n) + np.diag(np.full(n, 2.0))
+    if diag == "U":
+        # stored diagonal deliberately not 1: a unit solve must ignore it
+        A[np.diag_indices(n)] = 3.0
+    T = np.triu(A) if uplo == "U" else np.tril(A)
+    if diag == "U":
+        T[np.diag_indices(n)] = 1.0
+    return A, T
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("vector", [False, True])
+def test_triangular_ldiv(dm, uplo, diag, vector):
+    n, nb = 700, 128
+    A, T = tri_system(n, uplo, diag, 60 + 2 * vector + (uplo == "L"))
+    B = gen.randn(n if vector else n * 9, 61).reshape((n,) if vector else (9, n)).T.copy()
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    DB = dm.DArray.from_array(B, dm.Blocks(nb) if vector else dm.Blocks(nb, nb))
+    dm.ldiv_triangular_(uplo, diag, DA, DB)
+    x = DB.collect()
+    if vector:
+        ref = O.trsv(uplo, "N", diag, A, B, nb)
+    else:
+        ref = O.trsm("L", uplo, "N", diag, 1.0, A, B, nb)
+    assert rel(x, ref) <= 1e-12
+    assert rel(x, np.linalg.solve(T, B)) <= 1e-11
+
+
+def test_triangular_ldiv_reblocks_rhs(dm):
+    # B blocked unlike A: re-blocked to A's block size and copied back
+    n = 500
+    A, T = tri_system(n, "L", "N", 70)
+    B = gen.randn(n * 4, 71).reshape(4, n).T.copy()
+    DA = dm.DArray.from_array(A, dm.Blocks(128, 128))
+    DB = dm.DArray.from_array(B, dm.Blocks(77, 4))
+    dm.ldiv_triangular_("L", "N", DA, DB)
+    assert DB.partitioning == dm.Blocks(77, 4)
+    assert rel(DB.collect(), O.trsm("L", "L", "N", "N", 1.0, A, B, 128)) <= 1e-12

@@ -148,6 +148,22 @@ def test_trsm_all_branches(side, uplo, trans, diag):
 
 
 @pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trsv_all_branches(uplo, trans, diag):
+    # ragged last tile (100 = 3*32 + 4); the stored diagonal is not 1, so a
+    # unit-diagonal solve that read it would fail
+    n, nb = 100, 32
+    rng = np.random.default_rng(sum(map(ord, "v" + uplo + trans + diag)))
+    A = 0.05 * rng.standard_normal((n, n)) + np.diag(np.full(n, 2.0))
+    b = rng.standard_normal(n)
+    x = O.trsv(uplo, trans, diag, A, b, nb)
+    T = _tri(A, uplo, diag)
+    ref = np.linalg.solve(T.T if trans == "T" else T, b)
+    assert np.allclose(x, ref, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("vector", [False, True])
 def test_ldiv_solves_system(uplo, vector):
     n, nb = 160, 64
