@@ -8,6 +8,7 @@ Host mirror of Dagger's tiled linear algebra for this path:
   * `ldiv(C, B)`         C \\ B staged in the factor's blocking (src/array/linalg.jl:208-246)
   * `ldiv_triangular_`   ldiv!(UpperTriangular(A), B) etc. (src/array/linalg.jl:310-335)
   * `trsm_`, `trsv_`     Dagger.trsm!/trsv! (src/array/trsm.jl:1-192)
+  * `mul_`               mul!(C, op(A), op(B), alpha, beta) (src/array/mul.jl:11-210)
 Block re-partitioning follows maybe_copy_buffered (src/array/copy.jl:3-10):
 arguments whose partitioning differs from the requested square Blocks are
 copied into a buffer of that partitioning, processed, and copied back.
@@ -412,3 +413,46 @@ def ldiv_triangular_(uplo, diag, A, B, options=None):
     _maybe_copy_buffered([(A, Blocks(bsa, bsa)), (B, Blocks(bsa, bsa))],
                          lambda pa, pb: trsm_("L", uplo, "N", diag, 1.0, pa, pb, options=options))
     return B
+
+
+def mul_(Cm, A, B, alpha=1.0, beta=0.0, transA="N", transB="N", options=None):
+    """`mul!(C, op(A), op(B), alpha, beta)` on DMatrices: Dagger's
+    generic_matmatmul! (src/array/mul.jl:11-43) and gemm_dagger!'s tile loop
+    (src/array/mul.jl:107-210: for each C tile, k-ordered tile products, beta
+    applied by the first), planned and run by the native executor.
+
+    Operands are re-blocked (maybe_copy_buffered, src/array/copy.jl:3-34) to
+    square blocks of the smallest block dimension of A and B, the size
+    _repartition_matmatmul falls back to (src/array/mul.jl:44-93); C is
+    copied back into its own blocking afterwards. For A is B with one
+    operand transposed the reference takes syrk_dagger! (upper tiles, then
+    copytri!); here the full product is formed by the same tile products,
+    which equals it for symmetric C or beta = 0."""
+    ta = "T" if transA in ("T", "C") else transA
+    tb = "T" if transB in ("T", "C") else transB
+    if ta not in ("N", "T") or tb not in ("N", "T"):
+        raise ValueError("trans must be 'N', 'T' or 'C'")
+    if A.ndim != 2 or B.ndim != 2 or Cm.ndim != 2:
+        raise ValueError("mul_ takes matrices")
+    if not (Cm.dtype == A.dtype == B.dtype):
+        raise ValueError("A, B and C must share one element type")
+    am, ak = (A.n, A.m) if ta == "T" else (A.m, A.n)
+    bk, bn = (B.n, B.m) if tb == "T" else (B.m, B.n)
+    if ak != bk or am != Cm.m or bn != Cm.n:
+        raise ValueError("shapes do not conform")
+    sz = min(*A.partitioning.blocksize, *B.partitioning.blocksize)
+    sq = Blocks(sz, sz)
+    options = options or DEFAULT_OPTIONS
+
+    def run(c, a, b):
+        tc, tx, ty = c.tiled(), a.tiled(), b.tiled()
+        o = options.c()
+        call("dm_gemm_tiled", ta.encode(), tb.encode(), float(alpha), C.byref(tx), C.byref(ty),
+             float(beta), C.byref(tc), C.byref(o), None)
+        call("dm_stream_sync", None)
+
+    if A is B:
+        _maybe_copy_buffered([(Cm, sq), (A, sq)], lambda c, a: run(c, a, a))
+    else:
+        _maybe_copy_buffered([(Cm, sq), (A, sq), (B, sq)], run)
+    return Cm

@@ -453,6 +453,32 @@ def trsv_region(uplo, trans, diag, TA, TB):
     return R
 
 
+def gemm_region(ta, tb, alpha, TA, TB, beta, TC):
+    """gemm_dagger!'s tile loop (src/array/mul.jl:107-210): for each C tile,
+    the k-ordered tile products, beta applied by the first."""
+    Ac, Bc, Cc = TA.chunks, TB.chunks, TC.chunks
+    K = TA.mt if ta == "T" else TA.nt
+    R = Region()
+    for j in range(TC.nt):
+        for i in range(TC.mt):
+            for k in range(K):
+                a = Ac[k, i] if ta == "T" else Ac[i, k]
+                b = Bc[j, k] if tb == "T" else Bc[k, j]
+                R.spawn("gemm", tile_gemm, (ta, tb, alpha, a, b, beta if k == 0 else 1.0, Cc[i, j]),
+                        [(a, IN), (b, IN), (Cc[i, j], INOUT)])
+    return R
+
+
+def gemm(ta, tb, alpha, A, B, beta, Cm, nb):
+    """mul!(C, op(A), op(B), alpha, beta) on matrices blocked nb x nb
+    (src/array/mul.jl:11-43 then gemm_dagger!); returns C."""
+    TA = TiledMatrix(np.array(A, copy=True), nb, nb)
+    TB = TiledMatrix(np.array(B, copy=True), nb, nb)
+    TC = TiledMatrix(np.array(Cm, copy=True), nb, nb)
+    run_sequential(gemm_region(ta, tb, alpha, TA, TB, beta, TC))
+    return TC.collect()
+
+
 def trsv(uplo, trans, diag, A, b, nb):
     """Dagger.trsv!(uplo, trans, diag, 1, A, b) (src/array/trsm.jl:1-57) on
     A tiled nb x nb and b tiled nb; returns x."""

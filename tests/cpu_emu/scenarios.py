@@ -443,6 +443,12 @@ def quick():
         DBm = dm.DArray.from_array(Bm, dm.Blocks(20, 2))
         dm.ldiv_triangular_(uplo, "U", DT, DBm)
         check(relerr(DBm.collect(), np.linalg.solve(Tu, Bm)) < 1e-12, ("trsm unit", uplo))
+    # mul! on DMatrices of unequal blockings (re-blocked to the smallest)
+    Am, Bm, Cm = (rng.standard_normal(s) for s in ((50, 37), (37, 29), (50, 29)))
+    DCm = dm.DArray.from_array(Cm, dm.Blocks(24, 24))
+    dm.mul_(DCm, dm.DArray.from_array(Am, dm.Blocks(16, 16)),
+            dm.DArray.from_array(Bm, dm.Blocks(20, 20)), 2.0, -1.0)
+    check(relerr(DCm.collect(), 2.0 * Am @ Bm - Cm) < 1e-13, "mul_")
     for ta, tb, (m, n, k) in [("T", "N", (70, 40, 33)), ("N", "T", (5, 9, 3))]:
         A = rng.standard_normal((k, m) if ta == "T" else (m, k))
         B = rng.standard_normal((n, k) if tb == "T" else (k, n))

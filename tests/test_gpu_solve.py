@@ -100,3 +100,36 @@ def test_triangular_ldiv_reblocks_rhs(dm):
     dm.ldiv_triangular_("L", "N", DA, DB)
     assert DB.partitioning == dm.Blocks(77, 4)
     assert rel(DB.collect(), O.trsm("L", "L", "N", "N", 1.0, A, B, 128)) <= 1e-12
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k,bs", [(700, 450, 380, (128, 128, 128)),
+                                      (512, 512, 1024, (256, 256, 256)),
+                                      (300, 200, 100, (96, 64, 128))])
+def test_mul_matches_oracle(dm, ta, tb, m, n, k, bs):
+    # bs = (A's, B's, C's block size); unequal sizes are re-blocked to the
+    # smallest of A's and B's, C is copied back into its own blocking
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    Cm = rng.standard_normal((m, n))
+    DA = dm.DArray.from_array(A, dm.Blocks(bs[0], bs[0]))
+    DB = dm.DArray.from_array(B, dm.Blocks(bs[1], bs[1]))
+    DC = dm.DArray.from_array(Cm, dm.Blocks(bs[2], bs[2]))
+    dm.mul_(DC, DA, DB, -0.75, 0.5, ta, tb)
+    assert DC.partitioning == dm.Blocks(bs[2], bs[2])
+    ref = O.gemm(ta, tb, -0.75, A, B, 0.5, Cm, min(bs[:2]))
+    assert rel(DC.collect(), ref) <= 1e-13
+
+
+def test_mul_same_operand_and_errors(dm):
+    # A * A^T with A passed twice (the syrk_dagger! case of the reference)
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((400, 300))
+    DA = dm.DArray.from_array(A, dm.Blocks(128, 128))
+    DC = dm.zeros(dm.Blocks(128, 128), np.float64, 400, 400)
+    dm.mul_(DC, DA, DA, 1.0, 0.0, "N", "T")
+    got = DC.collect()
+    assert rel(got, A @ A.T) <= 1e-13
+    with pytest.raises(ValueError):
+        dm.mul_(DC, DA, DA, 1.0, 0.0, "N", "N")

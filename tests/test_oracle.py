@@ -147,6 +147,21 @@ def test_trsm_all_branches(side, uplo, trans, diag):
     assert np.allclose(X, ref, rtol=1e-10, atol=1e-11)
 
 
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_gemm_tile_loop(ta, tb):
+    # ragged tiles on every dimension (m=70, n=45, k=38 over nb=16); beta
+    # applied once, by the first k-step
+    m, n, k, nb = 70, 45, 38, 16
+    rng = np.random.default_rng(sum(map(ord, ta + tb)))
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    Cm = rng.standard_normal((m, n))
+    got = O.gemm(ta, tb, -0.5, A, B, 2.0, Cm, nb)
+    ref = 2.0 * Cm - 0.5 * (A.T if ta == "T" else A) @ (B.T if tb == "T" else B)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("diag", ["N", "U"])
