@@ -252,6 +252,12 @@ namespace dm {
 constexpr int kSumsqThreads = 256;
 constexpr int kSumsqMaxPerTile = 8;
 
+// sixteen bytes of one tile, read by one vector load
+template <typename T>
+struct alignas(16) Pack16 {
+  T v[16 / sizeof(T)];
+};
+
 template <typename T>
 __global__ __launch_bounds__(kSumsqThreads) void tiled_sumsq_kernel(
     const T* __restrict__ base, int64_t m, int64_t n, int mb, int nb, int part, int bpt,
@@ -271,26 +277,45 @@ __global__ __launch_bounds__(kSumsqThreads) void tiled_sumsq_kernel(
   double acc = 0.0;
   bool bad = false;
   if (!skip && whole) {
-    // four independent loads in flight per thread, four partial sums
-    // combined in a fixed order
+    // 16-byte loads (two FP64 or four FP32 elements) while the tile start is
+    // 16-byte aligned, two in flight per thread, then the elements the
+    // packs leave over one at a time; two partial sums in a fixed order
+    constexpr int kPer = (int)(16 / sizeof(T));
     const int count = h * w, stride = bpt * kSumsqThreads;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int e = sub * kSumsqThreads + threadIdx.x;
-    for (; e + 3 * stride < count; e += 4 * stride) {
-      const double x0 = (double)t[e], x1 = (double)t[e + stride];
-      const double x2 = (double)t[e + 2 * stride], x3 = (double)t[e + 3 * stride];
-      bad = bad || !isfinite(x0) || !isfinite(x1) || !isfinite(x2) || !isfinite(x3);
-      a0 += x0 * x0;
-      a1 += x1 * x1;
-      a2 += x2 * x2;
-      a3 += x3 * x3;
+    const int first = sub * kSumsqThreads + threadIdx.x;
+    double a0 = 0.0, a1 = 0.0;
+    int tail = 0;
+    if ((reinterpret_cast<uintptr_t>(t) & 15) == 0) {
+      const Pack16<T>* __restrict__ tv = reinterpret_cast<const Pack16<T>*>(t);
+      const int packs = count / kPer;
+      int e = first;
+      for (; e + stride < packs; e += 2 * stride) {
+        const Pack16<T> p0 = tv[e], p1 = tv[e + stride];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const double x0 = (double)p0.v[q], x1 = (double)p1.v[q];
+          bad = bad || !isfinite(x0) || !isfinite(x1);
+          a0 += x0 * x0;
+          a1 += x1 * x1;
+        }
+      }
+      for (; e < packs; e += stride) {
+        const Pack16<T> p0 = tv[e];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const double x0 = (double)p0.v[q];
+          bad = bad || !isfinite(x0);
+          a0 += x0 * x0;
+        }
+      }
+      tail = packs * kPer;
     }
-    for (; e < count; e += stride) {
+    for (int e = tail + first; e < count; e += stride) {
       const double x = (double)t[e];
       bad = bad || !isfinite(x);
       a0 += x * x;
     }
-    acc = (a0 + a1) + (a2 + a3);
+    acc = a0 + a1;
   } else if (!skip) {
     const int count = h * w;
     for (int e = sub * kSumsqThreads + threadIdx.x; e < count; e += bpt * kSumsqThreads) {
