@@ -16,7 +16,7 @@ from ._lib import DMError, load as load_library
 from .darray import Blocks, DArray, DeviceBuffer, DeviceInt, Tile, zeros, randn, spd_generate
 from .datadeps import (In, Out, InOut, BLAS, LAPACK, potrf_checked, spawn, spawn_datadeps,
                        PlanOptions, SEQUENTIAL, Dag)
-from .linalg import (Cholesky, PosDefException, cholesky, cholesky_, ldiv_, ldiv, ldiv_triangular_, mul_,
+from .linalg import (Cholesky, PosDefException, cholesky, cholesky_, ldiv_, ldiv, ldiv_triangular_, mul_, mul,
                      trsm_, trsv_,
                      ishermitian, norm2, chkfinite, DEFAULT_OPTIONS)
 
@@ -24,7 +24,7 @@ __all__ = [
     "Blocks", "DArray", "DeviceBuffer", "DeviceInt", "Tile", "zeros", "randn", "spd_generate",
     "In", "Out", "InOut", "BLAS", "LAPACK", "potrf_checked", "spawn", "spawn_datadeps",
     "PlanOptions", "SEQUENTIAL", "Dag", "Cholesky", "PosDefException", "cholesky",
-    "cholesky_", "ldiv_", "ldiv", "ldiv_triangular_", "mul_", "trsm_", "trsv_", "ishermitian", "norm2", "chkfinite",
+    "cholesky_", "ldiv_", "ldiv", "ldiv_triangular_", "mul_", "mul", "trsm_", "trsv_", "ishermitian", "norm2", "chkfinite",
     "DEFAULT_OPTIONS", "DMError",
     "load_library",
     "init",

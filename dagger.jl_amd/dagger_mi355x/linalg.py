@@ -8,7 +8,8 @@ Host mirror of Dagger's tiled linear algebra for this path:
   * `ldiv(C, B)`         C \\ B staged in the factor's blocking (src/array/linalg.jl:208-246)
   * `ldiv_triangular_`   ldiv!(UpperTriangular(A), B) etc. (src/array/linalg.jl:310-335)
   * `trsm_`, `trsv_`     Dagger.trsm!/trsv! (src/array/trsm.jl:1-192)
-  * `mul_`               mul!(C, op(A), op(B), alpha, beta) (src/array/mul.jl:11-210)
+  * `mul_`, `mul`        mul!(C, op(A), op(B), alpha, beta) and op(A) * op(B)
+                         (src/array/mul.jl:11-210)
 Block re-partitioning follows maybe_copy_buffered (src/array/copy.jl:3-10):
 arguments whose partitioning differs from the requested square Blocks are
 copied into a buffer of that partitioning, processed, and copied back.
@@ -27,7 +28,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call
-from .darray import DArray, Blocks, DeviceBuffer, DeviceInt
+from .darray import DArray, Blocks, DeviceBuffer, DeviceInt, zeros
 from .datadeps import (spawn, spawn_datadeps, In, Out, InOut, BLAS, potrf_checked,
                        PlanOptions)
 
@@ -456,3 +457,14 @@ def mul_(Cm, A, B, alpha=1.0, beta=0.0, transA="N", transB="N", options=None):
     else:
         _maybe_copy_buffered([(Cm, sq), (A, sq), (B, sq)], run)
     return Cm
+
+
+def mul(A, B, transA="N", transB="N", options=None):
+    """`op(A) * op(B)` on DMatrices (`A * B`, `A' * B`, ... of
+    src/array/mul.jl): a new C in HBM, square blocks of the smallest block
+    dimension of A and B, filled by `mul_` with alpha = 1, beta = 0."""
+    m = A.n if transA in ("T", "C") else A.m
+    n = B.m if transB in ("T", "C") else B.n
+    sz = min(*A.partitioning.blocksize, *B.partitioning.blocksize)
+    Cm = zeros(Blocks(sz, sz), A.dtype, m, n)
+    return mul_(Cm, A, B, 1.0, 0.0, transA, transB, options)

@@ -133,3 +133,31 @@ def test_mul_same_operand_and_errors(dm):
     assert rel(got, A @ A.T) <= 1e-13
     with pytest.raises(ValueError):
         dm.mul_(DC, DA, DA, 1.0, 0.0, "N", "N")
+
+
+def test_mul_with_without_transpose(dm):
+    # the reference's matmul testset: X'X, XX', XX on Blocks(10, 20), 40x40,
+    # ||collect(Xk) - ref|| < 1e-12
+    X = np.random.default_rng(40).random((40, 40))
+    X1 = dm.DArray.from_array(X, dm.Blocks(10, 20))
+    assert np.linalg.norm(dm.mul(X1, X1, "T", "N").collect() - X.T @ X) < 1e-12
+    assert np.linalg.norm(dm.mul(X1, X1, "N", "T").collect() - X @ X.T) < 1e-12
+    assert np.linalg.norm(dm.mul(X1, X1).collect() - X @ X) < 1e-12
+    assert np.array_equal(X1.collect(), X)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_mul_float32(dm, ta, tb):
+    # FP32 tile products (sum of k = 600 products): relative 1e-5 in norm
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((600, 520)).astype(np.float32)
+    B = rng.standard_normal((600, 480) if ta == "T" else (520, 480)).astype(np.float32)
+    if tb == "T":
+        B = np.ascontiguousarray(B.T)
+    DA = dm.DArray.from_array(A, dm.Blocks(128, 128))
+    DB = dm.DArray.from_array(B, dm.Blocks(128, 128))
+    got = dm.mul(DA, DB, ta, tb).collect()
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = (A64.T if ta == "T" else A64) @ (B64.T if tb == "T" else B64)
+    assert got.dtype == np.float32
+    assert rel(got.astype(np.float64), ref) <= 1e-5
