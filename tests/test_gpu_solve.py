@@ -21,6 +21,12 @@ def rel(x, ref):
     return np.linalg.norm(x - ref) / np.linalg.norm(ref)
 
 
+def isapprox(x, y, rtol=np.sqrt(EPS)):
+    """Julia's `x ≈ y` for arrays: norm(x - y) <= rtol * max(norm(x), norm(y)),
+    rtol defaulting to sqrt(eps)."""
+    return np.linalg.norm(x - y) <= rtol * max(np.linalg.norm(x), np.linalg.norm(y))
+
+
 def factor(dm, n, nb, uplo, seed):
     A = gen.spd(n, seed)
     DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
@@ -161,3 +167,27 @@ def test_mul_float32(dm, ta, tb):
     ref = (A64.T if ta == "T" else A64) @ (B64.T if tb == "T" else B64)
     assert got.dtype == np.float32
     assert rel(got.astype(np.float64), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("bs_i", [128, 64])
+def test_reference_solve_testset_cholesky(dm, bs_i):
+    # test/array/linalg/solve.jl's Cholesky part: N = 128, Blocks(bs_i, 128)
+    # for A and B (non-square blocks for bs_i = 64), A = A'A + I from rand,
+    # ldiv! with a DMatrix (`≈`) and a DVector (`≈ rtol=1e-10`) right-hand side
+    N = 128
+    rng = np.random.default_rng(bs_i)
+    A = rng.random((N, N))
+    A = A.T @ A + np.eye(N)
+    B = rng.random((N, N))
+    b = rng.random(N)
+    DA = dm.DArray.from_array(A, dm.Blocks(bs_i, N))
+    F = dm.cholesky(DA)
+    assert np.array_equal(DA.collect(), A)
+    DB = dm.DArray.from_array(B, dm.Blocks(bs_i, N))
+    dm.ldiv_(F, DB)
+    assert DB.partitioning == dm.Blocks(bs_i, N)
+    X = DB.collect()
+    assert isapprox(X, np.linalg.solve(A, B))
+    Db = dm.DArray.from_array(b, dm.Blocks(bs_i))
+    dm.ldiv_(F, Db)
+    assert isapprox(Db.collect(), np.linalg.solve(A, b), rtol=1e-10)
