@@ -191,3 +191,32 @@ def test_reference_solve_testset_cholesky(dm, bs_i):
     Db = dm.DArray.from_array(b, dm.Blocks(bs_i))
     dm.ldiv_(F, Db)
     assert isapprox(Db.collect(), np.linalg.solve(A, b), rtol=1e-10)
+
+
+@pytest.mark.parametrize("T,rtol", [(np.float32, np.sqrt(np.finfo(np.float32).eps)),
+                                    (np.float64, np.sqrt(EPS))])
+def test_reference_cholesky_testset(dm, T, rtol):
+    # test/array/linalg/cholesky.jl for the real types: rand(Blocks(4,4), T,
+    # 32, 32) is not Hermitian; A*A' + n I on Blocks(32, 32) is; out of place
+    # leaves A; in place writes the factor into DA; L and U `≈` the dense
+    # factor; a negative A[1,1] keeps A Hermitian and fails the factorization
+    rng = np.random.default_rng(128)
+    D = dm.DArray.from_array(rng.random((32, 32)).astype(T), dm.Blocks(4, 4))
+    assert not dm.ishermitian(D)
+    A = rng.random((128, 128)).astype(T)
+    A = (A @ A.T).astype(T)
+    A[np.diag_indices(128)] += 128
+    DA = dm.DArray.from_array(A, dm.Blocks(32, 32))
+    assert dm.ishermitian(DA)
+    U = np.linalg.cholesky(A.astype(np.float64)).T
+    F = dm.cholesky(DA)
+    assert np.array_equal(DA.collect(), A)
+    assert isapprox(F.U, U, rtol) and isapprox(F.L, U.T, rtol)
+    F2 = dm.cholesky_(DA)
+    assert isapprox(np.triu(DA.collect()), U, rtol) and isapprox(F2.U, U, rtol)
+    A[0, 0] = -100
+    DN = dm.DArray.from_array(A, dm.Blocks(32, 32))
+    assert dm.ishermitian(DN)
+    with pytest.raises(dm.PosDefException) as e:
+        dm.cholesky(DN)
+    assert e.value.info == 1
