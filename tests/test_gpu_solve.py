@@ -220,3 +220,34 @@ def test_reference_cholesky_testset(dm, T, rtol):
     with pytest.raises(dm.PosDefException) as e:
         dm.cholesky(DN)
     assert e.value.info == 1
+
+
+def _gemm_cases():
+    sizes, parts = [], []
+    for r, c in ((4, 4), (12, 12)):
+        sizes += [((r, c), (c, r)), ((r // 2, c), (c, r // 2)), ((r, c // 2), (c // 2, r))]
+        parts += [((r, c), (c, r)), ((r // 2, c), (c, r // 2)), ((r, c // 2), (c // 2, r))]
+    return [(sa, sb, pa, pb) for sa, sb in sizes for pa, pb in parts]
+
+
+@pytest.mark.parametrize("szA,szB,partA,partB", _gemm_cases())
+def test_reference_gemm_testset(dm, szA, szB, partA, partB):
+    # test/array/linalg/matmul.jl's test_gemm! for Float64: out-of-place and
+    # in-place products with every transpose the testset uses, and the syrk
+    # forms (A A', A' A) for square A; every result `≈` the dense product
+    rng = np.random.default_rng(sum(szA) * 31 + sum(partA))
+    A, B = rng.random(szA), rng.random(szB)
+    DA = dm.DArray.from_array(A, dm.Blocks(*partA))
+    DB = dm.DArray.from_array(B, dm.Blocks(*partB))
+    partC = dm.Blocks(partA[0], partB[1])
+    sq = szA == szB
+    outs = [("N", "N", DA, DB, A @ B), ("T", "T", DA, DB, A.T @ B.T)]
+    if sq:
+        outs += [("N", "T", DA, DB, A @ B.T), ("T", "N", DA, DB, A.T @ B),
+                 ("N", "T", DA, DA, A @ A.T), ("T", "N", DA, DA, A.T @ A)]
+    for ta, tb, X, Y, ref in outs:
+        assert isapprox(dm.mul(X, Y, ta, tb).collect(), ref), ("out of place", ta, tb)
+        DC = dm.DArray.from_array(np.zeros(ref.shape), partC)
+        dm.mul_(DC, X, Y, 1.0, 0.0, ta, tb)
+        assert DC.partitioning == partC
+        assert isapprox(DC.collect(), ref), ("in place", ta, tb)
