@@ -433,8 +433,10 @@ def mul_(Cm, A, B, alpha=1.0, beta=0.0, transA="N", transB="N", options=None):
     tb = "T" if transB in ("T", "C") else transB
     if ta not in ("N", "T") or tb not in ("N", "T"):
         raise ValueError("trans must be 'N', 'T' or 'C'")
+    if A.ndim == 2 and B.ndim == 1 and Cm.ndim == 1:
+        return _mul_vec(Cm, A, B, alpha, beta, ta, options)
     if A.ndim != 2 or B.ndim != 2 or Cm.ndim != 2:
-        raise ValueError("mul_ takes matrices")
+        raise ValueError("mul_ takes three matrices, or C and B vectors")
     if not (Cm.dtype == A.dtype == B.dtype):
         raise ValueError("A, B and C must share one element type")
     am, ak = (A.n, A.m) if ta == "T" else (A.m, A.n)
@@ -459,11 +461,52 @@ def mul_(Cm, A, B, alpha=1.0, beta=0.0, transA="N", transB="N", options=None):
     return Cm
 
 
+def _mul_vec(y, A, x, alpha, beta, ta, options):
+    """`mul!(y, op(A), x, alpha, beta)` for DVectors x and y: Dagger's
+    generic_matvecmul! (src/array/mul.jl:437-449) with its repartitioning
+    (_repartition_matvecmul, :450-472: x and A's inner blocking fall back to
+    the smallest block dimension when they differ, y takes op(A)'s row
+    blocking) and gemv_dagger!'s loop (:474-541: for each y block, the
+    k-ordered tile GEMVs, beta applied by the first), run by the native
+    executor's FP64 GEMV tile kernel."""
+    if not (y.dtype == A.dtype == x.dtype == np.float64):
+        raise NotImplementedError("matrix-vector products run in FP64")
+    am, ak = (A.n, A.m) if ta == "T" else (A.m, A.n)
+    if ak != x.m or am != y.m:
+        raise ValueError("shapes do not conform")
+    pa = A.partitioning.blocksize
+    pb = x.partitioning.blocksize[0]
+    dim_a, dim_inner = (pa[1], pa[0]) if ta == "T" else (pa[0], pa[1])
+    part_a, part_x = pa, pb
+    if dim_inner != pb:
+        sz = min(pa[0], pa[1], pb)
+        part_a = (sz, pa[1]) if ta == "T" else (pa[0], sz)
+        part_x = sz
+    options = options or PlanOptions(num_streams=1, batch=False, lookahead=0)
+
+    def run(c, a, b):
+        Ac, Bc, Cc = a.chunks, b.chunks, c.chunks
+        K = a.mt if ta == "T" else a.nt
+        with spawn_datadeps(options=options):
+            for m in range(len(Cc)):
+                for k in range(K):
+                    tile = Ac[k, m] if ta == "T" else Ac[m, k]
+                    spawn(BLAS.gemv, ta, float(alpha), In(tile), In(Bc[k]),
+                          float(beta) if k == 0 else 1.0, InOut(Cc[m]))
+
+    _maybe_copy_buffered([(y, Blocks(dim_a)), (A, Blocks(*part_a)), (x, Blocks(part_x))], run)
+    return y
+
+
 def mul(A, B, transA="N", transB="N", options=None):
     """`op(A) * op(B)` on DMatrices (`A * B`, `A' * B`, ... of
     src/array/mul.jl): a new C in HBM, square blocks of the smallest block
     dimension of A and B, filled by `mul_` with alpha = 1, beta = 0."""
     m = A.n if transA in ("T", "C") else A.m
+    if B.ndim == 1:
+        pa = A.partitioning.blocksize
+        y = zeros(Blocks(pa[1] if transA in ("T", "C") else pa[0]), A.dtype, m)
+        return _mul_vec(y, A, B, 1.0, 0.0, "T" if transA in ("T", "C") else "N", options)
     n = B.m if transB in ("T", "C") else B.n
     sz = min(*A.partitioning.blocksize, *B.partitioning.blocksize)
     Cm = zeros(Blocks(sz, sz), A.dtype, m, n)

@@ -449,6 +449,15 @@ def quick():
     dm.mul_(DCm, dm.DArray.from_array(Am, dm.Blocks(16, 16)),
             dm.DArray.from_array(Bm, dm.Blocks(20, 20)), 2.0, -1.0)
     check(relerr(DCm.collect(), 2.0 * Am @ Bm - Cm) < 1e-13, "mul_")
+    # matrix-vector mul! (GEMV tile tasks), x blocked unlike A's columns
+    Av, xv, yv = rng.standard_normal((45, 38)), rng.standard_normal(38), rng.standard_normal(45)
+    Dy = dm.DArray.from_array(yv, dm.Blocks(16))
+    dm.mul_(Dy, dm.DArray.from_array(Av, dm.Blocks(16, 12)), dm.DArray.from_array(xv, dm.Blocks(10)),
+            1.5, 0.5)
+    check(relerr(Dy.collect(), 1.5 * Av @ xv + 0.5 * yv) < 1e-13, "mul_ vector")
+    Dz = dm.mul(dm.DArray.from_array(Av, dm.Blocks(16, 12)), dm.DArray.from_array(yv, dm.Blocks(9)),
+                "T")
+    check(relerr(Dz.collect(), Av.T @ yv) < 1e-13, "mul vector T")
     for ta, tb, (m, n, k) in [("T", "N", (70, 40, 33)), ("N", "T", (5, 9, 3))]:
         A = rng.standard_normal((k, m) if ta == "T" else (m, k))
         B = rng.standard_normal((n, k) if tb == "T" else (k, n))

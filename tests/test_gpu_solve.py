@@ -251,3 +251,26 @@ def test_reference_gemm_testset(dm, szA, szB, partA, partB):
         dm.mul_(DC, X, Y, 1.0, 0.0, ta, tb)
         assert DC.partitioning == partC
         assert isapprox(DC.collect(), ref), ("in place", ta, tb)
+
+
+@pytest.mark.parametrize("szA,partA,pb", [((4, 4), (4, 4), 4), ((2, 4), (2, 4), 4),
+                                          ((4, 2), (4, 2), 2), ((12, 12), (6, 12), 12),
+                                          ((12, 6), (12, 6), 6), ((12, 12), (12, 6), 4),
+                                          ((700, 500), (128, 96), 128)])
+def test_matvec(dm, szA, partA, pb):
+    # matmul.jl's test_gemv! shapes: op(A) x out of place and in place, x
+    # blocked like or unlike A's inner blocking (re-blocked to the smallest)
+    rng = np.random.default_rng(sum(szA) + pb)
+    A = rng.random(szA)
+    DA = dm.DArray.from_array(A, dm.Blocks(*partA))
+    for ta, n_in in (("N", szA[1]), ("T", szA[0])):
+        x = rng.random(n_in)
+        Dx = dm.DArray.from_array(x, dm.Blocks(pb))
+        ref = (A.T if ta == "T" else A) @ x
+        assert isapprox(dm.mul(DA, Dx, ta).collect(), ref), ("out of place", ta)
+        y0 = rng.random(ref.shape[0])
+        Dy = dm.DArray.from_array(y0, dm.Blocks(5))
+        dm.mul_(Dy, DA, Dx, 2.0, -1.0, ta)
+        assert Dy.partitioning == dm.Blocks(5)
+        assert isapprox(Dy.collect(), 2.0 * ref - y0), ("in place", ta)
+        assert np.array_equal(Dx.collect(), x)
