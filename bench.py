@@ -34,6 +34,9 @@ sys.path.insert(0, os.path.join(ROOT, "dagger.jl_amd"))
 
 METRIC = "Cholesky TFLOP/s N=32k FP64 vs mixed+IR @1e-13 bwd err; served solves/s + p99"
 FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix (vendor figure)
+# accuracy gate of the timed FP64 solve (max column backward error); the
+# generator's kappa is about 5, so a correct factor and solve land near 1e-16
+BERR_LIMIT = 1e-12
 
 
 def log(*a):
@@ -91,6 +94,38 @@ def norm2_leg(dm, A, timer, reps=5):
     return {"ms": round(ms, 3), "gb_per_s": round(gbs, 1), "peak_gb_per_s": 8000.0,
             "frac": round(gbs / 8000.0, 3), "bytes": A.m * A.n * 8, "norm2": float(val[0]),
             "finite": finite}
+
+
+def device_backward_error(dm, A, X, B):
+    """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) of a
+    solve, on the device: R = B - A X by the tiled GEMM (the streaming
+    block-column kernel for up to 64 columns), then the FP64 infinity norms."""
+    from dagger_mi355x.darray import DeviceBuffer
+    call = dm._lib.call
+    R = B.copy()
+    ta, tx, tr = A.tiled(), X.tiled(), R.tiled()
+    o = dm.PlanOptions(num_streams=1, batch=True, lookahead=0).c()
+    call("dm_gemm_tiled", b"N", b"N", -1.0, C.byref(ta), C.byref(tx), 1.0, C.byref(tr),
+         C.byref(o), None)
+    work = DeviceBuffer(8 * A.m * (1 + -(-A.n // A.nb)))
+    nrm = DeviceBuffer(8 * max(X.n, 1))
+
+    def colnorms(D):
+        t = D.tiled()
+        call("dm_tiled_col_norm_inf", C.byref(t), C.c_void_p(nrm.ptr), None)
+        out = np.empty(D.n)
+        call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(nrm.ptr),
+             C.c_size_t(8 * D.n), None)
+        return out
+
+    call("dm_tiled_norm_inf", C.byref(ta), C.c_void_p(nrm.ptr), C.c_void_p(work.ptr), None)
+    na = np.zeros(1)
+    call("dm_memcpy_d2h", na.ctypes.data_as(C.c_void_p), C.c_void_p(nrm.ptr), C.c_size_t(8), None)
+    nr, nx, nbv = colnorms(R), colnorms(X), colnorms(B)
+    R.unsafe_free()
+    work.free()
+    nrm.free()
+    return float(np.max(nr / (na[0] * nx + nbv)))
 
 
 def restore(call, dst, src):
@@ -375,6 +410,12 @@ def main():
             f"ldiv {s_ms:.2f} ms")
     call("dm_device_sync")
     wall = time.perf_counter() - t0
+    # the last timed solve: backward error against the pristine A and B
+    berr = device_backward_error(dm, A0, DB, B0)
+    log(f"backward error of the timed cholesky!+ldiv!: {berr:.3e}")
+    if not berr <= BERR_LIMIT:
+        raise SystemExit(f"timed solve failed its accuracy check: backward error {berr:.3e} "
+                         f"> {BERR_LIMIT:g}")
     chol_ms = sum(chol) / len(chol)
     solve_ms = sum(solve) / len(solve)
     value = n ** 3 / 3 / (chol_ms * 1e-3) / 1e12
@@ -426,6 +467,7 @@ def main():
         "cholesky_ms": round(chol_ms, 3),
         "ldiv_ms": round(solve_ms, 3),
         "ldiv_tflops": round(2.0 * n * n * nrhs / (solve_ms * 1e-3) / 1e12, 3),
+        "backward_error": berr,
         "first_call_s": round(first_s, 3),
         "roofline": roof,
         "groups_by_op": by_op,

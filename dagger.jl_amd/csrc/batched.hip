@@ -4,7 +4,7 @@
 // size (A_m n_m x n_m column-major with ld = n_m, B_m n_m x nrhs_m), each
 // factorized A = U^T U (the default cholesky! uplo) and solved in place.
 //
-// One launch of persistent workgroups (256 threads = 4 waves).  A
+// One launch of persistent workgroups (512 threads = 8 waves, one per CU).  A
 // workgroup pops the next system from a device work queue (an atomic
 // counter over a largest-first order, so long factorizations start early)
 // and walks that system's blocked right-looking Cholesky with panel width P
@@ -348,36 +348,16 @@ extern "C" int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_b
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
   // workgroup size 512 threads (8 waves), one workgroup per CU; measured on
   // 4096 systems n in [32, 2048]: 170 ms against 198 ms for 2 x 256 per CU.
-  // Environment DM_BATCHED_THREADS (256, 512, 1024) and
-  // DM_BATCHED_WGS_PER_CU (1-4) override both for tuning.
-  static const int nt = [] {
-    const char* e = std::getenv("DM_BATCHED_THREADS");
-    const int v = e ? std::atoi(e) : 512;
-    return (v == 256 || v == 1024) ? v : 512;
-  }();
-  static const int per_cu = [] {
-    const char* e = std::getenv("DM_BATCHED_WGS_PER_CU");
-    const int v = e ? std::atoi(e) : 1;
-    return v < 1 ? 1 : (v > 4 ? 4 : v);
-  }();
-  int grid = workgroups > 0 ? workgroups : per_cu * cus;
+  int grid = workgroups > 0 ? workgroups : cus;
   if (grid > count) grid = count;
-  auto go = [&](auto tag, auto ntag) {
-    typedef decltype(tag) T;
-    constexpr int NT = decltype(ntag)::value;
-    hipLaunchKernelGGL((batched_posv_kernel<T, NT>), dim3(grid), dim3(NT), 0, st, (T*)A_base,
-                       (T*)B_base, (const BatchItem*)(base + 256), (const int*)(base + 256 + ib),
-                       count, (int*)base, info_dev, factor_only);
-  };
-  if (dtype == DM_F64) {
-    if (nt == 1024) go(double(), std::integral_constant<int, 1024>{});
-    else if (nt == 512) go(double(), std::integral_constant<int, 512>{});
-    else go(double(), std::integral_constant<int, 256>{});
-  } else {
-    if (nt == 1024) go(float(), std::integral_constant<int, 1024>{});
-    else if (nt == 512) go(float(), std::integral_constant<int, 512>{});
-    else go(float(), std::integral_constant<int, 256>{});
-  }
+  if (dtype == DM_F64)
+    hipLaunchKernelGGL((batched_posv_kernel<double, 512>), dim3(grid), dim3(512), 0, st,
+                       (double*)A_base, (double*)B_base, (const BatchItem*)(base + 256),
+                       (const int*)(base + 256 + ib), count, (int*)base, info_dev, factor_only);
+  else
+    hipLaunchKernelGGL((batched_posv_kernel<float, 512>), dim3(grid), dim3(512), 0, st,
+                       (float*)A_base, (float*)B_base, (const BatchItem*)(base + 256),
+                       (const int*)(base + 256 + ib), count, (int*)base, info_dev, factor_only);
   DM_LAUNCH_CHECK("batched_posv_kernel");
   DM_HIP(hipFreeAsync(mem, st));
   return 0;

@@ -93,8 +93,6 @@ struct Dag {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_nodes = 0;
-  int cus = 0;       // compute units of the device
-  int crit_cus = 0;  // CUs kept free of bulk launches for the critical stream
   bool capturing = false;
   // profiling (opts.profile, eager runs): events around each group
   std::vector<hipEvent_t> prof_start, prof_stop;

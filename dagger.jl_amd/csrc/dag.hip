@@ -266,40 +266,13 @@ int Dag::plan(const dm_plan_opts& o) {
 }
 
 int Dag::create_streams(bool eager) {
-  int dev = 0;
-  DM_HIP(hipGetDevice(&dev));
-  DM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  // CUs kept free of bulk launches (environment DM_CRIT_CUS; none by
-  // default: measured at N = 32768, reserving 16 or 32 CUs cost more bulk
-  // throughput than it saved on the critical path)
-  crit_cus = 0;
-  if (const char* e = std::getenv("DM_CRIT_CUS")) crit_cus = std::atoi(e);
-  // Environment DM_CU_RESERVE=R (tuning): with look-ahead, the bulk streams
-  // are created with a CU mask that leaves R CUs (R/8 per XCD, spread over
-  // the mask's XCD-interleaved bit order) to the critical stream.
-  std::vector<uint32_t> mask;
-  int reserve = 0;
-  if (const char* e = std::getenv("DM_CU_RESERVE")) reserve = std::atoi(e);
-  if (reserve > 0 && reserve < cus && opts.lookahead > 0 && opts.num_streams >= 2) {
-    mask.assign((size_t)(cus + 31) / 32, 0u);
-    for (int i = 0; i < cus; ++i) mask[i / 32] |= 1u << (i % 32);
-    for (int k = 0; k < reserve; ++k) {
-      const int i = cus - 1 - k;  // the last bits: k % 8 walks the 8 XCDs
-      mask[i / 32] &= ~(1u << (i % 32));
-    }
-  }
   // An eagerly issued plan runs on the executor streams of its caller's
   // stream (shared by all plans issued from it); a plan captured into a
-  // hipGraph, or one with CU-masked streams, owns its streams and the
-  // origin stream the capture runs on.
-  if (!eager || !mask.empty()) {
+  // hipGraph owns its streams and the origin stream the capture runs on.
+  if (!eager) {
     streams.resize(opts.num_streams);
-    for (size_t i = 0; i < streams.size(); ++i) {
-      if (i > 0 && !mask.empty())
-        DM_HIP(hipExtStreamCreateWithCUMask(&streams[i], (uint32_t)mask.size(), mask.data()));
-      else
-        DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
-    }
+    for (size_t i = 0; i < streams.size(); ++i)
+      DM_HIP(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
     DM_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   }
   events.resize(groups.size());
@@ -421,8 +394,7 @@ int Dag::build_device_tables() {
         }
         G.batched_trsm = true;
         std::vector<TrsmStep<T>> steps;
-        const bool crit_group = opts.lookahead > 0 && opts.num_streams >= 2 && G.crit == 0;
-        trsm_plan_steps<T>(G.shape, jobs, steps, crit_group ? trsm_crit_leaf_len() : 0);
+        trsm_plan_steps<T>(G.shape, jobs, steps);
         for (TrsmStep<T>& st : steps) {
           Group::Step rec;
           rec.leaf = st.leaf;
@@ -474,7 +446,7 @@ int Dag::build_device_tables() {
   task_ws_off.assign(tasks.size(), (size_t)-1);
   size_t ws_bytes = 0;
   for (Group& G : groups) {
-    const bool needs = G.op == DM_OP_POTRF || G.op == DM_OP_TRSV ||
+    const bool needs = G.op == DM_OP_TRSV ||
                        (G.op == DM_OP_TRSM && !G.batched_trsm);
     if (!needs) continue;
     const size_t per = G.dtype == DM_F64 ? trsm_ws_elems<double>() * sizeof(double)
@@ -519,17 +491,11 @@ int Dag::build_device_tables() {
 template <typename T>
 int Dag::launch_group_t(const Group& G, hipStream_t s) {
   const int* ab = abort_dev;
-  // Bulk (non-critical) launches of a look-ahead schedule leave crit_cus CUs
-  // free, so the critical stream's small launches find free CUs instead of
-  // queueing behind milliseconds of trailing update.
-  const bool cap = opts.lookahead > 0 && opts.num_streams >= 2 && G.crit != 0 && cus > 0 &&
-                   crit_cus > 0 && crit_cus < cus;
-  const long bulk_cus = cus - crit_cus;
   if (G.op == DM_OP_GEMM) {
     if (G.njobs == 0) return 0;
     GemmJob<T> dummy{};
     return gemm_launch<T>(G.ta, G.tb, G.vec, G.cfg, (const GemmJob<T>*)G.table_dev, G.njobs, dummy,
-                          G.total_wg, ab, s, cap ? bulk_cus * gemm_wgs_per_cu(G.cfg) : 0);
+                          G.total_wg, ab, s);
   }
   if (G.op == DM_OP_TRSM && G.batched_trsm) {
     for (const InvSrc& src : G.inv_src) {
@@ -543,13 +509,12 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       if (st.leaf) {
         TrsmJob<T> dummy{};
         dummy.L = st.maxL;
-        // leaf LDS: 147 KB (FP64) -> one workgroup per CU, two for FP32
         rc = trsm_launch<T>(G.shape, (const TrsmJob<T>*)st.table_dev, st.njobs, dummy, st.total_wg,
-                            ab, s, cap ? bulk_cus * (sizeof(T) == 8 ? 1 : 2) : 0);
+                            ab, s);
       } else {
         GemmJob<T> dummy{};
         rc = gemm_launch<T>(st.ta, st.tb, st.vec, st.cfg, (const GemmJob<T>*)st.table_dev, st.njobs,
-                            dummy, st.total_wg, ab, s, cap ? bulk_cus * gemm_wgs_per_cu(st.cfg) : 0);
+                            dummy, st.total_wg, ab, s);
       }
       if (rc) return rc;
       if (prof && st.ev1) DM_HIP(hipEventRecord(st.ev1, s));
@@ -584,9 +549,8 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
         break;
       }
       case DM_OP_GEMV:
-        if (sizeof(T) != 8) return arg_fail(0, "gemv: only FP64");
-        rc = dgemv_launch(is_trans(t.transa), t.m, t.n, t.alpha, (const double*)t.a, t.lda,
-                          (const double*)t.b, t.ldb, t.beta, (double*)t.c, t.ldc, ab, s);
+        rc = gemv_launch<T>(is_trans(t.transa), t.m, t.n, (T)t.alpha, (const T*)t.a, t.lda,
+                            (const T*)t.b, t.ldb, (T)t.beta, (T*)t.c, t.ldc, ab, s);
         break;
       case DM_OP_TRSV:
         if (t.ldc == 1)

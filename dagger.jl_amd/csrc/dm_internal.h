@@ -163,7 +163,7 @@ TrsmShape trsm_shape(char side, char uplo, char transa, char diag);
 
 constexpr int TRSM_BS = 32;     // diagonal block inverted up front
 constexpr int TRSM_LEAF = 256;  // longest solve held in LDS (longer ones recurse)
-int trsm_leaf_len();            // leaf length in use (TRSM_LEAF unless tuned down)
+int trsm_leaf_len();            // recursion leaf length in use (128)
 constexpr int TRSM_VPW = 64;    // vectors per workgroup of the leaf kernel (16 per wave)
 // Workspace (elements) for any trsm_full / potrf_full call: the inverted
 // diagonal blocks of one leaf solve.
@@ -202,7 +202,6 @@ struct TrsmStep {
 template <typename T>
 void trsm_plan_steps(const TrsmShape& sh, const std::vector<TrsmJob<T>>& jobs,
                      std::vector<TrsmStep<T>>& steps, int leaf = 0);
-int trsm_crit_leaf_len();  // leaf length for critical-stream solves
 
 // Full BLAS trsm (any size; host recursion above TRSM_LEAF, inverse blocks
 // formed per leaf in ws).
@@ -230,9 +229,11 @@ int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
                int global_offset, T* ws, hipStream_t s);
 
 // ------------------------------------------------------------ vectors
-int dgemv_launch(bool trans, int m, int n, double alpha, const double* A,
-                 int lda, const double* x, int incx, double beta, double* y,
-                 int incy, const int* abort, hipStream_t s);
+// y := alpha op(A) x + beta y (BLAS gemv; an empty inner dimension still
+// scales y by beta).
+template <typename T>
+int gemv_launch(bool trans, int m, int n, T alpha, const T* A, int lda, const T* x, int incx,
+                T beta, T* y, int incy, const int* abort, hipStream_t s);
 
 int fill_i32_launch(int* dst, int value, size_t count, hipStream_t s);
 

@@ -17,7 +17,6 @@
 // of L^T, broadcasting the current row through LDS -- no barriers).
 // Every launch first checks the abort flag (and info), so after a failed
 // pivot the remaining launches are no-ops.
-#include <cstdlib>
 #include <type_traits>
 #include "dm_internal.h"
 #include "potrf_diag.h"
@@ -106,7 +105,7 @@ int potrf_panels(bool upper, int n, T* A, int lda, int* info, int* abort, int gl
                  hipStream_t s) {
   // Launches after a failed pivot are no-ops: they watch the plan-level
   // abort flag when there is one, else info.
-  const int* chk = abort ? abort : info;
+  static_assert(PW == 64, "the fused update kernel factors 64-wide diagonal blocks");
   bool diag_done = false;  // the previous panel's fused update factored this block
   for (int j0 = 0; j0 < n; j0 += PW) {
     if (!diag_done) {
@@ -130,36 +129,12 @@ int potrf_panels(bool upper, int n, T* A, int lda, int* info, int* abort, int gl
       hipLaunchKernelGGL((potrf_row_kernel<T, false, PW>), dim3(nwg), dim3(64), 0, s, A, n, lda,
                          j0, info, abort);
     DM_LAUNCH_CHECK("potrf_row_kernel");
-    int rc;
-    if (PW == 64) {  // trailing update + next diagonal block in one launch
-      rc = potrf_update_launch<T>(upper, n, A, lda, j0, info, abort, global_offset, s);
-      if (rc) return rc;
-      diag_done = true;
-      continue;
-    }
-    if (upper) {
-      const T* A12 = A + (size_t)(j0 + kb) * lda + j0;
-      T* A22 = A + (size_t)(j0 + kb) * lda + j0 + kb;
-      rc = gemm_single<T>(true, false, rest, rest, kb, T(-1), A12, lda, A12, lda, T(1), A22, lda, 1,
-                          chk, s);
-    } else {
-      const T* A21 = A + (size_t)j0 * lda + j0 + kb;
-      T* A22 = A + (size_t)(j0 + kb) * lda + j0 + kb;
-      rc = gemm_single<T>(false, true, rest, rest, kb, T(-1), A21, lda, A21, lda, T(1), A22, lda, 2,
-                          chk, s);
-    }
+    // trailing update + next diagonal block in one launch
+    const int rc = potrf_update_launch<T>(upper, n, A, lda, j0, info, abort, global_offset, s);
     if (rc) return rc;
+    diag_done = true;
   }
   return 0;
-}
-
-int potrf_panel_width() {
-  // environment DM_POTRF_PANEL: 32 or 64 (tuning); 64 by default
-  static const int w = [] {
-    const char* e = std::getenv("DM_POTRF_PANEL");
-    return (e && std::atoi(e) == 32) ? 32 : 64;
-  }();
-  return w;
 }
 }  // namespace
 
@@ -167,8 +142,6 @@ template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort, int global_offset, T* ws,
                hipStream_t s) {
   (void)ws;
-  if (potrf_panel_width() == 32)
-    return potrf_panels<T, 32>(upper, n, A, lda, info, abort, global_offset, s);
   return potrf_panels<T, 64>(upper, n, A, lda, info, abort, global_offset, s);
 }
 
@@ -179,60 +152,73 @@ template int potrf_full<float>(bool, int, float*, int, int*, int*, int, float*, 
 // y := alpha op(A) x + beta y.  'N': a workgroup owns 64 rows, its 4 waves
 // split the columns and reduce through LDS (coalesced along rows).
 // 'T': one wave per output element (column dot product, coalesced).
-__global__ __launch_bounds__(256) void dgemv_n_kernel(int m, int n, double alpha,
-                                                      const double* __restrict__ A, int lda,
-                                                      const double* __restrict__ x, int incx,
-                                                      double beta, double* __restrict__ y,
-                                                      int incy, const int* __restrict__ abort) {
-  __shared__ double part[4][64];
+// An empty inner dimension (n = 0 for 'N', m = 0 for 'T') leaves
+// y := beta y, as BLAS does (beta == 0 writes zeros without reading y).
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_n_kernel(int m, int n, T alpha,
+                                                     const T* __restrict__ A, int lda,
+                                                     const T* __restrict__ x, int incx,
+                                                     T beta, T* __restrict__ y,
+                                                     int incy, const int* __restrict__ abort) {
+  __shared__ T part[4][64];
   if (abort && *abort) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane;
-  double acc = 0.0;
+  T acc = T(0);
   if (i < m)
     for (int j = wave; j < n; j += 4) acc += A[(size_t)j * lda + i] * x[(size_t)j * incx];
   part[wave][lane] = acc;
   __syncthreads();
   if (wave == 0 && i < m) {
-    const double v = alpha * (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-    double* py = y + (size_t)i * incy;
-    *py = (beta == 0.0) ? v : v + beta * *py;
+    const T v = alpha * (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+    T* py = y + (size_t)i * incy;
+    *py = (beta == T(0)) ? v : v + beta * *py;
   }
 }
 
-__global__ __launch_bounds__(256) void dgemv_t_kernel(int m, int n, double alpha,
-                                                      const double* __restrict__ A, int lda,
-                                                      const double* __restrict__ x, int incx,
-                                                      double beta, double* __restrict__ y,
-                                                      int incy, const int* __restrict__ abort) {
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, T alpha,
+                                                     const T* __restrict__ A, int lda,
+                                                     const T* __restrict__ x, int incx,
+                                                     T beta, T* __restrict__ y,
+                                                     int incy, const int* __restrict__ abort) {
   if (abort && *abort) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = blockIdx.x * 4 + wave;
   if (j >= n) return;
-  double acc = 0.0;
+  T acc = T(0);
   for (int i = lane; i < m; i += 64) acc += A[(size_t)j * lda + i] * x[(size_t)i * incx];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) {
-    const double v = alpha * acc;
-    double* py = y + (size_t)j * incy;
-    *py = (beta == 0.0) ? v : v + beta * *py;
+    const T v = alpha * acc;
+    T* py = y + (size_t)j * incy;
+    *py = (beta == T(0)) ? v : v + beta * *py;
   }
 }
 
-int dgemv_launch(bool trans, int m, int n, double alpha, const double* A, int lda,
-                 const double* x, int incx, double beta, double* y, int incy, const int* abort,
-                 hipStream_t s) {
-  if (m <= 0 || n <= 0) return 0;
+template <typename T>
+int gemv_launch(bool trans, int m, int n, T alpha, const T* A, int lda, const T* x, int incx,
+                T beta, T* y, int incy, const int* abort, hipStream_t s) {
+  const int ly = trans ? n : m;  // output length
+  if (ly <= 0) return 0;
+  // an empty inner dimension still scales y (both kernels then skip the
+  // products: the 'N' kernel's column loop and the 'T' kernel's row loop
+  // run zero times)
   if (!trans) {
-    hipLaunchKernelGGL(dgemv_n_kernel, dim3((m + 63) / 64), dim3(256), 0, s, m, n, alpha, A, lda,
-                       x, incx, beta, y, incy, abort);
+    hipLaunchKernelGGL(gemv_n_kernel<T>, dim3((m + 63) / 64), dim3(256), 0, s, m, n, alpha, A,
+                       lda, x, incx, beta, y, incy, abort);
   } else {
-    hipLaunchKernelGGL(dgemv_t_kernel, dim3((n + 3) / 4), dim3(256), 0, s, m, n, alpha, A, lda, x,
-                       incx, beta, y, incy, abort);
+    hipLaunchKernelGGL(gemv_t_kernel<T>, dim3((n + 3) / 4), dim3(256), 0, s, m, n, alpha, A, lda,
+                       x, incx, beta, y, incy, abort);
   }
-  DM_LAUNCH_CHECK("dgemv_kernel");
+  DM_LAUNCH_CHECK("gemv_kernel");
   return 0;
 }
+
+template int gemv_launch<double>(bool, int, int, double, const double*, int, const double*, int,
+                                 double, double*, int, const int*, hipStream_t);
+template int gemv_launch<float>(bool, int, int, float, const float*, int, const float*, int,
+                                float, float*, int, const int*, hipStream_t);
 
 // ------------------------------------------------------------ fill
 __global__ void fill_i32_kernel(int* __restrict__ dst, int value, size_t count) {

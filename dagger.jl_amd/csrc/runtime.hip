@@ -1,10 +1,7 @@
 // C ABI: runtime (errors, device, memory, streams, events, graphs) and the
 // BLAS/LAPACK-shaped tile entry points.
-#include <execinfo.h>
 #include <pthread.h>
-#include <unistd.h>
 #include <condition_variable>
-#include <csignal>
 #include <functional>
 #include <cstdio>
 #include <cstdlib>
@@ -22,26 +19,6 @@ static int g_device = -1;
 static hipStream_t g_default_stream = nullptr;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
-
-// Opt-in host crash report: with DM_SEGV_BACKTRACE=1 in the environment a
-// SIGSEGV in host code prints the native stack (library offsets) to stderr.
-namespace {
-void segv_report(int sig) {
-  void* frames[64];
-  const int n = backtrace(frames, 64);
-  const char msg[] = "dagger_mi355x: host SIGSEGV, native backtrace:\n";
-  (void)!write(2, msg, sizeof msg - 1);
-  backtrace_symbols_fd(frames, n, 2);
-  signal(sig, SIG_DFL);
-  raise(sig);
-}
-struct SegvInstaller {
-  SegvInstaller() {
-    const char* e = getenv("DM_SEGV_BACKTRACE");
-    if (e && e[0] == '1') signal(SIGSEGV, segv_report);
-  }
-} g_segv_installer;
-}  // namespace
 
 int hip_fail(hipError_t e, const char* what) {
   char buf[512];
@@ -475,9 +452,7 @@ static int potrf_api(char uplo, int n, T* A, int lda, int* info_dev, dm_stream_t
   int rc = fill_i32_launch(info_dev, 0, 1, st);
   if (rc) return rc;
   if (n == 0) return 0;
-  T* ws = (T*)stream_workspace(st, trsm_ws_elems<T>() * sizeof(T));
-  if (!ws) return arg_fail(0, "potrf: workspace allocation failed");
-  return potrf_full<T>(is_upper(uplo), n, A, lda, info_dev, nullptr, 0, ws, st);
+  return potrf_full<T>(is_upper(uplo), n, A, lda, info_dev, nullptr, 0, nullptr, st);
 }
 
 extern "C" int dm_dpotrf(char uplo, int n, double* A, int lda, int* info_dev, dm_stream_t s) {
@@ -624,8 +599,8 @@ extern "C" int dm_dgemv(char trans, int m, int n, double alpha, const double* A,
   if (incx <= 0) return arg_fail(8, "incx must be positive");
   if (incy <= 0) return arg_fail(11, "incy must be positive");
   DM_INIT();
-  return dgemv_launch(is_trans(trans), m, n, alpha, A, lda, x, incx, beta, y, incy, nullptr,
-                      stream_of(s));
+  return gemv_launch<double>(is_trans(trans), m, n, alpha, A, lda, x, incx, beta, y, incy,
+                             nullptr, stream_of(s));
 }
 
 extern "C" int dm_fill_randn_d(double* x, size_t count, uint64_t seed, uint64_t offset, dm_stream_t s) {

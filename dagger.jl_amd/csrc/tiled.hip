@@ -458,8 +458,11 @@ int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A, con
   const int64_t am = is_trans(transa) ? A->n : A->m, ak = is_trans(transa) ? A->m : A->n;
   const int64_t bk = is_trans(transb) ? B->n : B->m, bn = is_trans(transb) ? B->m : B->n;
   if (am != Cm->m || bn != Cm->n || ak != bk) return arg_fail(7, "shapes do not conform");
+  // One square block size throughout; C may only differ in its column block
+  // when it has a single block column that B's blocking also gives (n <= nb
+  // of A), since the GEMM tasks take C's column count from B's tiles.
   if (A->mb != A->nb || B->mb != A->mb || B->nb != A->mb || Cm->mb != A->mb ||
-      (Cm->n > 0 && Cm->nb != A->mb && Cm->n > Cm->nb))
+      (Cm->nb != A->mb && Cm->n > A->mb))
     return arg_fail(7, "all operands must use one square block size");
   rc = ensure_init();
   if (rc) return rc;

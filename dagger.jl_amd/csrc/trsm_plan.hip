@@ -1,23 +1,16 @@
 // Host planning of batched recursive triangular solves (no device code):
 // the effective-solve shape of a BLAS trsm call and the leaf/GEMM launch
 // sequence of a group of solves.  Used by the datadeps executor.
-#include <cstdlib>
 #include <map>
 #include "trsm_plan.h"
 
 namespace dm {
 
 int trsm_leaf_len() {
-  // 128 by default: leaves of 128 rows need half the LDS of 256-row leaves,
-  // so two leaf workgroups share a CU (measured at N = 32768: ldiv! 33.5 ms
-  // against 36.6 ms, cholesky! unchanged).  Environment DM_TRSM_LEAF: a
-  // multiple of 32 in [64, TRSM_LEAF].
-  static const int len = [] {
-    const char* e = std::getenv("DM_TRSM_LEAF");
-    const int v = e ? std::atoi(e) : 128;
-    return (v >= 64 && v <= TRSM_LEAF && v % TRSM_BS == 0) ? v : 128;
-  }();
-  return len;
+  // Leaves of 128 rows need half the LDS of 256-row leaves, so two leaf
+  // workgroups share a CU (measured at N = 32768: ldiv! 33.5 ms against
+  // 36.6 ms, cholesky! unchanged).
+  return 128;
 }
 
 TrsmShape trsm_shape(char side, char uplo, char transa, char diag) {
@@ -90,18 +83,6 @@ void trsm_plan_steps(const TrsmShape& sh, const std::vector<TrsmJob<T>>& jobs,
   for (const TrsmJob<T>& j : jobs)
     if (j.L > 0 && j.NV > 0) byL[j.L].push_back(j);
   for (auto& kv : byL) plan_rec<T>(sh, kv.first, kv.second, steps, leaf);
-}
-
-int trsm_crit_leaf_len() {
-  // Leaf length of the solves on a look-ahead schedule's critical stream
-  // (environment DM_TRSM_LEAF_CRIT, a multiple of 32 in [64, TRSM_LEAF];
-  // by default the general leaf length).
-  static const int len = [] {
-    const char* e = std::getenv("DM_TRSM_LEAF_CRIT");
-    const int v = e ? std::atoi(e) : 0;
-    return (v >= 64 && v <= TRSM_LEAF && v % TRSM_BS == 0) ? v : trsm_leaf_len();
-  }();
-  return len;
 }
 
 template void trsm_plan_steps<double>(const TrsmShape&, const std::vector<TrsmJob<double>>&,

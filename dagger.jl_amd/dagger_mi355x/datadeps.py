@@ -84,8 +84,17 @@ def _tile(a, what):
     return a
 
 
+def _dims(ok, what):
+    """BLAS DimensionMismatch: raised before a task with inconsistent
+    operand shapes reaches the executor (the kernels trust the shapes)."""
+    if not ok:
+        raise ValueError(f"DimensionMismatch: {what}")
+
+
 def make_task(f, args):
-    """Translate a Julia-style tile call into a dm_task."""
+    """Translate a Julia-style tile call into a dm_task.  Operand shapes are
+    checked as BLAS.gemm!/syrk!/trsm!/gemv!/trsv! and LAPACK.potrf! check
+    them (DimensionMismatch, raised as ValueError)."""
     t = _lib.dm_task()
     t.op = f.code
     t.side = t.uplo = t.transa = t.transb = t.diag = b"N"
@@ -96,6 +105,10 @@ def make_task(f, args):
         t.transa, t.transb = ch(ta), ch(tb)
         t.m, t.n = Cm.rows, Cm.cols
         t.k = A.rows if ta in ("T", "C") else A.cols
+        am = A.cols if ta in ("T", "C") else A.rows
+        bk, bn = (B.cols, B.rows) if tb in ("T", "C") else (B.rows, B.cols)
+        _dims(am == t.m and bn == t.n and bk == t.k,
+              f"gemm: op(A) {am}x{t.k}, op(B) {bk}x{bn}, C {t.m}x{t.n}")
         t.alpha, t.beta = alpha, beta
         t.a, t.lda, t.b, t.ldb, t.c, t.ldc = A.ptr, A.ld, B.ptr, B.ld, Cm.ptr, Cm.ld
         t.access[0], t.access[1], t.access[2] = ma, mb, mc
@@ -106,6 +119,9 @@ def make_task(f, args):
         t.uplo, t.transa = ch(uplo), ch(tr)
         t.n = Cm.rows
         t.k = A.rows if tr in ("T", "C") else A.cols
+        an = A.cols if tr in ("T", "C") else A.rows
+        _dims(Cm.rows == Cm.cols and an == t.n,
+              f"syrk: C {Cm.rows}x{Cm.cols} must be square with op(A)'s {an} rows")
         t.alpha, t.beta = alpha, beta
         t.a, t.lda, t.c, t.ldc = A.ptr, A.ld, Cm.ptr, Cm.ld
         t.access[0], t.access[2] = ma, mc
@@ -115,6 +131,9 @@ def make_task(f, args):
         A, B = _tile(A, "A"), _tile(B, "B")
         t.side, t.uplo, t.transa, t.diag = ch(side), ch(uplo), ch(tr), ch(diag)
         t.m, t.n = B.rows, B.cols
+        ka = B.rows if side == "L" else B.cols
+        _dims(A.rows == A.cols == ka,
+              f"trsm: A {A.rows}x{A.cols} must be square of order {ka}")
         t.alpha = alpha
         t.a, t.lda, t.c, t.ldc = A.ptr, A.ld, B.ptr, B.ld
         t.access[0], t.access[2] = ma, mb
@@ -123,6 +142,7 @@ def make_task(f, args):
         uplo = vals[0][0]
         A, ma = vals[1]
         A = _tile(A, "A")
+        _dims(A.rows == A.cols, f"potrf: A {A.rows}x{A.cols} must be square")
         t.uplo = ch(uplo)
         t.n = A.rows
         t.c, t.ldc = A.ptr, A.ld
@@ -141,6 +161,9 @@ def make_task(f, args):
         A, x, y = _tile(A, "A"), _tile(x, "x"), _tile(y, "y")
         t.transa = ch(tr)
         t.m, t.n = A.rows, A.cols
+        lx, ly = (A.rows, A.cols) if tr in ("T", "C") else (A.cols, A.rows)
+        _dims(x.rows * x.cols == lx and y.rows * y.cols == ly and x.cols == 1 and y.cols == 1,
+              f"gemv: op(A) {ly}x{lx}, x {x.rows}x{x.cols}, y {y.rows}x{y.cols}")
         t.alpha, t.beta = alpha, beta
         t.a, t.lda, t.b, t.ldb, t.c, t.ldc = A.ptr, A.ld, x.ptr, 1, y.ptr, 1
         t.access[0], t.access[1], t.access[2] = ma, mx, my
@@ -150,6 +173,8 @@ def make_task(f, args):
         A, x = _tile(A, "A"), _tile(x, "x")
         t.uplo, t.transa, t.diag = ch(uplo), ch(tr), ch(diag)
         t.n = A.rows
+        _dims(A.rows == A.cols and x.rows == A.rows and x.cols == 1,
+              f"trsv: A {A.rows}x{A.cols} must be square, x {x.rows}x{x.cols} of its order")
         t.a, t.lda, t.c, t.ldc = A.ptr, A.ld, x.ptr, 1
         t.access[0], t.access[2] = ma, mx
         dt = A.dtype

@@ -242,11 +242,53 @@ def trsm_(side, uplo, trans, diag, alpha, A, B, via="native", options=None):
     return B
 
 
-def _trsm_datadeps(side, uplo, trans, diag, alpha, Ac, Bc, options):
-    """src/array/trsm.jl:88-190 (left side; the branches ldiv! uses)."""
-    if side != "L":
-        raise NotImplementedError("datadeps trsm_ mirror covers side 'L'; use via='native'")
+def _trsm_datadeps_right(uplo, trans, diag, alpha, Ac, Bc):
+    """src/array/trsm.jl:143-189 (side 'R'), inside an open region.  The
+    transposed branches pass alpha to every diagonal solve and 1 to every
+    update, as the reference does."""
     Bmt, Bnt = Bc.shape
+    if uplo == "U" and trans == "N":
+        for k in range(Bnt):
+            la = alpha if k == 0 else 1.0
+            for m in range(Bmt):
+                spawn(BLAS.trsm, "R", "U", "N", diag, la, In(Ac[k, k]), InOut(Bc[m, k]))
+            for m in range(Bmt):
+                for n in range(k + 1, Bnt):
+                    spawn(BLAS.gemm, "N", "N", -1.0, In(Bc[m, k]), In(Ac[k, n]), la,
+                          InOut(Bc[m, n]))
+    elif uplo == "U":
+        for k in range(1, Bnt + 1):
+            kk = Bnt - k
+            for m in range(Bmt):
+                spawn(BLAS.trsm, "R", "U", trans, diag, alpha, In(Ac[kk, kk]), InOut(Bc[m, kk]))
+                for n in range(k + 1, Bnt + 1):
+                    spawn(BLAS.gemm, "N", trans, -1.0, In(Bc[m, kk]), In(Ac[Bnt - n, kk]), 1.0,
+                          InOut(Bc[m, Bnt - n]))
+    elif trans == "N":
+        for k in range(1, Bnt + 1):
+            la = alpha if k == 1 else 1.0
+            kk = Bnt - k
+            for m in range(Bmt):
+                spawn(BLAS.trsm, "R", "L", "N", diag, la, In(Ac[kk, kk]), InOut(Bc[m, kk]))
+                for n in range(k + 1, Bnt + 1):
+                    spawn(BLAS.gemm, "N", "N", -1.0, In(Bc[m, kk]), In(Ac[kk, Bnt - n]), la,
+                          InOut(Bc[m, Bnt - n]))
+    else:
+        for k in range(Bnt):
+            for m in range(Bmt):
+                spawn(BLAS.trsm, "R", "L", trans, diag, alpha, In(Ac[k, k]), InOut(Bc[m, k]))
+                for n in range(k + 1, Bnt):
+                    spawn(BLAS.gemm, "N", trans, -1.0, In(Bc[m, k]), In(Ac[n, k]), 1.0,
+                          InOut(Bc[m, n]))
+
+
+def _trsm_datadeps(side, uplo, trans, diag, alpha, Ac, Bc, options):
+    """src/array/trsm.jl:88-190, every side/uplo/trans branch."""
+    Bmt, Bnt = Bc.shape
+    if side == "R":
+        with spawn_datadeps(options=options):
+            _trsm_datadeps_right(uplo, trans, diag, alpha, Ac, Bc)
+        return
     with spawn_datadeps(options=options):
         if uplo == "U" and trans == "N":
             for k in range(1, Bmt + 1):
@@ -291,33 +333,37 @@ def _trsm_datadeps(side, uplo, trans, diag, alpha, Ac, Bc, options):
 
 
 def trsv_(uplo, trans, diag, alpha, A, b, options=None):
-    """Dagger.trsv! (src/array/trsm.jl:1-57): trsv/gemv tile tasks."""
+    """Dagger.trsv! (src/array/trsm.jl:1-57): trsv/gemv tile tasks.  As in
+    the reference, alpha is the GEMV beta of the first step's updates only
+    (`lalpha`); the first diagonal block solved is not scaled."""
     options = options or PlanOptions(num_streams=1, batch=False, lookahead=0)
-    if alpha != 1.0:
-        raise NotImplementedError("trsv_ mirrors the ldiv! use (alpha = 1)")
     Ac, Bc = A.chunks, b.chunks
     Bnt = len(Bc)
     with spawn_datadeps(options=options):
         if uplo == "U" and trans == "N":
             for k in reversed(range(Bnt)):
+                la = alpha if k == Bnt - 1 else 1.0
                 spawn(BLAS.trsv, "U", "N", diag, In(Ac[k, k]), InOut(Bc[k]))
                 for i in range(k):
-                    spawn(BLAS.gemv, "N", -1.0, In(Ac[i, k]), In(Bc[k]), 1.0, InOut(Bc[i]))
+                    spawn(BLAS.gemv, "N", -1.0, In(Ac[i, k]), In(Bc[k]), la, InOut(Bc[i]))
         elif uplo == "U":
             for k in range(Bnt):
+                la = alpha if k == 0 else 1.0
                 spawn(BLAS.trsv, "U", trans, diag, In(Ac[k, k]), InOut(Bc[k]))
                 for i in range(k + 1, Bnt):
-                    spawn(BLAS.gemv, trans, -1.0, In(Ac[k, i]), In(Bc[k]), 1.0, InOut(Bc[i]))
+                    spawn(BLAS.gemv, trans, -1.0, In(Ac[k, i]), In(Bc[k]), la, InOut(Bc[i]))
         elif trans == "N":
             for k in range(Bnt):
+                la = alpha if k == 0 else 1.0
                 spawn(BLAS.trsv, "L", "N", diag, In(Ac[k, k]), InOut(Bc[k]))
                 for i in range(k + 1, Bnt):
-                    spawn(BLAS.gemv, "N", -1.0, In(Ac[i, k]), In(Bc[k]), 1.0, InOut(Bc[i]))
+                    spawn(BLAS.gemv, "N", -1.0, In(Ac[i, k]), In(Bc[k]), la, InOut(Bc[i]))
         else:
             for k in reversed(range(Bnt)):
+                la = alpha if k == Bnt - 1 else 1.0
                 spawn(BLAS.trsv, "L", trans, diag, In(Ac[k, k]), InOut(Bc[k]))
                 for i in range(k):
-                    spawn(BLAS.gemv, trans, -1.0, In(Ac[k, i]), In(Bc[k]), 1.0, InOut(Bc[i]))
+                    spawn(BLAS.gemv, trans, -1.0, In(Ac[k, i]), In(Bc[k]), la, InOut(Bc[i]))
     return b
 
 
@@ -468,9 +514,9 @@ def _mul_vec(y, A, x, alpha, beta, ta, options):
     the smallest block dimension when they differ, y takes op(A)'s row
     blocking) and gemv_dagger!'s loop (:474-541: for each y block, the
     k-ordered tile GEMVs, beta applied by the first), run by the native
-    executor's FP64 GEMV tile kernel."""
-    if not (y.dtype == A.dtype == x.dtype == np.float64):
-        raise NotImplementedError("matrix-vector products run in FP64")
+    executor's GEMV tile kernel (FP64 or FP32)."""
+    if not (y.dtype == A.dtype == x.dtype):
+        raise ValueError("y, A and x must share one element type")
     am, ak = (A.n, A.m) if ta == "T" else (A.m, A.n)
     if ak != x.m or am != y.m:
         raise ValueError("shapes do not conform")

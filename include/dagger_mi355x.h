@@ -200,10 +200,7 @@ typedef struct dm_plan_opts {
     int num_streams;     /* streams to spread groups over (>=1) */
     int batch;           /* 1: batch independent same-shape tasks per launch */
     int lookahead;       /* slack (in levels) below which tasks go on the
-                            critical stream; 0 disables the split.  The
-                            environment variable DM_CRIT_CUS (a CU count,
-                            default 0) makes bulk GEMM/TRSM launches leave
-                            that many CUs free for the critical path */
+                            critical stream; 0 disables the split */
     int use_graph;       /* 1: capture the schedule into a hipGraph (plans
                             with at most 2 streams and at most 2e11 flops;
                             other plans issue eagerly, and

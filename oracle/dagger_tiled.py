@@ -19,6 +19,8 @@ What it restates (paths inside JuliaParallel/Dagger.jl):
   * norm2 of a DArray and of its UpperTriangular / LowerTriangular views
                                         src/array/linalg.jl:1-33
   * LAPACK.chkfinite(::DArray)          src/array/linalg.jl:127-148
+  * ishermitian(::DArray) with is_cross_hermitian
+                                        src/array/linalg.jl:38-48, 90-124
 The tile kernels are the same LAPACK/BLAS routines the reference calls on
 its CPU path (LAPACK.potrf!, BLAS.trsm!/syrk!/gemm!/gemv!/trsv!), here
 through scipy's bundled OpenBLAS.
@@ -415,8 +417,11 @@ def trsm(side, uplo, trans, diag, alpha, A, B, nb):
     return TB.collect()
 
 
-def trsv_region(uplo, trans, diag, TA, TB):
-    """src/array/trsm.jl:16-54 (alpha = 1, the ldiv! use)."""
+def trsv_region(uplo, trans, diag, TA, TB, alpha=1.0):
+    """src/array/trsm.jl:16-54.  alpha enters only as the GEMV beta
+    (`lalpha`) of the first step's updates, as in the reference: the first
+    diagonal block solved is never scaled, so for one block alpha has no
+    effect and for more the result is not alpha * op(A)^-1 b."""
     Ac = TA.chunks
     Bc = [TB.chunks[i, 0] for i in range(TB.mt)]
     Bnt = len(Bc)
@@ -425,31 +430,35 @@ def trsv_region(uplo, trans, diag, TA, TB):
     def trsv(u, t, a, x):
         R.spawn("trsv", tile_trsv, (u, t, diag, a, x), [(a, IN), (x, INOUT)])
 
-    def gemv(t, a, x, y):
-        R.spawn("gemv", tile_gemv, (t, -1.0, a, x, 1.0, y), [(a, IN), (x, IN), (y, INOUT)])
+    def gemv(t, a, x, y, beta):
+        R.spawn("gemv", tile_gemv, (t, -1.0, a, x, beta, y), [(a, IN), (x, IN), (y, INOUT)])
 
     if uplo == "U":
         if trans == "N":
             for k in reversed(range(Bnt)):
+                la = alpha if k == Bnt - 1 else 1.0
                 trsv("U", "N", Ac[k, k], Bc[k])
                 for i in range(k):
-                    gemv("N", Ac[i, k], Bc[k], Bc[i])
+                    gemv("N", Ac[i, k], Bc[k], Bc[i], la)
         else:
             for k in range(Bnt):
+                la = alpha if k == 0 else 1.0
                 trsv("U", trans, Ac[k, k], Bc[k])
                 for i in range(k + 1, Bnt):
-                    gemv(trans, Ac[k, i], Bc[k], Bc[i])
+                    gemv(trans, Ac[k, i], Bc[k], Bc[i], la)
     else:
         if trans == "N":
             for k in range(Bnt):
+                la = alpha if k == 0 else 1.0
                 trsv("L", "N", Ac[k, k], Bc[k])
                 for i in range(k + 1, Bnt):
-                    gemv("N", Ac[i, k], Bc[k], Bc[i])
+                    gemv("N", Ac[i, k], Bc[k], Bc[i], la)
         else:
             for k in reversed(range(Bnt)):
+                la = alpha if k == Bnt - 1 else 1.0
                 trsv("L", trans, Ac[k, k], Bc[k])
                 for i in range(k):
-                    gemv(trans, Ac[k, i], Bc[k], Bc[i])
+                    gemv(trans, Ac[k, i], Bc[k], Bc[i], la)
     return R
 
 
@@ -479,12 +488,12 @@ def gemm(ta, tb, alpha, A, B, beta, Cm, nb):
     return TC.collect()
 
 
-def trsv(uplo, trans, diag, A, b, nb):
-    """Dagger.trsv!(uplo, trans, diag, 1, A, b) (src/array/trsm.jl:1-57) on
-    A tiled nb x nb and b tiled nb; returns x."""
+def trsv(uplo, trans, diag, A, b, nb, alpha=1.0):
+    """Dagger.trsv!(uplo, trans, diag, alpha, A, b) (src/array/trsm.jl:1-57)
+    on A tiled nb x nb and b tiled nb; returns x."""
     TA = TiledMatrix(np.array(A, copy=True), nb, nb)
     TB = TiledMatrix(np.array(b, copy=True), nb)
-    run_sequential(trsv_region(uplo, trans, diag, TA, TB))
+    run_sequential(trsv_region(uplo, trans, diag, TA, TB, alpha))
     return TB.collect().ravel()
 
 
@@ -580,4 +589,33 @@ def chkfinite(A, nb):
         for j in range(T.nt):
             if not np.all(np.isfinite(T.chunks[i, j])):
                 raise ValueError("matrix has Inf or NaN")
+    return True
+
+
+def ishermitian(A, nb):
+    """ishermitian(::DArray) (src/array/linalg.jl:90-124) on A tiled
+    Blocks(nb, nb): every diagonal chunk exactly symmetric (stdlib
+    ishermitian: element equality, so a NaN fails), and for every chunk pair
+    below the diagonal is_cross_hermitian (:38-48):
+        ||A_ij - A_ji^T||_F <= sqrt(eps) * max(||A_ij||_F, ||A_ji^T||_F),
+    eps of the element type, each Frobenius norm sqrt(sum(abs2, .)).
+    A non-square matrix is not Hermitian."""
+    A = np.asarray(A)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        return False
+    rtol = np.sqrt(np.finfo(A.dtype).eps)
+    T = TiledMatrix(A, nb, nb)
+    mt = T.mt
+    for i in range(mt):
+        d = T.chunks[i, i]
+        if not np.array_equal(d, d.T):
+            return False
+    for i in range(1, mt):
+        for j in range(i):
+            a1 = T.chunks[i, j].astype(np.float64)
+            b = T.chunks[j, i].T.astype(np.float64)
+            lhs = np.sqrt(np.sum((a1 - b) ** 2))
+            rhs = rtol * max(np.sqrt(np.sum(a1 ** 2)), np.sqrt(np.sum(b ** 2)))
+            if not lhs <= rhs:
+                return False
     return True

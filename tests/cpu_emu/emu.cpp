@@ -334,11 +334,11 @@ hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
   return hipSuccess;
 }
 hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
-hipError_t hipExtStreamCreateWithCUMask(hipStream_t* s, uint32_t n, const uint32_t* m) {
-  if (!n || !m) return hipErrorInvalidValue;
-  return hipStreamCreateWithFlags(s, 0);
-}
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 hipError_t hipEventCreate(hipEvent_t* e) { *e = new ihipEvent_t; return hipSuccess; }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }

@@ -31,6 +31,8 @@ enum {
   hipErrorInvalidValue = 1,
   hipErrorOutOfMemory = 2,
   hipErrorNoDevice = 100,
+  hipErrorInvalidDevice = 101,
+  hipErrorInvalidSymbol = 701,
   hipErrorLaunchFailure = 719
 };
 typedef struct ihipStream_t* hipStream_t;
@@ -63,7 +65,6 @@ hipError_t hipDeviceGetAttribute(int*, hipDeviceAttribute_t, int);
 hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int);
 hipError_t hipStreamCreateWithFlags(hipStream_t*, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);
-hipError_t hipExtStreamCreateWithCUMask(hipStream_t*, uint32_t, const uint32_t*);
 hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipEventCreate(hipEvent_t*);
 hipError_t hipEventCreateWithFlags(hipEvent_t*, unsigned);
@@ -80,6 +81,11 @@ hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t);
 hipError_t hipGraphDestroy(hipGraph_t);
 hipError_t hipGraphExecDestroy(hipGraphExec_t);
 hipError_t hipMalloc(void**, size_t);
+// a __device__ variable is an ordinary host object here: its "device
+// address" is its host address
+#define HIP_SYMBOL(x) (&(x))
+#define __align__(n) __attribute__((aligned(n)))
+hipError_t hipGetSymbolAddress(void**, const void*);
 hipError_t hipFree(void*);
 hipError_t hipMallocAsync(void**, size_t, hipStream_t);
 hipError_t hipFreeAsync(void*, hipStream_t);

@@ -16,7 +16,6 @@ enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMe
 #define hipEventDisableTiming 2u
 hipError_t hipStreamCreateWithFlags(hipStream_t*, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);
-hipError_t hipExtStreamCreateWithCUMask(hipStream_t*, uint32_t, const uint32_t*);
 enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
 hipError_t hipGetDevice(int*);
 hipError_t hipDeviceGetAttribute(int*, hipDeviceAttribute_t, int);

@@ -8,10 +8,6 @@
 #include "dm_internal.h"
 static int g_streams = 0, g_events = 0;
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (hipStream_t)(uintptr_t)(0x1000 + ++g_streams); return 0; }
-hipError_t hipExtStreamCreateWithCUMask(hipStream_t* s, uint32_t n, const uint32_t* m) {
-  if (!n || !m) abort();
-  return hipStreamCreateWithFlags(s, 0);
-}
 hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 256; return 0; }
 hipError_t hipStreamDestroy(hipStream_t s) { if (!s) { fprintf(stderr, "destroy null stream\n"); abort(); } return 0; }
@@ -65,7 +61,7 @@ template <typename T> int gemm_single(bool, bool, int, int, int, T, const T*, in
 template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t) { if (!inv) abort(); ++g_launch; return 0; }
 template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long max_wg) { if (max_wg < 0) abort(); ++g_launch; return 0; }
 template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t) { if (!ws) abort(); ++g_launch; return 0; }
-template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T* ws, hipStream_t) { if (!ws) { fprintf(stderr, "potrf without workspace\n"); abort(); } ++g_launch; return 0; }
+template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T*, hipStream_t) { ++g_launch; return 0; }
 bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
   return ta && !tb && !((uintptr_t)j.A & 15) && !((uintptr_t)j.B & 15) && !(j.lda & 3) &&
          !(j.ldb & 3);
@@ -86,7 +82,7 @@ int gemm_split_launch(int terms, int, const GemmJob<float>*, int, const GemmJob<
   ++g_launch;
   return 0;
 }
-int dgemv_launch(bool, int, int, double, const double*, int, const double*, int, double, double*, int, const int*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int gemv_launch(bool, int, int, T, const T*, int, const T*, int, T, T*, int, const int*, hipStream_t) { ++g_launch; return 0; }
 int fill_i32_launch(int*, int, size_t, hipStream_t) { ++g_launch; return 0; }
 int copy_i32_launch(int*, const int*, size_t, hipStream_t) { ++g_launch; return 0; }
 #define INST(T) \
@@ -98,7 +94,8 @@ int copy_i32_launch(int*, const int*, size_t, hipStream_t) { ++g_launch; return 
   template int trinv_blocks_launch<T>(const TrsmShape&, const T*, int, int, T*, const int*, hipStream_t); \
   template int trsm_launch<T>(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long); \
   template int trsm_full<T>(char, char, char, char, int, int, T, const T*, int, T*, int, T*, const int*, hipStream_t); \
-  template int potrf_full<T>(bool, int, T*, int, int*, int*, int, T*, hipStream_t);
+  template int potrf_full<T>(bool, int, T*, int, int*, int*, int, T*, hipStream_t); \
+  template int gemv_launch<T>(bool, int, int, T, const T*, int, const T*, int, T, T*, int, const int*, hipStream_t);
 INST(double)
 INST(float)
 }  // namespace dm

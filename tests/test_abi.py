@@ -145,10 +145,11 @@ def test_partial_overlap_orders_accesses():
     b = d.Tile(base + 8 * 64, 128, 1, 128, np.float64)   # overlaps a
     c = d.Tile(base + 8 * 128, 64, 1, 64, np.float64)    # overlaps b only
     M = d.Tile(base + (1 << 20), 128, 128, 128, np.float64)
+    M64 = d.Tile(base + (1 << 21), 64, 64, 64, np.float64)
     from dagger_mi355x.datadeps import make_task
     t0 = make_task(d.BLAS.trsv, ("U", "N", "N", d.In(M), d.InOut(a)))
     t1 = make_task(d.BLAS.gemv, ("N", -1.0, d.In(M), d.In(a), 1.0, d.InOut(b)))
-    t2 = make_task(d.BLAS.trsv, ("U", "N", "N", d.In(M), d.InOut(c)))
+    t2 = make_task(d.BLAS.trsv, ("U", "N", "N", d.In(M64), d.InOut(c)))
     e = _edges_of([t0, t1, t2])
     assert (0, 1) in e      # b overlaps a: RAW + WAW
     assert (1, 2) in e      # c overlaps b
@@ -166,3 +167,40 @@ def test_dag_plan_requires_device_only_for_planning():
     assert dag.edges() == [(0, 1)]
     st = dag.stats()
     assert st["num_tasks"] == 2 and st["num_edges"] == 1
+
+
+def _fake_tiled(m, n, mb, nb, base=1 << 40):
+    import dagger_mi355x._lib as L
+    return L.dm_tiled(base, m, n, mb, nb, L.DM_F64)
+
+
+@pytest.mark.parametrize("cm,cn,cmb,cnb,ok", [
+    # C with one wide block column whose width exceeds A's block: B's tiles
+    # (A's block wide) would be read across their neighbours
+    (256, 100, 64, 128, False),
+    (256, 128, 64, 128, False),
+    # a single block column no wider than A's block is the residual's layout
+    (256, 48, 64, 128, True),
+    (256, 64, 64, 64, True),
+    # several block columns of another width
+    (256, 256, 64, 128, False),
+])
+def test_gemm_tiled_rejects_wide_c_blocks(cm, cn, cmb, cnb, ok):
+    """dm_gemm_tiled takes C's tile widths from B's tiles (one square block
+    size); a C whose single block column is wider than A's block would make
+    the tasks read past B's tiles, so the argument check rejects it before
+    any device work (no GPU needed for the rejection)."""
+    import dagger_mi355x._lib as L
+    lib = L.load()
+    A = _fake_tiled(cm, 256, 64, 64)
+    B = _fake_tiled(256, cn, 64, 64, base=1 << 41)
+    Cm = _fake_tiled(cm, cn, cmb, cnb, base=1 << 42)
+    o = L.dm_plan_opts(1, 0, 0, 0, 0, 0)
+    rc = lib.dm_gemm_tiled(b"N", b"N", 1.0, C.byref(A), C.byref(B), 0.0, C.byref(Cm),
+                           C.byref(o), None)
+    if ok:
+        # passes the checks; without a GPU it stops at device initialisation
+        assert rc == 0 or rc > 0, rc
+    else:
+        assert rc == -7, rc
+        assert b"block size" in lib.dm_last_error_string()

@@ -280,6 +280,37 @@ def test_ishermitian_rules(dm):
     assert dm.ishermitian(dm.DArray.from_array(A, dm.Blocks(32, 64)))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ishermitian_matches_oracle_at_size(dm, dtype):
+    """The device pre-check against the oracle's restatement
+    (src/array/linalg.jl:38-48, 90-124) on an 8192 x 8192 matrix in
+    Blocks(1024, 1024) (36 chunk pairs): symmetric, a far chunk pair
+    perturbed at a quarter / four times its own threshold, one ulp off in
+    the last diagonal chunk, and a NaN in a pair."""
+    n, nb = 8192, 1024
+    A = gen.spd(n, 61).astype(dtype)
+    eps = np.finfo(dtype).eps
+    i, j = 7 * nb + 100, 2 * nb + 300          # chunk pair (7, 2)
+    a1 = A[7 * nb:8 * nb, 2 * nb:3 * nb].astype(np.float64)
+    thr = np.sqrt(eps) * np.linalg.norm(a1)
+    cases = [("symmetric", A, True)]
+    for scale, want in ((0.25, True), (4.0, False)):
+        B = A.copy()
+        B[i, j] = dtype(float(B[i, j]) + scale * thr)
+        cases.append((f"pair x{scale}", B, want))
+    B = A.copy()
+    B[n - 3, n - 1] = np.nextafter(B[n - 3, n - 1], np.inf)
+    cases.append(("ulp in diagonal chunk", B, False))
+    B = A.copy()
+    B[i, j] = np.nan
+    cases.append(("nan in pair", B, False))
+    for name, M, want in cases:
+        assert O.ishermitian(M, nb) == want, name
+        D = dm.DArray.from_array(M, dm.Blocks(nb, nb))
+        assert dm.ishermitian(D) == want, name
+        D.unsafe_free()
+
+
 @pytest.mark.parametrize("m,nb,nrhs", [(1024, 128, 1), (1000, 128, 17), (2048, 256, 32),
                                        (3000, 1024, 64)])
 def test_tiled_gemm_block_column(dm, m, nb, nrhs):

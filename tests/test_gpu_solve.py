@@ -274,3 +274,55 @@ def test_matvec(dm, szA, partA, pb):
         assert Dy.partitioning == dm.Blocks(5)
         assert isapprox(Dy.collect(), 2.0 * ref - y0), ("in place", ta)
         assert np.array_equal(Dx.collect(), x)
+
+
+@pytest.mark.parametrize("uplo,trans", [("U", "N"), ("U", "T"), ("L", "N"), ("L", "T")])
+def test_trsv_alpha_matches_oracle(dm, uplo, trans):
+    """Dagger.trsv! with alpha != 1 (src/array/trsm.jl:16-54): alpha is the
+    first step's GEMV beta, as in the oracle (pinned by closed forms)."""
+    n, nb = 600, 128
+    rng = np.random.default_rng(sum(map(ord, "t" + uplo + trans)))
+    A = 0.05 * rng.standard_normal((n, n)) + np.diag(np.full(n, 2.0))
+    b = rng.standard_normal(n)
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    Db = dm.DArray.from_array(b, dm.Blocks(nb))
+    dm.trsv_(uplo, trans, "N", -2.5, DA, Db)
+    ref = O.trsv(uplo, trans, "N", A, b, nb, alpha=-2.5)
+    assert np.allclose(Db.collect(), ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("uplo,trans", [("U", "N"), ("U", "T"), ("L", "N"), ("L", "T")])
+def test_trsm_right_side_datadeps_equals_native(dm, uplo, trans):
+    """The datadeps mirror of trsm!'s side 'R' branches
+    (src/array/trsm.jl:143-189) records the same tasks as the native
+    builder, so both results are bitwise equal and match the oracle."""
+    n, nb, m = 384, 128, 200
+    rng = np.random.default_rng(sum(map(ord, "R" + uplo + trans)))
+    A = 0.05 * rng.standard_normal((n, n)) + np.diag(np.full(n, 2.0))
+    B = rng.standard_normal((m, n))
+    DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    outs = []
+    for via in ("native", "datadeps"):
+        DB = dm.DArray.from_array(B, dm.Blocks(nb, nb))
+        dm.trsm_("R", uplo, trans, "N", 0.75, DA, DB, via=via)
+        outs.append(DB.collect())
+    assert np.array_equal(outs[0], outs[1])
+    ref = O.trsm("R", uplo, trans, "N", 0.75, A, B, nb)
+    assert np.allclose(outs[0], ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_float32_vector_ldiv(dm, uplo):
+    """ldiv! of an FP32 Cholesky with a DVector right-hand side runs the
+    trsv!/gemv! tile tasks in FP32 (GEMV tile kernel for FP32)."""
+    n, nb = 640, 128
+    A = gen.spd(n, 90)
+    b = gen.randn(n, 91)
+    DA = dm.DArray.from_array(A.astype(np.float32), dm.Blocks(nb, nb))
+    F = dm.cholesky_(DA, uplo=uplo)
+    Db = dm.DArray.from_array(b.astype(np.float32), dm.Blocks(nb))
+    dm.ldiv_(F, Db)
+    x = Db.collect().astype(np.float64)
+    xref = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) < 1e-4
+    assert O.backward_error(A, x, b) < 50 * n * np.finfo(np.float32).eps

@@ -185,6 +185,28 @@ def test_dgemv(dm, rng, trans):
     assert relerr(dy.get(dm), ref) < 1e-14
 
 
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("beta", [0.0, -1.5])
+def test_dgemv_empty_inner_dimension_scales_y(dm, rng, trans, beta):
+    """BLAS gemv with an empty inner dimension (n = 0 for 'N', m = 0 for
+    'T') still sets y := beta y; beta = 0 writes zeros without reading y
+    (a NaN there must not survive)."""
+    m, n = (300, 0) if trans == "N" else (0, 300)
+    y = rng.standard_normal(300)
+    y[7] = np.nan
+    A = np.zeros((max(m, 1), 1))
+    dA, dx, dy = Dev(dm, A), Dev(dm, np.zeros(1)), Dev(dm, y)
+    dm._lib.call("dm_dgemv", trans.encode(), m, n, 2.0, dA.p, max(m, 1), dx.p, 1, beta, dy.p, 1,
+                 None)
+    got = dy.get(dm)
+    if beta == 0.0:
+        assert np.array_equal(got, np.zeros(300))
+    else:
+        ref = beta * y
+        assert np.array_equal(got[~np.isnan(ref)], ref[~np.isnan(ref)])
+        assert np.isnan(got[7])
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_dtrsv(dm, uplo, trans):

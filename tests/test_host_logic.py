@@ -36,6 +36,43 @@ def test_make_task_dimensions_follow_blas_calls():
         make_task(d.BLAS.gemm, ("N", "N", 1.0, np.zeros((2, 2)), T(2, 2), 0.0, T(2, 2)))
 
 
+@pytest.mark.parametrize("f,args", [
+    # gemm: B's tile is not k x n (k = 64 from op(A) = A^T)
+    ("gemm", ("T", "N", -1.0, "A64x32", "B48x48", 1.0, "C32x48")),
+    # gemm: C's rows differ from op(A)'s
+    ("gemm", ("N", "N", 1.0, "A64x32", "B32x48", 0.0, "C32x48")),
+    # gemm: transposed B with the wrong inner size
+    ("gemm", ("N", "T", 1.0, "A32x16", "B48x8", 0.0, "C32x48")),
+    # syrk: C not op(A)'s order
+    ("syrk", ("U", "T", -1.0, "A16x40", 1.0, "C32x32")),
+    # trsm: A not square of B's row count (side L) / column count (side R)
+    ("trsm", ("L", "U", "T", "N", 1.0, "A20x20", "B7x20")),
+    ("trsm", ("R", "L", "T", "N", 1.0, "A7x7", "B7x20")),
+    # gemv: x length differs from op(A)'s columns, y from its rows
+    ("gemv", ("N", -1.0, "A32x16", "x32x1", 1.0, "y32x1")),
+    ("gemv", ("T", -1.0, "A32x16", "x32x1", 1.0, "y32x1")),
+    # trsv: x not of A's order
+    ("trsv", ("U", "N", "N", "A32x32", "x16x1")),
+    ("potrf", ("U", "A8x4")),
+])
+def test_make_task_rejects_mismatched_tiles(f, args):
+    """BLAS.gemm!/syrk!/trsm!/gemv!/trsv! and LAPACK.potrf! throw
+    DimensionMismatch on inconsistent operands; a task with such operands
+    must never reach the executor, whose kernels trust the shapes."""
+    import re as _re
+    import dagger_mi355x as d
+    from dagger_mi355x.datadeps import make_task
+
+    def conv(a):
+        if isinstance(a, str) and _re.fullmatch(r"[A-Za-z]\d+x\d+", a):
+            r, c = map(int, a[1:].split("x"))
+            return d.In(d.Tile(1 << 30, r, c, r, np.float64))
+        return a
+    fn = d.LAPACK.potrf if f == "potrf" else getattr(d.BLAS, f)
+    with pytest.raises(ValueError, match="DimensionMismatch"):
+        make_task(fn, tuple(conv(a) for a in args))
+
+
 def test_serving_trace_model():
     from dagger_mi355x import serving
     tr = serving.make_trace(rate=500.0, count=4000, seed=1)

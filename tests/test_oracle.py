@@ -178,6 +178,85 @@ def test_trsv_all_branches(uplo, trans, diag):
     assert np.allclose(x, ref, rtol=1e-12, atol=1e-13)
 
 
+@pytest.mark.parametrize("uplo,trans", [("U", "N"), ("U", "T"), ("L", "N"), ("L", "T")])
+def test_trsv_alpha_enters_first_updates_only(uplo, trans):
+    """trsv! with alpha (src/array/trsm.jl:16-54): alpha is only the GEMV
+    beta of the first step, so with one block it has no effect, and with two
+    blocks the first block solved is op(T_ff)^-1 b_f while the other is
+    op(T_ss)^-1 (alpha b_s - op(T)_sf x_f) -- closed forms from dense
+    solves.  Uniform scaling by alpha (a plausible restatement) fails."""
+    n, nb = 64, 32
+    rng = np.random.default_rng(sum(map(ord, "a" + uplo + trans)))
+    A = 0.05 * rng.standard_normal((n, n)) + np.diag(np.full(n, 2.0))
+    b = rng.standard_normal(n)
+    alpha = -2.5
+    # one block: alpha ignored
+    x1 = O.trsv(uplo, trans, "N", A, b, n, alpha)
+    T = _tri(A, uplo, "N")
+    op = T.T if trans == "T" else T
+    assert np.allclose(x1, np.linalg.solve(op, b), rtol=1e-12, atol=1e-13)
+    # two blocks: which block is solved first follows op(T)'s triangle
+    x2 = O.trsv(uplo, trans, "N", A, b, nb, alpha)
+    lower_op = (uplo == "L") == (trans == "N")
+    f, s_ = (slice(0, nb), slice(nb, n)) if lower_op else (slice(nb, n), slice(0, nb))
+    xf = np.linalg.solve(op[f, f], b[f])
+    xs = np.linalg.solve(op[s_, s_], alpha * b[s_] - op[s_, f] @ xf)
+    ref = np.empty(n)
+    ref[f], ref[s_] = xf, xs
+    assert np.allclose(x2, ref, rtol=1e-12, atol=1e-13)
+    assert not np.allclose(x2, alpha * np.linalg.solve(op, b), rtol=1e-6)
+
+
+def test_ishermitian_restatement():
+    """ishermitian (src/array/linalg.jl:38-48, 90-124): exact symmetry on
+    diagonal chunks, the sqrt(eps)-relative Frobenius rule on chunk pairs.
+    Each rule is pinned by a case that only it decides: a one-ulp asymmetry
+    inside a diagonal chunk fails; the same asymmetry across chunks passes;
+    a pair perturbed at half / twice the threshold passes / fails; NaN and
+    non-square fail; a pair check that compared A_ij with A_ji (no
+    transpose) would fail the plain symmetric case."""
+    n, nb = 96, 32
+    rng = np.random.default_rng(5)
+    G = rng.standard_normal((n, n))
+    A = G + G.T
+    assert O.ishermitian(A, nb)
+    assert not O.ishermitian(A[:, :64], nb)
+    B = A.copy()
+    B[3, 5] = np.nextafter(B[3, 5], np.inf)          # inside diagonal chunk 0
+    assert not O.ishermitian(B, nb)
+    B = A.copy()
+    B[70, 5] = np.nextafter(B[70, 5], np.inf)        # chunk pair (2, 0)
+    assert O.ishermitian(B, nb)
+    # threshold of pair (2, 0): sqrt(eps) * max of the two chunk norms
+    a1, a2 = A[64:96, 0:32], A[0:32, 64:96]
+    thr = np.sqrt(np.finfo(float).eps) * max(np.linalg.norm(a1), np.linalg.norm(a2))
+    for scale, want in ((0.5, True), (2.0, False)):
+        B = A.copy()
+        B[70, 5] += scale * thr                        # ||A_ij - A_ji^T||_F = scale * thr
+        assert O.ishermitian(B, nb) == want
+    B = A.copy()
+    B[40, 40] = np.nan
+    assert not O.ishermitian(B, nb)
+    B = A.copy()
+    B[70, 5] = B[5, 70] = np.nan
+    assert not O.ishermitian(B, nb)
+    # ragged last chunk (n = 100 over 32): the corner chunk is 4 x 4
+    C2 = rng.standard_normal((100, 100))
+    C2 = C2 + C2.T
+    assert O.ishermitian(C2, 32)
+    C2[98, 99] += 1.0
+    assert not O.ishermitian(C2, 32)
+    # FP32 uses sqrt(eps(Float32))
+    A32 = A.astype(np.float32)
+    B32 = A32.copy()
+    thr32 = np.sqrt(np.finfo(np.float32).eps) * max(np.linalg.norm(A32[64:96, 0:32]),
+                                                    np.linalg.norm(A32[0:32, 64:96]))
+    B32[70, 5] += np.float32(0.25 * thr32)
+    assert O.ishermitian(B32, nb)
+    B32[70, 5] += np.float32(4.0 * thr32)
+    assert not O.ishermitian(B32, nb)
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("vector", [False, True])
 def test_ldiv_solves_system(uplo, vector):
