@@ -97,8 +97,10 @@ struct Dag {
   // profiling (opts.profile, eager runs): events around each group
   std::vector<hipEvent_t> prof_start, prof_stop;
   bool profiled = false;
-  // Extra work appended to every run on the origin stream (captured too).
-  std::function<int(Dag*, hipStream_t)> post_hook;
+  // Extra work issued on the origin stream before the groups fork onto
+  // their streams (pre_hook) and after they join (post_hook), in every run
+  // (captured too).
+  std::function<int(Dag*, hipStream_t)> pre_hook, post_hook;
 
   int add(const dm_task& t);
   int plan(const dm_plan_opts& o);

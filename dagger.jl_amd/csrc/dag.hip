@@ -570,6 +570,10 @@ int Dag::issue(hipStream_t o) {
   // reset the abort flag, fork, run groups, join
   int rc = fill_i32_launch(abort_dev, 0, 1, o);
   if (rc) return rc;
+  if (pre_hook) {
+    rc = pre_hook(this, o);
+    if (rc) return rc;
+  }
   // worker streams: the plan's own, else the caller's shared executor
   // streams; a one-stream eager plan runs on the caller's stream itself
   hipStream_t ss[8];

@@ -30,6 +30,19 @@ __global__ void convert_s2d_kernel(double* __restrict__ dst, const float* __rest
   if (i < count) dst[i] = (double)src[i];
 }
 
+// A[i,i] *= factor for i < n on a square tiled matrix with square blocks
+// (tile (t,t) of a Blocks(nb, nb) m x m slab starts at t*nb*m + t*nb*cols(t)
+// and has ld = rows(t) = cols(t)).
+template <typename T>
+__global__ void scale_diag_kernel(T* __restrict__ A, int64_t m, int nb, double factor) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t t = i / nb, l = i - t * nb;
+  const int64_t ext = (m - t * nb) < nb ? (m - t * nb) : nb;  // rows = cols of tile t
+  T* d = A + t * nb * m + t * nb * ext + l * ext + l;
+  *d = (T)((double)*d * factor);
+}
+
 // x += (double)d over a packed slab.
 __global__ void axpy_s2d_kernel(double* __restrict__ x, const float* __restrict__ d,
                                 size_t count) {
@@ -339,6 +352,25 @@ int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s) {
   hipLaunchKernelGGL(col_abs_max_kernel, dim3((unsigned)X->n), dim3(256), 0, stream_of(s),
                      (const double*)X->base, X->m, X->n, X->mb, X->nb, out_dev);
   DM_LAUNCH_CHECK("col_abs_max_kernel");
+  return 0;
+}
+
+int dm_tiled_scale_diag(const dm_tiled* A, double factor, dm_stream_t s) {
+  if (!A || !A->base) return arg_fail(1, "NULL tiled matrix");
+  if (A->m != A->n || A->mb != A->nb) return arg_fail(1, "A must be square with square blocks");
+  if (A->dtype != DM_F64 && A->dtype != DM_F32) return arg_fail(1, "unknown dtype");
+  if (!(factor > 0.0)) return arg_fail(2, "factor must be positive");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (A->m == 0) return 0;
+  const dim3 grid(nblocks((size_t)A->m));
+  if (A->dtype == DM_F64)
+    hipLaunchKernelGGL(scale_diag_kernel<double>, grid, dim3(256), 0, stream_of(s),
+                       (double*)A->base, A->m, A->nb, factor);
+  else
+    hipLaunchKernelGGL(scale_diag_kernel<float>, grid, dim3(256), 0, stream_of(s),
+                       (float*)A->base, A->m, A->nb, factor);
+  DM_LAUNCH_CHECK("scale_diag_kernel");
   return 0;
 }
 

@@ -20,6 +20,7 @@
 #include <unordered_map>
 #include <vector>
 #include "dag.h"
+#include "potrs.h"
 
 namespace dm {
 
@@ -275,6 +276,78 @@ void build_gemm_tiled(Dag& d, char ta, char tb, double alpha, const dm_tiled& A,
 
 namespace {
 
+// ldiv!(C::Cholesky, B) (src/array/linalg.jl:385-401) as two GEMM-only
+// substitution sweeps over the factor A with the inverses D_k of its
+// diagonal tiles (dense rows(k) x rows(k) arrays at dinv(k)), for every
+// block column c of B; Y (B's shape) holds the intermediate solution.
+//   'U' (A = U):  Y_k = D_k^T R_k,  R_j -= U_kj^T Y_k (j > k);
+//                 X_k = D_k Y_k,    Y_i -= U_ik X_k   (i < k, k descending)
+//   'L' (A = L):  Y_k = D_k R_k,    R_j -= L_jk Y_k   (j > k);
+//                 X_k = D_k^T Y_k,  Y_i -= L_ki^T X_k (i < k, k descending)
+// R is B itself (updated in place by the first sweep) and X lands in B.
+// The update order over k for each block row is the reference's: tile
+// products are subtracted in increasing (first sweep) and decreasing
+// (second sweep) k, as trsm! applies them (src/array/trsm.jl:90-114).
+void build_potrs_inv(Dag& d, char uplo, const TileView& A, const TileView& B, const TileView& Y,
+                     const std::function<void*(int)>& dinv) {
+  const int dt = B.dtype, mt = A.mt;
+  const bool up = is_upper(uplo);
+  for (int c = 0; c < B.nt; ++c) {
+    const int nc = B.cols(c);
+    for (int k = 0; k < mt; ++k) {
+      const int rk = A.rows(k);
+      add_gemm(d, dt, up ? 'T' : 'N', 'N', rk, nc, rk, 1.0, dinv(k), rk, B.ptr(k, c), B.ld(k),
+               0.0, Y.ptr(k, c), Y.ld(k));
+      for (int j = k + 1; j < mt; ++j) {
+        if (up)
+          add_gemm(d, dt, 'T', 'N', A.rows(j), nc, rk, -1.0, A.ptr(k, j), A.ld(k), Y.ptr(k, c),
+                   Y.ld(k), 1.0, B.ptr(j, c), B.ld(j));
+        else
+          add_gemm(d, dt, 'N', 'N', A.rows(j), nc, rk, -1.0, A.ptr(j, k), A.ld(j), Y.ptr(k, c),
+                   Y.ld(k), 1.0, B.ptr(j, c), B.ld(j));
+      }
+    }
+    for (int k = mt - 1; k >= 0; --k) {
+      const int rk = A.rows(k);
+      add_gemm(d, dt, up ? 'N' : 'T', 'N', rk, nc, rk, 1.0, dinv(k), rk, Y.ptr(k, c), Y.ld(k),
+               0.0, B.ptr(k, c), B.ld(k));
+      for (int i = 0; i < k; ++i) {
+        if (up)
+          add_gemm(d, dt, 'N', 'N', A.rows(i), nc, rk, -1.0, A.ptr(i, k), A.ld(i), B.ptr(k, c),
+                   B.ld(k), 1.0, Y.ptr(i, c), Y.ld(i));
+        else
+          add_gemm(d, dt, 'T', 'N', A.rows(i), nc, rk, -1.0, A.ptr(k, i), A.ld(k), B.ptr(k, c),
+                   B.ld(k), 1.0, Y.ptr(i, c), Y.ld(i));
+      }
+    }
+  }
+}
+
+// Device memory owned by a cached plan (freed when the plan is dropped; the
+// plan's destructor first waits for the device).
+struct DevMem {
+  void* p = nullptr;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// The streaming ldiv! plan: inverse diagonal tiles formed on the origin
+// stream before the GEMM-only sweeps fork onto the executor's streams.
+template <typename T>
+int setup_potrs_inv(Dag& dag, char up, const TileView& ta, const TileView& tb) {
+  auto plan = std::make_shared<TrtriPlan<T>>();
+  int rc = plan->build(is_upper(up), ta.m, ta.mb, [&](int k) { return (const T*)ta.ptr(k, k); });
+  if (rc) return rc;
+  auto ybuf = std::make_shared<DevMem>();
+  DM_HIP(hipMalloc(&ybuf->p, std::max<size_t>(sizeof(T) * (size_t)tb.m * tb.n, 256)));
+  dm_tiled yt{ybuf->p, tb.m, tb.n, tb.mb, tb.nb, tb.dtype};
+  TileView ty(yt);
+  build_potrs_inv(dag, up, ta, tb, ty, [&](int k) { return (void*)plan->inv(k); });
+  dag.pre_hook = [plan, ybuf](Dag* self, hipStream_t s) { return plan->run(s, self->abort_dev); };
+  return 0;
+}
+
 // ------------------------------------------------------------ plan cache
 struct CacheEntry {
   std::unique_ptr<Dag> dag;
@@ -428,16 +501,9 @@ int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B, const dm_pla
   Dag* d = nullptr;
   rc = get_plan(key, {A->base, B->base}, o,
                 [&](Dag& dag) {
-                  // The two trsm! regions of ldiv! recorded into one DAG: the
-                  // datadeps rules order the second after the first per tile.
-                  if (up == 'U') {
-                    build_trsm(dag, 'L', 'U', 'T', 'N', 1.0, ta, tb);
-                    build_trsm(dag, 'L', 'U', 'N', 'N', 1.0, ta, tb);
-                  } else {
-                    build_trsm(dag, 'L', 'L', 'N', 'N', 1.0, ta, tb);
-                    build_trsm(dag, 'L', 'L', 'T', 'N', 1.0, ta, tb);
-                  }
-                  return 0;
+                  // inverse diagonal tiles, then the two sweeps of GEMM tasks
+                  return A->dtype == DM_F64 ? setup_potrs_inv<double>(dag, up, ta, tb)
+                                            : setup_potrs_inv<float>(dag, up, ta, tb);
                 },
                 &d);
   if (rc) return rc;

@@ -126,6 +126,7 @@ _PROTOS = {
     "dm_tiled_identity": (i32, [C.POINTER(dm_tiled), vp]),
     "dm_tiled_col_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp]),
     "dm_tiled_axpy_s2d": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp]),
+    "dm_tiled_scale_diag": (i32, [C.POINTER(dm_tiled), f64, vp]),
     "dm_add_diag_d": (i32, [vp, i64, i64, vp, vp]),
     "dm_tiled_pack": (i32, [C.POINTER(dm_tiled), vp, i64, f64, vp]),
     "dm_tiled_unpack": (i32, [vp, i64, C.POINTER(dm_tiled), vp]),

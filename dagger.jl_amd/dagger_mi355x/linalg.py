@@ -14,13 +14,16 @@ Block re-partitioning follows maybe_copy_buffered (src/array/copy.jl:3-10):
 arguments whose partitioning differs from the requested square Blocks are
 copied into a buffer of that partitioning, processed, and copied back.
 
-Two executions of the same task DAG are available:
-  * via="native": the C++ builders (dm_cholesky_tiled / dm_potrs_tiled /
-    dm_trsm_tiled) record the reference's loop nest natively and cache the
-    plan + hipGraph per matrix (the steady-state fast path);
+Two executions are available:
+  * via="native": the C++ builders (dm_cholesky_tiled / dm_trsm_tiled)
+    record the reference's loop nest natively and cache the plan + hipGraph
+    per matrix (the steady-state fast path); ldiv! (dm_potrs_tiled) runs the
+    same two triangular sweeps as GEMM-only substitution over the inverted
+    diagonal tiles (the arithmetic reordered, equal to 1e-12 relative);
   * via="datadeps": the loop nest is written here with spawn/In/InOut and
     handed to the executor, exactly as the Julia source spawns it.
-Both produce bit-identical results (same tasks, same kernels).
+Cholesky and trsm! give bit-identical results both ways (same tasks, same
+kernels).
 """
 import ctypes as C
 

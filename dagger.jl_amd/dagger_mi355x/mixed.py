@@ -163,18 +163,19 @@ class MixedCholesky:
         self._isolve = aux["isolve"]
         self.refactor()
 
+    # Scaled diagonal shifts A + c u32 diag(A) tried in turn when the FP32
+    # copy of A is not positive definite (kappa(A) u32 near or above 1);
+    # c = 0 first.  The shifted factor is only a preconditioner: refinement
+    # (GMRES-IR when classic refinement stalls) still solves A x = b
+    # (Higham & Pranesh, "Exploiting lower precision arithmetic in solving
+    # symmetric positive definite linear systems", SISC 2021).
+    SHIFTS = (0.0, 4.0, 32.0, 256.0)
+
     def refactor(self):
-        """(Re)build the FP32 factor from the current FP64 A."""
+        """(Re)build the FP32 factor from the current FP64 A; on a failed
+        pivot retry with the next scaled diagonal shift (self.shift: the c
+        that succeeded)."""
         s = self.stream
-        _convert(self.F, self.A, s)
-        t = self.F.tiled()
-        o = self.options.c()
-        call("dm_cholesky_tiled", b"U", C.byref(t), C.c_void_p(self.info.ptr),
-             C.c_void_p(self.ginfo.ptr), C.byref(o), s)
-        info = int(self.info.get(s)[0])
-        if info != 0:
-            raise PosDefException(info)
-        self._isolve.invert(s)
         ta = self.A.tiled()
         call("dm_tiled_norm_inf", C.byref(ta), C.c_void_p(self._norm.ptr),
              C.c_void_p(self._work.ptr), s)
@@ -182,6 +183,23 @@ class MixedCholesky:
         call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self._norm.ptr),
              C.c_size_t(8), s)
         self.normA = float(out[0])
+        t = self.F.tiled()
+        o = self.options.c()
+        u32 = 2.0 ** -24
+        info = 0
+        for c in self.SHIFTS:
+            _convert(self.F, self.A, s)
+            if c:
+                call("dm_tiled_scale_diag", C.byref(t), 1.0 + c * u32, s)
+            call("dm_cholesky_tiled", b"U", C.byref(t), C.c_void_p(self.info.ptr),
+                 C.c_void_p(self.ginfo.ptr), C.byref(o), s)
+            info = int(self.info.get(s)[0])
+            if info == 0:
+                self.shift = c
+                break
+        if info != 0:
+            raise PosDefException(info)
+        self._isolve.invert(s)
 
     # ---------------------------------------------------------------- pieces
     def _buffers(self, nrhs):
@@ -268,7 +286,8 @@ class MixedCholesky:
             info["iterations"] += it
             info["escalated"] = True
             return Xd, info
-        return Xd, dict(iterations=it, berr=berr, method=method, history=hist, escalated=False)
+        return Xd, dict(iterations=it, berr=berr, method=method, history=hist, escalated=False,
+                        shift=self.shift)
 
     def _gmres_ir(self, B, X, tol, max_outer):
         """GMRES-IR: outer FP64 refinement; each correction solves

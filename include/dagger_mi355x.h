@@ -334,6 +334,11 @@ int dm_tiled_convert(const dm_tiled* dst, const dm_tiled* src, dm_stream_t s);
 int dm_tiled_norm_inf(const dm_tiled* A, double* out_dev, double* work_dev, dm_stream_t s);
 /* out_dev[j] = max_i |X_ij| for each column j (FP64). */
 int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s);
+/* A[i,i] *= factor on a square tiled matrix with square blocks (FP64 or
+ * FP32): the scaled diagonal shift A + c u diag(A) that keeps a
+ * low-precision copy of an ill-conditioned SPD matrix positive definite
+ * (Higham & Pranesh, SISC 2021). */
+int dm_tiled_scale_diag(const dm_tiled* A, double factor, dm_stream_t s);
 /* X += D with X FP64 and D FP32 (same shape and blocks). */
 int dm_tiled_axpy_s2d(const dm_tiled* X, const dm_tiled* D, dm_stream_t s);
 

@@ -6,7 +6,22 @@
 #include <string>
 #include <hip/hip_runtime.h>
 #include "dm_internal.h"
+#include "potrs.h"
 static int g_streams = 0, g_events = 0;
+// Optional trace of the stream operations (environment DM_HARNESS_TRACE =
+// output path): one line per launch ("L stream"), event record ("R event
+// stream"), stream wait ("W stream event") and capture begin/end, so a test
+// can rebuild the graph a stream capture of the same issue sequence forms.
+static FILE* trace_file() {
+  static FILE* f = [] {
+    const char* p = getenv("DM_HARNESS_TRACE");
+    return p ? fopen(p, "w") : (FILE*)nullptr;
+  }();
+  return f;
+}
+static void trace_launch(hipStream_t s) {
+  if (FILE* f = trace_file()) fprintf(f, "L %p\n", (void*)s);
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (hipStream_t)(uintptr_t)(0x1000 + ++g_streams); return 0; }
 hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 256; return 0; }
@@ -14,12 +29,27 @@ hipError_t hipStreamDestroy(hipStream_t s) { if (!s) { fprintf(stderr, "destroy 
 hipError_t hipEventCreate(hipEvent_t* e) { *e = (hipEvent_t)(uintptr_t)(0x100000 + ++g_events); return 0; }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t) { return 0; }
-hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { if (!e) abort(); return 0; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) {
+  if (!e) abort();
+  if (FILE* f = trace_file()) fprintf(f, "R %p %p\n", (void*)e, (void*)s);
+  return 0;
+}
 hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 1.f; return 0; }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t e, unsigned) { if (!e) abort(); return 0; }
-hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return 0; }
-hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) { *g = (hipGraph_t)0x777; return 0; }
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
+  if (!e) abort();
+  if (FILE* f = trace_file()) fprintf(f, "W %p %p\n", (void*)s, (void*)e);
+  return 0;
+}
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+  if (FILE* f = trace_file()) fprintf(f, "B %p\n", (void*)s);
+  return 0;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
+  if (FILE* f = trace_file()) { fprintf(f, "E %p\n", (void*)s); fflush(f); }
+  *g = (hipGraph_t)0x777;
+  return 0;
+}
 hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t, hipGraphNode_t*, char*, size_t) { *e = (hipGraphExec_t)0x778; return 0; }
 hipError_t hipGraphGetNodes(hipGraph_t, hipGraphNode_t*, size_t* n) { *n = 0; return 0; }
 hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 0; }
@@ -55,13 +85,13 @@ template <typename T> long gemm_prepare_jobs(GemmJob<T>* jobs, int nj, GemmCfgId
   long t = 0; for (int i = 0; i < nj; ++i) { jobs[i].tiles_m = (jobs[i].M + 127) / 128; jobs[i].tiles_n = (jobs[i].N + 127) / 128; jobs[i].wg_begin = (int)t; t += (long)jobs[i].tiles_m * jobs[i].tiles_n; } return t; }
 template <typename T> bool gemm_job_vec_ok(bool, bool, const GemmJob<T>&) { return true; }
 static long g_launch = 0;
-template <typename T> int gemm_launch(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t, long max_wg) { if (max_wg < 0) abort(); ++g_launch; return 0; }
+template <typename T> int gemm_launch(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t s, long max_wg) { if (max_wg < 0) abort(); trace_launch(s); ++g_launch; return 0; }
 int gemm_wgs_per_cu(GemmCfgId c) { return c.id == 0 ? 1 : 2; }
-template <typename T> int gemm_single(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t) { ++g_launch; return 0; }
-template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t) { if (!inv) abort(); ++g_launch; return 0; }
-template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long max_wg) { if (max_wg < 0) abort(); ++g_launch; return 0; }
-template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t) { if (!ws) abort(); ++g_launch; return 0; }
-template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T*, hipStream_t) { ++g_launch; return 0; }
+template <typename T> int gemm_single(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
+template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t s) { if (!inv) abort(); trace_launch(s); ++g_launch; return 0; }
+template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t s, long max_wg) { if (max_wg < 0) abort(); trace_launch(s); ++g_launch; return 0; }
+template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t s) { if (!ws) abort(); trace_launch(s); ++g_launch; return 0; }
+template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
 bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
   return ta && !tb && !((uintptr_t)j.A & 15) && !((uintptr_t)j.B & 15) && !(j.lda & 3) &&
          !(j.ldb & 3);
@@ -83,8 +113,28 @@ int gemm_split_launch(int terms, int, const GemmJob<float>*, int, const GemmJob<
   return 0;
 }
 template <typename T> int gemv_launch(bool, int, int, T, const T*, int, const T*, int, T, T*, int, const int*, hipStream_t) { ++g_launch; return 0; }
-int fill_i32_launch(int*, int, size_t, hipStream_t) { ++g_launch; return 0; }
-int copy_i32_launch(int*, const int*, size_t, hipStream_t) { ++g_launch; return 0; }
+int fill_i32_launch(int*, int, size_t, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
+int copy_i32_launch(int*, const int*, size_t, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
+// The diagonal-tile inversion: the plan's inverse tiles are real host
+// memory (their addresses become GEMM task operands); run() is one launch.
+template <typename T> TrtriPlan<T>::~TrtriPlan() { free(mem); }
+template <typename T>
+int TrtriPlan<T>::build(bool up, int64_t m_, int nb_, const std::function<const T*(int)>& tile) {
+  upper = up; m = m_; nb = nb_;
+  mt = (int)((m + nb - 1) / nb);
+  for (int k = 0; k < mt; ++k) if (!tile(k)) abort();
+  inv_bytes = sizeof(T) * (size_t)mt * nb * nb;
+  mem = malloc(inv_bytes ? inv_bytes : 1);
+  return 0;
+}
+template <typename T> int TrtriPlan<T>::run(hipStream_t s, const int*) const {
+  if (!mem) abort();
+  trace_launch(s);
+  ++g_launch;
+  return 0;
+}
+template struct TrtriPlan<double>;
+template struct TrtriPlan<float>;
 #define INST(T) \
   template GemmCfgId gemm_pick_cfg<T>(int, int, int, int); \
   template long gemm_prepare_jobs<T>(GemmJob<T>*, int, GemmCfgId); \

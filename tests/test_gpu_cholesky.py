@@ -182,18 +182,31 @@ def test_ldiv_vector_rhs(dm, uplo):
     assert O.backward_error(A, x, b) <= 10 * n * EPS
 
 
-def test_ldiv_datadeps_equals_native(dm):
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_ldiv_datadeps_and_native(dm, uplo):
+    """The datadeps mirror spawns the reference's two trsm! regions
+    (src/array/trsm.jl:90-141); the native dm_dag_build_potrs DAG records
+    the same tasks, so the two are bitwise equal.  The native ldiv! (inverse
+    diagonal tiles, GEMM-only sweeps) reorders the arithmetic and agrees
+    with both, and with the oracle, to the 1e-12 parity bar."""
     n, nb, nrhs = 768, 256, 8
     A = gen.spd(n, 40)
     B = gen.randn(n * nrhs, 41).reshape(nrhs, n).T.copy()
     DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
-    F = dm.cholesky_(DA)
-    outs = []
+    F = dm.cholesky_(DA, uplo=uplo)
+    outs = {}
     for via in ("native", "datadeps"):
         DB = dm.DArray.from_array(B, dm.Blocks(nb, nb))
         dm.ldiv_(F, DB, via=via)
-        outs.append(DB.collect())
-    assert np.array_equal(outs[0], outs[1])
+        outs[via] = DB.collect()
+    DB = dm.DArray.from_array(B, dm.Blocks(nb, nb))
+    dag = dm.Dag.potrs(DA, DB, uplo).plan(dm.DEFAULT_OPTIONS.replace(use_graph=False))
+    dag.run()
+    assert np.array_equal(DB.collect(), outs["datadeps"])
+    ref = O.ldiv(DA.collect(), uplo, B, nb)
+    for X in outs.values():
+        assert np.allclose(X, ref, rtol=1e-12, atol=1e-13)
+        assert O.backward_error(A, X, B) <= 10 * n * EPS
 
 
 def test_reblocking_non_square_blocks(dm):
