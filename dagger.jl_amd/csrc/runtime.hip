@@ -9,14 +9,29 @@
 #include <unordered_map>
 #include "dm_internal.h"
 
+// Device scratch of the allocation-free TRSM entry point (dm_dtrsm_noalloc):
+// the inverted 32x32 diagonal blocks of one leaf solve, which every leaf
+// writes in full before reading it.  One copy per GPU, held until the
+// process ends.  Host code never passes this name to a kernel or a copy: it
+// is a host-side stand-in, and the device address of each GPU's copy comes
+// from hipGetSymbolAddress at initialisation (scratch_dev below).
+constexpr size_t kScratchElems = 8192;  // 64 KiB of FP64
+static __device__ __align__(256) double scratch[kScratchElems];
+
 namespace dm {
 int randn_launch(double* x, size_t count, uint64_t seed, uint64_t offset, hipStream_t s);
+static_assert(trsm_ws_elems<double>() <= kScratchElems,
+              "the static TRSM scratch must hold one leaf's inverted diagonal blocks");
 
 static thread_local std::string g_last_error;
 static std::mutex g_init_mu;
 static bool g_inited = false;
 static int g_device = -1;
 static hipStream_t g_default_stream = nullptr;
+// Device address of `scratch` on each device the library initialised
+// (indexed by device id; written once under g_init_mu).
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices] = {};
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -54,8 +69,23 @@ int ensure_init() {
   DM_HIP(hipStreamCreateWithFlags(&g_default_stream, hipStreamNonBlocking));
   int rc = trsm_init_attributes();
   if (rc) return rc;
+  // this device's copy of the static scratch (the device is current)
+  if (g_device >= kMaxDevices) {
+    set_error("device id beyond the library's per-device table");
+    return DM_HIP_ERROR_BASE + (int)hipErrorInvalidDevice;
+  }
+  void* sp = nullptr;
+  DM_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_scratch_dev[g_device] = sp;
   g_inited = true;
   return 0;
+}
+
+// Device address of the static scratch on the library's device (after
+// ensure_init), or null.
+static double* scratch_dev() {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  return (g_device >= 0 && g_device < kMaxDevices) ? (double*)g_scratch_dev[g_device] : nullptr;
 }
 
 hipStream_t stream_of(dm_stream_t s) { return s ? s->h : g_default_stream; }
@@ -488,6 +518,33 @@ extern "C" int dm_dtrsm(char side, char uplo, char transa, char diag, int m, int
 extern "C" int dm_strsm(char side, char uplo, char transa, char diag, int m, int n, float alpha,
              const float* A, int lda, float* B, int ldb, dm_stream_t s) {
   return trsm_api<float>(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, s);
+}
+
+// dm_dtrsm with its workspace in the static per-GPU scratch instead of the
+// per-stream workspace (which is allocated, and grown with a stream
+// synchronization, on first use): this entry point allocates nothing.  The
+// scratch need does not grow with m or n -- longer solves recurse through
+// the same one-leaf buffer -- so no size is rejected for it.
+extern "C" int dm_dtrsm_noalloc(char side, char uplo, char transa, char diag, int m, int n,
+                                double alpha, const double* A, int lda, double* B, int ldb,
+                                dm_stream_t s) {
+  if (!is_left(side) && !is_right(side)) return arg_fail(1, "side must be 'L' or 'R'");
+  if (!is_upper(uplo) && !is_lower(uplo)) return arg_fail(2, "uplo must be 'U' or 'L'");
+  if (!is_trans(transa) && !is_notrans(transa)) return arg_fail(3, "transa must be 'N','T','C'");
+  if (!is_unit(diag) && !is_nonunit(diag)) return arg_fail(4, "diag must be 'N' or 'U'");
+  if (m < 0) return arg_fail(5, "m < 0");
+  if (n < 0) return arg_fail(6, "n < 0");
+  const int ka = is_left(side) ? m : n;
+  if (lda < (ka > 1 ? ka : 1)) return arg_fail(9, "lda too small");
+  if (ldb < (m > 1 ? m : 1)) return arg_fail(11, "ldb < max(1,m)");
+  DM_INIT();
+  double* ws = scratch_dev();
+  if (!ws) {
+    set_error("dm_dtrsm_noalloc: no static scratch address for this device");
+    return DM_HIP_ERROR_BASE + (int)hipErrorInvalidSymbol;
+  }
+  return trsm_full<double>(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, ws, nullptr,
+                           stream_of(s));
 }
 
 template <typename T>

@@ -93,6 +93,8 @@ _PROTOS = {
     "dm_spotrf": (i32, [c_char, i32, vp, i32, vp, vp]),
     "dm_dtrsm": (i32, [c_char, c_char, c_char, c_char, i32, i32, f64, vp, i32, vp, i32, vp]),
     "dm_strsm": (i32, [c_char, c_char, c_char, c_char, i32, i32, f32, vp, i32, vp, i32, vp]),
+    "dm_dtrsm_noalloc": (i32, [c_char, c_char, c_char, c_char, i32, i32, f64, vp, i32, vp, i32,
+                               vp]),
     "dm_dsyrk": (i32, [c_char, c_char, i32, i32, f64, vp, i32, f64, vp, i32, vp]),
     "dm_ssyrk": (i32, [c_char, c_char, i32, i32, f32, vp, i32, f32, vp, i32, vp]),
     "dm_dgemm": (i32, [c_char, c_char, i32, i32, i32, f64, vp, i32, vp, i32, f64, vp, i32, vp]),

@@ -123,6 +123,21 @@ int dm_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
 int dm_strsm(char side, char uplo, char transa, char diag, int m, int n,
              float alpha, const float* A, int lda, float* B, int ldb,
              dm_stream_t s);
+/* dm_dtrsm that allocates nothing (no hipMalloc, hipFree or allocating
+ * library call on any call).  No counterpart in the reference: it is for
+ * callers that cannot allocate -- work under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation.  Same arguments, results
+ * bitwise equal to dm_dtrsm.  Its scratch (the inverted 32x32 diagonal
+ * blocks of one leaf solve, 64 KiB) is a static per-GPU device array, held
+ * until the process ends; its size does not depend on m or n (longer solves
+ * recurse through the same buffer), so no size is rejected for it.  Every
+ * call on one GPU shares that array: calls that overlap in time on one GPU
+ * (on different streams, or from different threads) race and must not be
+ * issued; calls ordered on one stream are safe.  Each call fully writes the
+ * part of the scratch it reads, so nothing carries over between calls. */
+int dm_dtrsm_noalloc(char side, char uplo, char transa, char diag, int m, int n,
+                     double alpha, const double* A, int lda, double* B, int ldb,
+                     dm_stream_t s);
 
 /* BLAS syrk: C := alpha op(A) op(A)^T + beta C on the uplo triangle only.
  * trans 'T': op(A) = A^T (A is k x n); 'N': op(A) = A (A is n x k).

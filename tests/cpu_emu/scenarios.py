@@ -394,6 +394,18 @@ def capped():
 def quick():
     """A few seconds of every kernel family: the CPU suite's default group."""
     rng = np.random.default_rng(2)
+    # the allocation-free TRSM: static scratch, bitwise equal to dm_dtrsm,
+    # two calls in a row (the second smaller), a solve longer than one leaf
+    for (side, uplo, tr, m, n) in (("L", "U", "T", 300, 20), ("R", "L", "N", 17, 70)):
+        ka = m if side == "L" else n
+        A = 0.1 * rng.standard_normal((ka, ka)) + np.diag(np.full(ka, 2.0))
+        B = rng.standard_normal((m, n))
+        dA, d1, d2 = Dev(A), Dev(B), Dev(B)
+        call("dm_dtrsm", side.encode(), uplo.encode(), tr.encode(), b"N", m, n, 0.5, dA.p, ka,
+             d1.p, m, None)
+        call("dm_dtrsm_noalloc", side.encode(), uplo.encode(), tr.encode(), b"N", m, n, 0.5,
+             dA.p, ka, d2.p, m, None)
+        check(np.array_equal(d1.get(), d2.get()), ("dtrsm_noalloc", side, uplo, tr))
     # norm2 / chkfinite pass over ragged tiles, whole and triangular parts
     for dt in (np.float64, np.float32):
         A = rng.standard_normal((75, 75)).astype(dt)
