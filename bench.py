@@ -295,19 +295,26 @@ def batched_leg(dm, count, seed, reps=3):
     return out
 
 
-def serving_leg(dm, rate, count, seed, workers=6):
-    """Serving form vs one-at-a-time plain FP64 on the same Poisson trace."""
+def serving_leg(dm, rates, count, seed, workers=6):
+    """Serving form vs one-at-a-time plain FP64 on the same Poisson traces,
+    one trace of `count` requests per arrival rate.  Both keep their device
+    buffers and cached plans; the server also overlaps requests on worker
+    streams, caches factors and refines FP32 factors for n >= 4096."""
     from dagger_mi355x import serving
-    tr1 = serving.make_trace(rate=rate, count=count, seed=seed)
-    tr2 = serving.make_trace(rate=rate, count=count, seed=seed)
-    srv = serving.Server(workers=workers, budget_bytes=200e9)
-    s = srv.run(tr1)
-    srv.close()
-    b = serving.serve_one_at_a_time(tr2)
-    return {"rate": rate, "requests": count, "workers": workers, "server": s,
-            "one_at_a_time_fp64": b,
+    out = {"requests_per_rate": count, "workers": workers, "rates": {}}
+    for i, rate in enumerate(rates):
+        tr1 = serving.make_trace(rate=rate, count=count, seed=seed + i)
+        tr2 = serving.make_trace(rate=rate, count=count, seed=seed + i)
+        srv = serving.Server(workers=workers, budget_bytes=200e9)
+        s = srv.run(tr1)
+        srv.close()
+        b = serving.serve_one_at_a_time(tr2)
+        out["rates"][str(int(rate))] = {
+            "server": s, "one_at_a_time_fp64": b,
             "solves_ratio": round(s["solves_per_s"] / b["solves_per_s"], 3),
             "p99_ratio": round(s["p99_ms"] / b["p99_ms"], 3)}
+        log(f"serving at {rate:g} req/s: {out['rates'][str(int(rate))]}")
+    return out
 
 
 def cpu_baseline(dm, n, nb, seed):
@@ -351,9 +358,10 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=32768,
                     help="N of the mixed-precision leg (0 disables it)")
-    ap.add_argument("--serving-requests", type=int, default=120,
-                    help="requests in the serving leg (0 disables it)")
-    ap.add_argument("--serving-rate", type=float, default=1000.0)
+    ap.add_argument("--serving-requests", type=int, default=500,
+                    help="requests per arrival rate in the serving leg (0 disables it)")
+    ap.add_argument("--serving-rates", type=str, default="200,500,1000,2000",
+                    help="comma-separated arrival rates (req/s) of the serving leg")
     ap.add_argument("--batched-count", type=int, default=4096,
                     help="systems in the batched leg (0 disables it)")
     a = ap.parse_args()
@@ -434,8 +442,8 @@ def main():
         log(f"mixed: {mixed_res}")
     serving_res = None
     if a.serving_requests > 0:
-        serving_res = serving_leg(dm, a.serving_rate, a.serving_requests, a.seed + 11)
-        log(f"serving: {serving_res}")
+        rates = [float(x) for x in a.serving_rates.split(",") if x]
+        serving_res = serving_leg(dm, rates, a.serving_requests, a.seed + 11)
     batched_res = None
     if a.batched_count > 0:
         batched_res = batched_leg(dm, a.batched_count, a.seed + 13)

@@ -52,7 +52,7 @@ void mat_span(const void* p, long rows, long cols, long ld, size_t es, uintptr_t
 
 int Dag::add(const dm_task& t) {
   if (planned) return arg_fail(1, "dag already planned");
-  if (t.op < DM_OP_POTRF || t.op > DM_OP_GEMV) return arg_fail(2, "unknown op");
+  if (t.op < DM_OP_POTRF || t.op > DM_OP_LASWP) return arg_fail(2, "unknown op");
   if (t.dtype != DM_F64 && t.dtype != DM_F32) return arg_fail(2, "unknown dtype");
   TaskRec r;
   r.t = t;
@@ -103,6 +103,24 @@ int Dag::add(const dm_task& t) {
       push(t.c, 1, t.n, t.ldc, t.access[2]);
       r.cost = 1.0 * t.n * t.n;
       break;
+    case DM_OP_GETRF:
+    case DM_OP_LASWP: {
+      if (t.m < 0 || t.n < 0 || t.k <= 0) return arg_fail(2, "getrf/laswp: bad panel shape");
+      // the block-column region: m x n elements, tiles packed contiguously
+      push(t.c, (long)t.m * t.n, 1, (long)t.m * t.n, t.access[2] ? t.access[2] : DM_INOUT);
+      const long npiv = t.op == DM_OP_GETRF ? std::min(t.m, t.n) : t.lda;
+      auto push_int = [&](const void* p, long count, int mode) {
+        if (!p || mode == 0 || count <= 0) return;
+        Access a;
+        mat_span(p, count, 1, count, 4, a.lo, a.hi);
+        a.mode = mode;
+        r.acc.push_back(a);
+      };
+      push_int(t.b, npiv, t.access[1] ? t.access[1] : (t.op == DM_OP_GETRF ? DM_OUT : DM_IN));
+      if (t.op == DM_OP_GETRF) push_int(t.info, 1, DM_INOUT);
+      r.cost = t.op == DM_OP_GETRF ? 1.0 * t.m * t.n * t.n : 1.0 * t.m * t.n;
+      break;
+    }
   }
   const int id = (int)tasks.size();
   // ---- dependencies (datadeps rules on overlapping spans)
@@ -445,6 +463,19 @@ int Dag::build_device_tables() {
   // allocated while a run is issued or captured.
   task_ws_off.assign(tasks.size(), (size_t)-1);
   size_t ws_bytes = 0;
+  // LU panels share one workspace: they are ordered by their data (each
+  // panel reads the updates of the previous one) and by the info word.
+  size_t getrf_bytes = 0;
+  for (const TaskRec& r : tasks)
+    if (r.t.op == DM_OP_GETRF)
+      getrf_bytes = std::max(getrf_bytes, r.t.dtype == DM_F64
+                                              ? getrf_panel_ws_bytes<double>(r.t.m, r.t.n)
+                                              : getrf_panel_ws_bytes<float>(r.t.m, r.t.n));
+  if (getrf_bytes) {
+    for (size_t i = 0; i < tasks.size(); ++i)
+      if (tasks[i].t.op == DM_OP_GETRF) task_ws_off[i] = 256 + inv_bytes;
+    ws_bytes = (getrf_bytes + 255) / 256 * 256;
+  }
   for (Group& G : groups) {
     const bool needs = G.op == DM_OP_TRSV ||
                        (G.op == DM_OP_TRSM && !G.batched_trsm);
@@ -551,6 +582,13 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       case DM_OP_GEMV:
         rc = gemv_launch<T>(is_trans(t.transa), t.m, t.n, (T)t.alpha, (const T*)t.a, t.lda,
                             (const T*)t.b, t.ldb, (T)t.beta, (T*)t.c, t.ldc, ab, s);
+        break;
+      case DM_OP_GETRF:
+        rc = getrf_panel<T>((T*)t.c, t.m, t.n, t.k, t.ldb, (int*)t.b, t.info, abort_dev,
+                            t.transa == 'P' || t.transa == 'p', ws, s);
+        break;
+      case DM_OP_LASWP:
+        rc = laswp_tiled<T>((T*)t.c, t.m, t.n, t.k, (const int*)t.b, t.lda, t.ldb, ab, s);
         break;
       case DM_OP_TRSV:
         if (t.ldc == 1)

@@ -228,6 +228,23 @@ template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
                int global_offset, T* ws, hipStream_t s);
 
+// ------------------------------------------------------------ LU (lu.hip)
+// Workspace bytes of one panel factorization (contiguous panel, TRSM
+// inverse blocks, pivots).
+template <typename T>
+size_t getrf_panel_ws_bytes(int64_t Mp, int W);
+// LU (partial pivoting when `pivot`) of a tiled panel of Mp rows in tiles of
+// mb rows and W columns starting at `tiles`; ipiv: global 1-based pivot rows
+// (row0 = the panel's first global row), *info: first zero pivot (global).
+template <typename T>
+int getrf_panel(T* tiles, int64_t Mp, int W, int mb, int64_t row0, int* ipiv, int* info,
+                int* abort, bool pivot, void* ws, hipStream_t s);
+// The swaps ipiv[0..npiv) (global 1-based rows) applied in order to the
+// rows of a tiled block-column region whose first row is global row0.
+template <typename T>
+int laswp_tiled(T* base, int64_t Mr, int W, int mb, const int* ipiv, int npiv, int64_t row0,
+                const int* abort, hipStream_t s);
+
 // ------------------------------------------------------------ vectors
 // y := alpha op(A) x + beta y (BLAS gemv; an empty inner dimension still
 // scales y by beta).

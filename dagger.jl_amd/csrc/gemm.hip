@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
     int tm, tn;
     locate_tile(jobs, njobs, single, rg.t, job, tm, tn, false);
     const int M = job.M, N = job.N, K = job.K;
-    const int m0 = tm * 32;
+    const int m0 = tm * 32, n0 = tn * 16 * NB;
     v4 acc[2][NB];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
       }
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const int j = 16 * b + lr;
+        const int j = n0 + 16 * b + lr;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           const int k = kbase + s;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
             const T v = ((acc[rb][b][r] + red[0][q][lane]) + red[1][q][lane]) + red[2][q][lane];
             // result lane l, item r: row Mfma<T>::row(l/16, r), column l%16 of the 16x16 block
             const int m = m0 + 16 * rb + Mfma<T>::row(lk, r);
-            const int n = 16 * b + lr;
+            const int n = n0 + 16 * b + lr;
             bool ok = m < M && n < N;
             if (tri == 1) ok = ok && m <= n;
             else if (tri == 2) ok = ok && m >= n;
@@ -429,8 +429,15 @@ template <typename T>
 GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs) {
   (void)K;
   const long nj = njobs > 0 ? njobs : 1;
+  // A launch of at most 64 columns too small to fill the device (fewer than
+  // 64 tiles of 64 x 64, e.g. the critical-path products of the ldiv! sweeps)
+  // runs the K-split skinny kernel on 32 x 16 tiles: four times the
+  // workgroups, each with a quarter of the columns.  An element's arithmetic
+  // does not depend on the column split.
+  if (N <= 64 && (long)((M + 63) / 64) * ((N + 63) / 64) * nj < 64)
+    return GemmCfgId{GEMM_CFG_SKINNY, 32, 16};
   // few columns: the K-split skinny kernel (at 64 columns the 64 x 64 tile
-  // measured slightly faster)
+  // measured slightly faster in large launches)
   if (N <= 48) return GemmCfgId{GEMM_CFG_SKINNY, 32, 16 * ((N + 15) / 16)};
   if (sizeof(T) == 8) {
     // Largest tile whose launch still puts about one workgroup on every CU.

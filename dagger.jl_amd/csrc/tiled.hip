@@ -155,6 +155,64 @@ void build_cholesky(Dag& d, const TileView& A, char uplo, int* info) {
   }
 }
 
+void add_getrf(Dag& d, const TileView& A, int k, bool pivot, int* ipiv, int* info, int64_t rows) {
+  dm_task t = make(DM_OP_GETRF, A.dtype);
+  t.transa = pivot ? 'P' : 'N';
+  t.m = (int)rows;
+  t.n = A.cols(k);
+  t.k = A.mb;
+  t.c = A.ptr(k, k);
+  t.b = ipiv + (size_t)k * A.nb;  // chunk k of the Blocks(nb) pivot vector
+  t.ldb = k * A.mb;
+  t.info = info;
+  t.access[1] = DM_OUT;
+  t.access[2] = DM_INOUT;
+  d.add(t);
+}
+
+void add_laswp(Dag& d, const TileView& A, int k, int j, const int* ipiv) {
+  dm_task t = make(DM_OP_LASWP, A.dtype);
+  t.m = (int)(A.m - (int64_t)k * A.mb);
+  t.n = A.cols(j);
+  t.k = A.mb;
+  t.c = A.ptr(k, j);
+  t.b = ipiv + (size_t)k * A.nb;
+  t.lda = (int)std::min<int64_t>(A.m - (int64_t)k * A.mb, A.cols(k));
+  t.ldb = k * A.mb;
+  t.access[1] = DM_IN;
+  t.access[2] = DM_INOUT;
+  d.add(t);
+}
+
+// lu!(A, RowMaximum()) (src/array/lu.jl:289-371): per block column k the
+// panel with partial pivoting (tiles k..mt-1), its row swaps in every other
+// block column, then trsm!('L','L','N','U') across the row and the GEMM
+// trailing update.  lu!(A, NoPivot()) (src/array/lu.jl:10-40): the diagonal
+// tile's LU without pivoting, trsm!('R','U','N','N') down the column,
+// trsm!('L','L','N','U') across the row, GEMM.
+void build_lu(Dag& d, const TileView& A, bool pivot, int* ipiv, int* info) {
+  const int mt = A.mt, nt = A.nt, dt = A.dtype;
+  for (int k = 0; k < std::min(mt, nt); ++k) {
+    if (pivot) {
+      add_getrf(d, A, k, true, ipiv, info, A.m - (int64_t)k * A.mb);
+      for (int j = 0; j < nt; ++j)
+        if (j != k) add_laswp(d, A, k, j, ipiv);
+    } else {
+      add_getrf(d, A, k, false, ipiv, info, A.rows(k));
+      for (int m = k + 1; m < mt; ++m)
+        add_trsm(d, dt, 'R', 'U', 'N', 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(m, k), A.rows(m),
+                 A.cols(k), A.ld(m));
+    }
+    for (int j = k + 1; j < nt; ++j) {
+      add_trsm(d, dt, 'L', 'L', 'N', 'U', 1.0, A.ptr(k, k), A.ld(k), A.ptr(k, j), A.rows(k),
+               A.cols(j), A.ld(k));
+      for (int i = k + 1; i < mt; ++i)
+        add_gemm(d, dt, 'N', 'N', A.rows(i), A.cols(j), A.cols(k), -1.0, A.ptr(i, k), A.ld(i),
+                 A.ptr(k, j), A.ld(k), 1.0, A.ptr(i, j), A.ld(i));
+    }
+  }
+}
+
 // src/array/trsm.jl:88-190, 0-based (Dagger's index (Bmt-k)+1 is Bmt-k here
 // with k counted from 1).
 void build_trsm(Dag& d, char side, char uplo, char trans, char diag, double alpha,
@@ -556,6 +614,56 @@ int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A, con
                 &d);
   if (rc) return rc;
   return d->run(stream_of(s));
+}
+
+int dm_getrf_tiled(int pivot, const dm_tiled* A, int* ipiv_dev, int* info_dev,
+                   const dm_plan_opts* opts, dm_stream_t s) {
+  if (pivot != 0 && pivot != 1) return arg_fail(1, "pivot must be 0 (NoPivot) or 1 (RowMaximum)");
+  int rc = check_tiled(A, 2);
+  if (rc) return rc;
+  if (A->mb != A->nb) return arg_fail(2, "tiles must be square (Blocks(nb, nb))");
+  if (!ipiv_dev && A->m > 0 && A->n > 0) return arg_fail(3, "ipiv is NULL");
+  if (!info_dev) return arg_fail(4, "info is NULL");
+  rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t st = stream_of(s);
+  rc = fill_i32_launch(info_dev, 0, 1, st);
+  if (rc) return rc;
+  if (A->m == 0 || A->n == 0) return 0;
+  const dm_plan_opts o = opts_or_default(opts);
+  TileView tv(*A);
+  const std::string key = key_of(pivot ? "getrfP" : "getrfN", {A->base, ipiv_dev, info_dev},
+                                 {A->m, A->n, A->mb, A->dtype}, o);
+  Dag* d = nullptr;
+  rc = get_plan(key, {A->base, ipiv_dev, info_dev}, o,
+                [&](Dag& dag) {
+                  build_lu(dag, tv, pivot == 1, ipiv_dev, info_dev);
+                  return 0;
+                },
+                &d);
+  if (rc) return rc;
+  return d->run(st);
+}
+
+int dm_tiled_laswp(const dm_tiled* B, const int* ipiv_dev, int npiv, dm_stream_t s) {
+  int rc = check_tiled(B, 1);
+  if (rc) return rc;
+  if (npiv < 0 || npiv > B->m) return arg_fail(3, "npiv must be in [0, rows of B]");
+  if (!ipiv_dev && npiv > 0) return arg_fail(2, "ipiv is NULL");
+  rc = ensure_init();
+  if (rc) return rc;
+  if (B->n == 0 || npiv == 0) return 0;
+  TileView tb(*B);
+  hipStream_t st = stream_of(s);
+  for (int j = 0; j < tb.nt; ++j) {
+    rc = B->dtype == DM_F64
+             ? laswp_tiled<double>((double*)tb.ptr(0, j), B->m, tb.cols(j), B->mb, ipiv_dev, npiv,
+                                   0, nullptr, st)
+             : laswp_tiled<float>((float*)tb.ptr(0, j), B->m, tb.cols(j), B->mb, ipiv_dev, npiv,
+                                  0, nullptr, st);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 int dm_dag_build_cholesky(dm_dag_t* dag, char uplo, const dm_tiled* A, int* info_dev) {

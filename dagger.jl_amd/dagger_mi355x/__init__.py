@@ -19,13 +19,14 @@ from .datadeps import (In, Out, InOut, BLAS, LAPACK, potrf_checked, spawn, spawn
 from .linalg import (Cholesky, PosDefException, cholesky, cholesky_, ldiv_, ldiv, ldiv_triangular_, mul_, mul,
                      trsm_, trsv_,
                      ishermitian, norm2, chkfinite, DEFAULT_OPTIONS)
+from .lu import LU, RowMaximum, NoPivot, SingularException, lu, lu_
 
 __all__ = [
     "Blocks", "DArray", "DeviceBuffer", "DeviceInt", "Tile", "zeros", "randn", "spd_generate",
     "In", "Out", "InOut", "BLAS", "LAPACK", "potrf_checked", "spawn", "spawn_datadeps",
     "PlanOptions", "SEQUENTIAL", "Dag", "Cholesky", "PosDefException", "cholesky",
     "cholesky_", "ldiv_", "ldiv", "ldiv_triangular_", "mul_", "mul", "trsm_", "trsv_", "ishermitian", "norm2", "chkfinite",
-    "DEFAULT_OPTIONS", "DMError",
+    "DEFAULT_OPTIONS", "DMError", "LU", "RowMaximum", "NoPivot", "SingularException", "lu", "lu_",
     "load_library",
     "init",
 ]

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libdagger_mi355x.so")
 
 DM_F64, DM_F32 = 0, 1
 DM_OP_POTRF, DM_OP_TRSM, DM_OP_SYRK, DM_OP_GEMM, DM_OP_TRSV, DM_OP_GEMV = 1, 2, 3, 4, 5, 6
+DM_OP_GETRF, DM_OP_LASWP = 7, 8
 DM_IN, DM_OUT, DM_INOUT = 1, 2, 3
 DM_HIP_ERROR_BASE = 1000
 
@@ -112,6 +113,8 @@ _PROTOS = {
     "dm_dag_get_stats": (i32, [vp, C.POINTER(dm_dag_stats)]),
     "dm_dag_profile": (i32, [vp, C.POINTER(dm_group_profile), C.POINTER(i32)]),
     "dm_dag_build_cholesky": (i32, [C.POINTER(vp), c_char, C.POINTER(dm_tiled), vp]),
+    "dm_getrf_tiled": (i32, [i32, C.POINTER(dm_tiled), vp, vp, C.POINTER(dm_plan_opts), vp]),
+    "dm_tiled_laswp": (i32, [C.POINTER(dm_tiled), vp, i32, vp]),
     "dm_dag_build_potrs": (i32, [C.POINTER(vp), c_char, C.POINTER(dm_tiled), C.POINTER(dm_tiled)]),
     "dm_cholesky_tiled": (i32, [c_char, C.POINTER(dm_tiled), vp, vp, C.POINTER(dm_plan_opts), vp]),
     "dm_potrs_tiled": (i32, [c_char, C.POINTER(dm_tiled), C.POINTER(dm_tiled),

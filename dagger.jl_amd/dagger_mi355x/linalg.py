@@ -371,7 +371,13 @@ def trsv_(uplo, trans, diag, alpha, A, b, options=None):
 
 
 def ldiv_(F, B, via="native", options=None):
-    """`ldiv!(C::Cholesky, B)` (src/array/linalg.jl:352-405), B overwritten."""
+    """`ldiv!(C::Cholesky, B)` (src/array/linalg.jl:352-405), B overwritten.
+    An LU factorization dispatches to `ldiv!(::LU, B)` (lu.py)."""
+    if not isinstance(F, Cholesky):
+        from .lu import LU as _LU, ldiv_ as _lu_ldiv_
+        if isinstance(F, _LU):
+            return _lu_ldiv_(F, B, options=options)
+        raise TypeError(f"ldiv! of a {type(F).__name__} is not supported")
     options = options or DEFAULT_OPTIONS
     pA = F.factors
     bs = min(pA.partitioning.blocksize)
@@ -417,7 +423,13 @@ def ldiv(F, B, via="native", options=None):
     the whole factor (src/array/linalg.jl:208-220). As the reference's LU
     override does (src/array/linalg.jl:221-246), B is staged once into a
     DArray with the factor's square blocking, so `ldiv!`
-    (src/array/linalg.jl:352-405) runs without any copy of the factor."""
+    (src/array/linalg.jl:352-405) runs without any copy of the factor.
+    An LU factorization dispatches to its own `\\` (lu.py)."""
+    if not isinstance(F, Cholesky):
+        from .lu import LU as _LU, ldiv as _lu_ldiv
+        if isinstance(F, _LU):
+            return _lu_ldiv(F, B, options=options)
+        raise TypeError(f"\\ of a {type(F).__name__} is not supported")
     A = F.factors
     if A.m != A.n:
         raise ValueError("the factor must be square")

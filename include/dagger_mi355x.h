@@ -188,7 +188,7 @@ int dm_dgemv(char trans, int m, int n, double alpha, const double* A, int lda,
  * (src/datadeps/aliasing.jl:920-1060): tiles stay resident. */
 
 enum { DM_OP_POTRF = 1, DM_OP_TRSM = 2, DM_OP_SYRK = 3, DM_OP_GEMM = 4,
-       DM_OP_TRSV = 5, DM_OP_GEMV = 6 };
+       DM_OP_TRSV = 5, DM_OP_GEMV = 6, DM_OP_GETRF = 7, DM_OP_LASWP = 8 };
 enum { DM_IN = 1, DM_OUT = 2, DM_INOUT = 3 };
 
 /* One tile task.  Operand slots by op:
@@ -196,6 +196,16 @@ enum { DM_IN = 1, DM_OUT = 2, DM_INOUT = 3 };
  *   TRSM : a = A, c = B          GEMM: a = A, b = B, c = C
  *   SYRK : a = A, c = C          GEMV: a = A, b = x, c = y
  *   TRSV : a = A, c = x
+ *   GETRF: the LU panel of a tiled block column (src/array/lu.jl:160-204):
+ *          c = its first tile (the panel: m rows in tiles of k rows, n
+ *          columns, tiles packed below each other), b = int pivots (Out,
+ *          global 1-based rows, ldb = global row of the panel's first
+ *          row), info = int (InOut: the first zero pivot, global, 1-based),
+ *          transa 'P' with partial pivoting, 'N' without
+ *   LASWP: a block column's row swaps (swaprows_trail!, src/array/lu.jl:
+ *          239-262): c = its tile at the panel's row (m rows in tiles of k,
+ *          n columns), b = the panel's pivots (In), lda = their count,
+ *          ldb = global row of c's first row
  * m,n,k follow the BLAS call (TRSM: m,n of B; SYRK: n,k; POTRF: n in n).
  * access[i] is DM_IN/DM_OUT/DM_INOUT for slots a,b,c (0 = unused). */
 typedef struct dm_task {
@@ -316,6 +326,21 @@ int dm_tiled_norm2(const dm_tiled* A, int part, double* out_dev, int* nonfinite_
  * resident factor.  B is tiled with Blocks(nb, nb) rows matching A. */
 int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B,
                    const dm_plan_opts* opts, dm_stream_t s);
+
+/* Tiled LU: lu!(A, RowMaximum()) (pivot = 1, src/array/lu.jl:289-371) or
+ * lu!(A, NoPivot()) (pivot = 0, src/array/lu.jl:10-40) of an m x n tiled
+ * matrix with square blocks, the reference's task DAG (panel factorization,
+ * row swaps, trsm!, gemm!) run through the executor with a cached plan.
+ * ipiv_dev (min(m, n) ints): the reference's ipiv, global 1-based rows
+ * (chunk k of Blocks(nb) holds panel k's pivots; NoPivot: 1..min(m, n)).
+ * info_dev: 0, or the global 1-based column of the first exactly zero
+ * pivot (LAPACK getrf's info; the factorization still completes). */
+int dm_getrf_tiled(int pivot, const dm_tiled* A, int* ipiv_dev, int* info_dev,
+                   const dm_plan_opts* opts, dm_stream_t s);
+/* Rows of a tiled B permuted by the swaps ipiv[0..npiv) (global 1-based
+ * rows) applied in order: _apply_ipiv_rows! of ldiv!(::LU, B)
+ * (src/array/linalg.jl:266-310). */
+int dm_tiled_laswp(const dm_tiled* B, const int* ipiv_dev, int npiv, dm_stream_t s);
 
 /* Record (unplanned) the task DAG of the tiled Cholesky / ldiv! on resident
  * tiles into a new dag handle -- the same tasks dm_cholesky_tiled and

@@ -21,6 +21,10 @@ What it restates (paths inside JuliaParallel/Dagger.jl):
   * LAPACK.chkfinite(::DArray)          src/array/linalg.jl:127-148
   * ishermitian(::DArray) with is_cross_hermitian
                                         src/array/linalg.jl:38-48, 90-124
+  * lu!(A, RowMaximum()) / lu!(A, NoPivot()) and ldiv!(::LU, B)
+                                        src/array/lu.jl:5-55, 160-204,
+                                        239-262, 289-371;
+                                        src/array/linalg.jl:248-310
 The tile kernels are the same LAPACK/BLAS routines the reference calls on
 its CPU path (LAPACK.potrf!, BLAS.trsm!/syrk!/gemm!/gemv!/trsv!), here
 through scipy's bundled OpenBLAS.
@@ -619,3 +623,113 @@ def ishermitian(A, nb):
             if not lhs <= rhs:
                 return False
     return True
+
+
+# ---------------------------------------------------------------- LU
+def lu_panel(panel_tiles, k, mb, ipiv_chunk, info):
+    """factor_panel_lapack! (src/array/lu.jl:160-204): stack the panel's
+    tiles, LAPACK getrf, write the tiles back; ipiv_chunk gets the global
+    1-based pivot rows, info[0] the global first zero pivot if unset."""
+    stacked = np.vstack(panel_tiles)
+    f = lapack.dgetrf if stacked.dtype == np.float64 else lapack.sgetrf
+    lu, piv, linfo = f(stacked, overwrite_a=0)
+    base = k * mb
+    if info[0] == 0 and linfo > 0:
+        info[0] = base + int(linfo)
+    ipiv_chunk[: len(piv)] = base + piv + 1       # scipy: 0-based
+    r0 = 0
+    for t in panel_tiles:
+        t[...] = lu[r0:r0 + t.shape[0]]
+        r0 += t.shape[0]
+
+
+def swaprows_trail(top, blocks, ipiv_chunk, k, mb):
+    """swaprows_trail! over tiles (k..mt-1) of one block column
+    (src/array/lu.jl:239-262): pivot p of panel k swaps row p of the top
+    tile with row ipiv[p] (global, 1-based) of the block column, in order
+    p = 1..npiv (the reference spawns one task per target tile in order,
+    which gives the same sequence)."""
+    col = np.vstack(blocks)
+    for p, g in enumerate(ipiv_chunk):
+        q = int(g) - 1 - k * mb
+        if q != p:
+            col[[p, q]] = col[[q, p]]
+    r0 = 0
+    for t in blocks:
+        t[...] = col[r0:r0 + t.shape[0]]
+        r0 += t.shape[0]
+
+
+def lu_nopivot_tile(A, info):
+    """generic_lufact!(A, NoPivot()) on one tile (Doolittle, no row
+    exchanges): unit lower L below the diagonal, U on and above; a zero
+    pivot sets info (1-based, local) and its column is not scaled."""
+    n = min(A.shape)
+    for j in range(n):
+        d = A[j, j]
+        if d == 0:
+            if info[0] == 0:
+                info[0] = j + 1
+            continue
+        A[j + 1:, j] /= d
+        A[j + 1:, j + 1:] -= np.outer(A[j + 1:, j], A[j, j + 1:])
+
+
+def lu(A, nb, pivot=True):
+    """lu!(A, RowMaximum()) (src/array/lu.jl:289-371) or lu!(A, NoPivot())
+    (:10-40) on A tiled Blocks(nb, nb); returns (factors, ipiv (global
+    1-based), info)."""
+    T = TiledMatrix(np.array(A, copy=True), nb, nb)
+    Ac = T.chunks
+    mt, nt = T.mt, T.nt
+    m, n = T.m, T.n
+    ipiv = np.arange(1, min(m, n) + 1)
+    info = [0]
+    for k in range(min(mt, nt)):
+        if pivot:
+            chunk = ipiv[k * nb:k * nb + min(m - k * nb, T.cols[k])]
+            lu_panel([Ac[i, k] for i in range(k, mt)], k, nb, chunk, info)
+            for j in list(range(k)) + list(range(k + 1, nt)):
+                swaprows_trail(Ac[k, j], [Ac[i, j] for i in range(k, mt)], chunk, k, nb)
+        else:
+            li = [0]
+            lu_nopivot_tile(Ac[k, k], li)
+            if info[0] == 0 and li[0]:
+                info[0] = k * nb + li[0]
+            for i in range(k + 1, mt):
+                tile_trsm("R", "U", "N", "N", 1.0, Ac[k, k][:T.cols[k], :T.cols[k]], Ac[i, k])
+        for j in range(k + 1, nt):
+            r = T.rows[k]
+            tile_trsm("L", "L", "N", "U", 1.0, Ac[k, k][:r, :r], Ac[k, j])
+            for i in range(k + 1, mt):
+                tile_gemm("N", "N", -1.0, Ac[i, k], Ac[k, j], 1.0, Ac[i, j])
+    return T.collect(), ipiv, info[0]
+
+
+def lu_ldiv(F, ipiv, B, nb):
+    """ldiv!(::LU, B) (src/array/linalg.jl:248-310): the pivots applied to
+    B's rows in order, then the unit lower and the upper tiled solves."""
+    X = np.array(B, dtype=F.dtype, copy=True)
+    vec = X.ndim == 1
+    X = X.reshape(X.shape[0], -1)
+    for p, g in enumerate(ipiv):
+        q = int(g) - 1
+        if q != p:
+            X[[p, q]] = X[[q, p]]
+    X = trsm("L", "L", "N", "U", 1.0, F, X, nb)
+    X = trsm("L", "U", "N", "N", 1.0, F, X, nb)
+    return X.ravel() if vec else X
+
+
+def lu_unpack(F, ipiv):
+    """(L, U, P) with P A = L U from the packed factors and ipiv."""
+    m, n = F.shape
+    r = min(m, n)
+    L = np.tril(F[:, :r], -1) + np.eye(m, r)
+    U = np.triu(F[:r, :])
+    perm = np.arange(m)
+    for p, g in enumerate(ipiv):
+        q = int(g) - 1
+        perm[[p, q]] = perm[[q, p]]
+    P = np.eye(m)[perm]
+    return L, U, P

@@ -40,6 +40,13 @@ int main() {
       dm_dag_destroy(d);
     }
     if (dm_gemm_tiled('N', 'N', -1.0, &t, &b, 1.0, &b, &o, nullptr)) return 8;
+    // LU with and without pivoting, square and ragged
+    std::vector<int> ipiv(1536);
+    for (int piv = 0; piv < 2; ++piv) {
+      if (dm_getrf_tiled(piv, &t, ipiv.data(), info, &o, nullptr)) return 12;
+      if (dm_getrf_tiled(piv, &rag, ipiv.data(), info + 1, &o, nullptr)) return 13;
+    }
+    if (dm_tiled_laswp(&b, ipiv.data(), 1536, nullptr)) return 14;
     for (char side : {'L', 'R'})
       for (char up : {'U', 'L'})
         for (char tr : {'N', 'T'}) {

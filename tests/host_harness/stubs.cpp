@@ -135,6 +135,25 @@ template <typename T> int TrtriPlan<T>::run(hipStream_t s, const int*) const {
 }
 template struct TrtriPlan<double>;
 template struct TrtriPlan<float>;
+template <typename T> size_t getrf_panel_ws_bytes(int64_t Mp, int W) {
+  return sizeof(T) * (size_t)Mp * W + 65536;
+}
+template <typename T>
+int getrf_panel(T* tiles, int64_t Mp, int W, int mb, int64_t, int* ipiv, int* info, int*, bool,
+                void* ws, hipStream_t s) {
+  if (!tiles || !ipiv || !info || !ws || Mp <= 0 || W <= 0 || mb <= 0) abort();
+  trace_launch(s);
+  ++g_launch;
+  return 0;
+}
+template <typename T>
+int laswp_tiled(T* base, int64_t Mr, int W, int mb, const int* ipiv, int npiv, int64_t,
+                const int*, hipStream_t s) {
+  if (!base || !ipiv || Mr <= 0 || W <= 0 || mb <= 0 || npiv < 0) abort();
+  trace_launch(s);
+  ++g_launch;
+  return 0;
+}
 #define INST(T) \
   template GemmCfgId gemm_pick_cfg<T>(int, int, int, int); \
   template long gemm_prepare_jobs<T>(GemmJob<T>*, int, GemmCfgId); \
@@ -145,7 +164,10 @@ template struct TrtriPlan<float>;
   template int trsm_launch<T>(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long); \
   template int trsm_full<T>(char, char, char, char, int, int, T, const T*, int, T*, int, T*, const int*, hipStream_t); \
   template int potrf_full<T>(bool, int, T*, int, int*, int*, int, T*, hipStream_t); \
-  template int gemv_launch<T>(bool, int, int, T, const T*, int, const T*, int, T, T*, int, const int*, hipStream_t);
+  template int gemv_launch<T>(bool, int, int, T, const T*, int, const T*, int, T, T*, int, const int*, hipStream_t); \
+  template size_t getrf_panel_ws_bytes<T>(int64_t, int); \
+  template int getrf_panel<T>(T*, int64_t, int, int, int64_t, int*, int*, int*, bool, void*, hipStream_t); \
+  template int laswp_tiled<T>(T*, int64_t, int, int, const int*, int, int64_t, const int*, hipStream_t);
 INST(double)
 INST(float)
 }  // namespace dm

@@ -358,3 +358,80 @@ def test_chkfinite_restatement():
         B[129, 69] = bad  # last element of the ragged corner chunk
         with pytest.raises(ValueError, match="Inf or NaN"):
             O.chkfinite(B, 32)
+
+
+@pytest.mark.parametrize("m,n,nb", [(128, 128, 64), (200, 200, 64), (300, 256, 64),
+                                    (200, 300, 64), (130, 130, 32), (64, 64, 64)])
+def test_lu_rowmaximum_matches_dense_lapack(m, n, nb):
+    """lu!(A, RowMaximum()) restatement (src/array/lu.jl:289-371): the tiled
+    panel/swap/trsm/gemm nest gives dense LAPACK getrf's pivots exactly and
+    its factors to rounding, P A = L U, and ldiv! solves the system."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(m * 3 + n)
+    A = rng.standard_normal((m, n))
+    F, ipiv, info = O.lu(A, nb, pivot=True)
+    lu, piv = sl.lu_factor(A)
+    assert info == 0
+    assert np.array_equal(ipiv, piv + 1)
+    assert np.abs(F - lu).max() <= 1e-11 * np.abs(lu).max()
+    L, U, P = O.lu_unpack(F, ipiv)
+    assert np.abs(P @ A - L @ U).max() <= 1e-13 * np.abs(A).max() * max(m, n)
+    if m == n:
+        b = rng.standard_normal((m, 3))
+        assert np.allclose(O.lu_ldiv(F, ipiv, b, nb), np.linalg.solve(A, b), rtol=1e-9,
+                           atol=1e-10)
+
+
+def test_lu_nopivot_and_singular_closed_forms():
+    """NoPivot (src/array/lu.jl:10-40) on a diagonally dominant matrix equals
+    the brute-force Doolittle LU of the whole matrix; ipiv is 1..n; an
+    all-ones matrix is singular with info 2 (LAPACK: the second pivot is
+    exactly zero); a diagonal matrix factors to itself with identity pivots."""
+    rng = np.random.default_rng(9)
+    n, nb = 96, 32
+    A = rng.standard_normal((n, n)) + n * np.eye(n)
+    F, ipiv, info = O.lu(A, nb, pivot=False)
+    D = A.copy()
+    for j in range(n):                      # brute force, no pivoting
+        for i in range(j + 1, n):
+            D[i, j] /= D[j, j]
+            D[i, j + 1:] -= D[i, j] * D[j, j + 1:]
+    assert np.allclose(F, D, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(ipiv, np.arange(1, n + 1))
+    _, _, info1 = O.lu(np.ones((128, 128)), 64, pivot=True)
+    assert info1 == 2
+    Dg = np.diag(np.arange(1.0, 65.0))
+    F2, ip2, info2 = O.lu(Dg, 16, pivot=True)
+    assert info2 == 0 and np.array_equal(ip2, np.arange(1, 65)) and np.array_equal(F2, Dg)
+
+
+def test_lu_planted_mistake_is_caught():
+    """Skipping the row swaps of the block columns left of the panel (a
+    plausible restatement error: swaprows_trail! over 1:k-1 as well as
+    k+1:nt) breaks P A = L U, which the dense comparison catches."""
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((192, 192))
+    orig_lu = O.lu
+
+    def lu_without_left_swaps(A, nb, pivot=True):
+        T = O.TiledMatrix(np.array(A, copy=True), nb, nb)
+        Ac = T.chunks
+        mt, nt = T.mt, T.nt
+        ipiv = np.arange(1, min(T.m, T.n) + 1)
+        info = [0]
+        for k in range(min(mt, nt)):
+            chunk = ipiv[k * nb:k * nb + min(T.m - k * nb, T.cols[k])]
+            O.lu_panel([Ac[i, k] for i in range(k, mt)], k, nb, chunk, info)
+            for j in range(k + 1, nt):            # the left columns are skipped
+                O.swaprows_trail(Ac[k, j], [Ac[i, j] for i in range(k, mt)], chunk, k, nb)
+            for j in range(k + 1, nt):
+                O.tile_trsm("L", "L", "N", "U", 1.0, Ac[k, k], Ac[k, j])
+                for i in range(k + 1, mt):
+                    O.tile_gemm("N", "N", -1.0, Ac[i, k], Ac[k, j], 1.0, Ac[i, j])
+        return T.collect(), ipiv, info[0]
+    F, ipiv, _ = lu_without_left_swaps(A, 64)
+    L, U, P = O.lu_unpack(F, ipiv)
+    assert np.abs(P @ A - L @ U).max() > 1e-3
+    F, ipiv, _ = orig_lu(A, 64)
+    L, U, P = O.lu_unpack(F, ipiv)
+    assert np.abs(P @ A - L @ U).max() < 1e-12
