@@ -522,15 +522,20 @@ int Dag::build_device_tables() {
 template <typename T>
 int Dag::launch_group_t(const Group& G, hipStream_t s) {
   const int* ab = abort_dev;
+  // the critical stream's launches of a look-ahead schedule raise their
+  // waves' priority over the bulk update sharing the CUs
+  const int prio = (opts.lookahead > 0 && opts.num_streams >= 2 && G.crit == 0) ? 1 : 0;
   if (G.op == DM_OP_GEMM) {
     if (G.njobs == 0) return 0;
     GemmJob<T> dummy{};
+    dummy.prio = prio;
     return gemm_launch<T>(G.ta, G.tb, G.vec, G.cfg, (const GemmJob<T>*)G.table_dev, G.njobs, dummy,
                           G.total_wg, ab, s);
   }
   if (G.op == DM_OP_TRSM && G.batched_trsm) {
     for (const InvSrc& src : G.inv_src) {
-      int rc = trinv_blocks_launch<T>(G.shape, (const T*)src.A, src.lda, src.L, (T*)src.dev, ab, s);
+      int rc = trinv_blocks_launch<T>(G.shape, (const T*)src.A, src.lda, src.L, (T*)src.dev, ab, s,
+                                      prio);
       if (rc) return rc;
     }
     const bool prof = opts.profile && !capturing;
@@ -540,10 +545,12 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
       if (st.leaf) {
         TrsmJob<T> dummy{};
         dummy.L = st.maxL;
+        dummy.prio = prio;
         rc = trsm_launch<T>(G.shape, (const TrsmJob<T>*)st.table_dev, st.njobs, dummy, st.total_wg,
                             ab, s);
       } else {
         GemmJob<T> dummy{};
+        dummy.prio = prio;
         rc = gemm_launch<T>(st.ta, st.tb, st.vec, st.cfg, (const GemmJob<T>*)st.table_dev, st.njobs,
                             dummy, st.total_wg, ab, s);
       }

@@ -89,7 +89,16 @@ struct GemmJob {
   int tri;
   int tiles_m, tiles_n;  // workgroup tiles along M and N (filled by host)
   int wg_begin;          // first workgroup of this job in a batched launch
+  int prio;              // launch argument (`single`) only: 1 for critical-path
+                         // launches, whose waves raise their issue priority
 };
+
+// Critical-path kernels (the diagonal POTRF, the panel TRSM and the next
+// panel's update of a look-ahead schedule) share CUs with the bulk trailing
+// update: raising their waves' instruction-issue priority (s_setprio, per
+// wave; the HIP streams keep the default priority) lets their short
+// dependent steps go ahead of the bulk's MFMA waves on the same SIMD.
+__device__ __forceinline__ void raise_wave_priority() { __builtin_amdgcn_s_setprio(3); }
 
 // Kernel configuration chosen by the host for a launch.
 struct GemmCfgId {
@@ -151,6 +160,7 @@ struct TrsmJob {
   int lda, ldb;
   T alpha;
   int wg_begin;
+  int prio;  // launch argument (`single`) only: 1 for critical-path launches
 };
 
 struct TrsmShape {
@@ -175,7 +185,7 @@ constexpr size_t trsm_ws_elems() {
 // Inverts the 32x32 diagonal blocks of Tm into inv (ceil(L/32)*1024 elems).
 template <typename T>
 int trinv_blocks_launch(const TrsmShape& sh, const T* A, int lda, int L, T* inv,
-                        const int* abort, hipStream_t s);
+                        const int* abort, hipStream_t s, int prio = 0);
 // Batched / single leaf solve (L <= TRSM_LEAF; jobs' inv point at the
 // inverted blocks of their own triangle).  single.L: longest L of the launch.
 template <typename T>

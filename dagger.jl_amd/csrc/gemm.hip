@@ -270,6 +270,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __
   typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC> G;
   __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
   if (abort && *abort) return;
+  if (single.prio) raise_wave_priority();
   constexpr bool packed = BM == BN;  // SYRK jobs enumerate only triangle tiles
   for (TileRange r = tile_range(total); r.t < r.end; r.t += r.step) {
     GemmJob<T> job;
@@ -301,6 +302,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
   typedef typename Mfma<T>::v4 v4;
   __shared__ T red[3][2 * NB * 4][64];
   if (abort && *abort) return;
+  if (single.prio) raise_wave_priority();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   for (TileRange rg = tile_range(total); rg.t < rg.end; rg.t += rg.step) {
@@ -402,6 +404,7 @@ __global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __
   static_assert(2 * (G::A_ELEMS + G::B_ELEMS) >= 129, "LDS reuse for the block step");
   if (abort && *abort) return;
   if (info && *info) return;
+  raise_wave_priority();  // always on the critical path of a factorization
   const int t = blockIdx.x;
   GemmJob<T> jb;
   int tm, tn;

@@ -211,6 +211,7 @@ __global__ __launch_bounds__(1024) void gemm_split_kernel(const GemmJob<float>* 
   typedef SplitGemm<TERMS, BM, 128> G;
   __shared__ __attribute__((aligned(16))) __bf16 smem[2 * G::STAGE];
   if (abort && *abort) return;
+  if (single.prio) raise_wave_priority();
   constexpr bool packed = BM == 128;  // SYRK jobs enumerate only triangle tiles
   for (TileRange r = tile_range(total); r.t < r.end; r.t += r.step) {
     GemmJob<float> job;

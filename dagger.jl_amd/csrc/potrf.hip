@@ -34,6 +34,7 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n
   __shared__ T pv;
   if (abort && *abort) return;
   if (info && *info) return;
+  raise_wave_priority();  // always on the critical path of a factorization
   potrf_diag_block<T, UPPER, PW>(A, n, lda, j0, info, abort, global_base, rowb, &pv);
 }
 
@@ -51,6 +52,7 @@ __global__ __launch_bounds__(64) void potrf_row_kernel(T* __restrict__ A, int n,
   __shared__ T invd[PW];
   if (abort && *abort) return;
   if (info && *info) return;
+  raise_wave_priority();
   const int t = threadIdx.x;
   if (t < PW) {
     // lane = column c; U: element (r, c) = A[j0+r, j0+c]; L: A[j0+c, j0+r]

@@ -53,10 +53,12 @@ __device__ __forceinline__ T tm_at(const T* __restrict__ A, int lda, int i, int 
 template <typename T, bool LOWER, bool TTRANS, bool UNIT>
 __global__ __launch_bounds__(32) void trinv_blocks_kernel(const T* __restrict__ A, int lda, int L,
                                                           T* __restrict__ inv,
-                                                          const int* __restrict__ abort) {
+                                                          const int* __restrict__ abort,
+                                                          int prio) {
   __shared__ T tb[TRSM_BS][TRSM_BS + 1];  // Tm_bb, row-major
   __shared__ T xs[TRSM_BS][TRSM_BS + 1];  // column c of the inverse in xs[.][c]
   if (abort && *abort) return;
+  if (prio) raise_wave_priority();
   const int b = blockIdx.x;
   const int c = threadIdx.x;
   const int r0 = b * TRSM_BS;
@@ -100,6 +102,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
   typedef typename MfmaT<T>::v4 v4;
   DM_DYNAMIC_LDS(dm_trsm_smem);
   if (abort && *abort) return;
+  if (single.prio) raise_wave_priority();
   // A launch may have fewer workgroups than units of 64 vectors: each
   // workgroup then loops over its share (waves never synchronise, so a wave
   // simply moves on to its next unit).
@@ -238,12 +241,12 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
 // ------------------------------------------------------------ host side
 template <typename T>
 int trinv_blocks_launch(const TrsmShape& sh, const T* A, int lda, int L, T* inv,
-                        const int* abort, hipStream_t s) {
+                        const int* abort, hipStream_t s, int prio) {
   const int nblk = (L + TRSM_BS - 1) / TRSM_BS;
   if (nblk <= 0) return 0;
 #define DM_TRINV(LO, TT, UN)                                                          \
   hipLaunchKernelGGL((trinv_blocks_kernel<T, LO, TT, UN>), dim3(nblk), dim3(TRSM_BS), 0, s, A, \
-                     lda, L, inv, abort)
+                     lda, L, inv, abort, prio)
   if (sh.lower) {
     if (sh.ttrans) { if (sh.unit) DM_TRINV(true, true, true); else DM_TRINV(true, true, false); }
     else { if (sh.unit) DM_TRINV(true, false, true); else DM_TRINV(true, false, false); }
@@ -361,9 +364,9 @@ int trsm_full(char side, char uplo, char transa, char diag, int m, int n, T alph
 }
 
 template int trinv_blocks_launch<double>(const TrsmShape&, const double*, int, int, double*,
-                                         const int*, hipStream_t);
+                                         const int*, hipStream_t, int);
 template int trinv_blocks_launch<float>(const TrsmShape&, const float*, int, int, float*,
-                                        const int*, hipStream_t);
+                                        const int*, hipStream_t, int);
 template int trsm_launch<double>(const TrsmShape&, const TrsmJob<double>*, int,
                                  const TrsmJob<double>&, long, const int*, hipStream_t, long);
 template int trsm_launch<float>(const TrsmShape&, const TrsmJob<float>*, int,

@@ -161,6 +161,7 @@ inline int readlane(int v, int l) {
 #define __builtin_amdgcn_wave_barrier() ::emu::wave_exchange(nullptr, 0, nullptr)
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)
+#define __builtin_amdgcn_s_setprio(p) ((void)0)
 #define __builtin_amdgcn_readlane(v, l) ::emu::readlane((v), (l))
 #define hipLaunchKernelGGL(K, G, B, SH, ST, ...) \
   ::emu::launch(dim3(G), dim3(B), (size_t)(SH), (ST), [=]() { K(__VA_ARGS__); })

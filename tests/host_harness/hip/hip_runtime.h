@@ -43,3 +43,8 @@ hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipDeviceSynchronize();
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t);
+// device-function qualifiers of the shared internal header (no device code
+// is compiled here)
+#define __device__
+#define __forceinline__ inline
+#define __builtin_amdgcn_s_setprio(p) ((void)0)

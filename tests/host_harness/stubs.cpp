@@ -88,7 +88,7 @@ static long g_launch = 0;
 template <typename T> int gemm_launch(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t s, long max_wg) { if (max_wg < 0) abort(); trace_launch(s); ++g_launch; return 0; }
 int gemm_wgs_per_cu(GemmCfgId c) { return c.id == 0 ? 1 : 2; }
 template <typename T> int gemm_single(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
-template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t s) { if (!inv) abort(); trace_launch(s); ++g_launch; return 0; }
+template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, int, T* inv, const int*, hipStream_t s, int) { if (!inv) abort(); trace_launch(s); ++g_launch; return 0; }
 template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t s, long max_wg) { if (max_wg < 0) abort(); trace_launch(s); ++g_launch; return 0; }
 template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t s) { if (!ws) abort(); trace_launch(s); ++g_launch; return 0; }
 template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
@@ -160,7 +160,7 @@ int laswp_tiled(T* base, int64_t Mr, int W, int mb, const int* ipiv, int npiv, i
   template bool gemm_job_vec_ok<T>(bool, bool, const GemmJob<T>&); \
   template int gemm_launch<T>(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t, long); \
   template int gemm_single<T>(bool, bool, int, int, int, T, const T*, int, const T*, int, T, T*, int, int, const int*, hipStream_t); \
-  template int trinv_blocks_launch<T>(const TrsmShape&, const T*, int, int, T*, const int*, hipStream_t); \
+  template int trinv_blocks_launch<T>(const TrsmShape&, const T*, int, int, T*, const int*, hipStream_t, int); \
   template int trsm_launch<T>(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t, long); \
   template int trsm_full<T>(char, char, char, char, int, int, T, const T*, int, T*, int, T*, const int*, hipStream_t); \
   template int potrf_full<T>(bool, int, T*, int, int*, int*, int, T*, hipStream_t); \
