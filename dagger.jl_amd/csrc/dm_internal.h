@@ -100,6 +100,24 @@ struct GemmJob {
 // dependent steps go ahead of the bulk's MFMA waves on the same SIMD.
 __device__ __forceinline__ void raise_wave_priority() { __builtin_amdgcn_s_setprio(3); }
 
+// Global-memory views of pointers that reach a kernel inside job structs.
+// The compiler cannot prove such a pointer global and emits flat accesses,
+// which count against lgkmcnt as well as vmcnt: the LDS wait in front of a
+// k-step's MFMAs would then also wait for the next k-step's prefetch and
+// expose its whole memory latency.  Through these views the accesses are
+// global_load/global_store, tracked by vmcnt only.
+#ifndef DM_GLOBAL_AS
+#define DM_GLOBAL_AS __attribute__((address_space(1)))
+#endif
+template <typename P>
+__device__ __forceinline__ const DM_GLOBAL_AS P* gptr(const P* p) {
+  return (const DM_GLOBAL_AS P*)p;
+}
+template <typename P>
+__device__ __forceinline__ DM_GLOBAL_AS P* gptr(P* p) {
+  return (DM_GLOBAL_AS P*)p;
+}
+
 // Kernel configuration chosen by the host for a launch.
 struct GemmCfgId {
   int id;     // index into the configuration table

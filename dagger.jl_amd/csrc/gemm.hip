@@ -88,11 +88,11 @@ struct GemmKernel {
         const bool ok = gm < M && gk < K;
         const T* p = A + (size_t)gm * lda + gk;
         if (VEC == 2) {
-          vt v = ok ? *(const vt*)p : vt{};
+          vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) ra[i][q] = reinterpret_cast<const T*>(&v)[q];
         } else {
-          ra[i][0] = ok ? *p : T(0);
+          ra[i][0] = ok ? *gptr(p) : T(0);
         }
       } else {   // A[m + k*lda], contiguous along m
         gm = m0 + (e % (BM / VEC)) * VEC;
@@ -100,11 +100,11 @@ struct GemmKernel {
         const bool ok = gm < M && gk < K;
         const T* p = A + (size_t)gk * lda + gm;
         if (VEC == 2) {
-          vt v = ok ? *(const vt*)p : vt{};
+          vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) ra[i][q] = reinterpret_cast<const T*>(&v)[q];
         } else {
-          ra[i][0] = ok ? *p : T(0);
+          ra[i][0] = ok ? *gptr(p) : T(0);
         }
       }
     }
@@ -136,11 +136,11 @@ struct GemmKernel {
         const bool ok = gn < N && gk < K;
         const T* p = B + (size_t)gn * ldb + gk;
         if (VEC == 2) {
-          vt v = ok ? *(const vt*)p : vt{};
+          vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) rb[i][q] = reinterpret_cast<const T*>(&v)[q];
         } else {
-          rb[i][0] = ok ? *p : T(0);
+          rb[i][0] = ok ? *gptr(p) : T(0);
         }
       } else {    // B[n + k*ldb], contiguous along n
         gn = n0 + (e % (BN / VEC)) * VEC;
@@ -148,11 +148,11 @@ struct GemmKernel {
         const bool ok = gn < N && gk < K;
         const T* p = B + (size_t)gk * ldb + gn;
         if (VEC == 2) {
-          vt v = ok ? *(const vt*)p : vt{};
+          vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) rb[i][q] = reinterpret_cast<const T*>(&v)[q];
         } else {
-          rb[i][0] = ok ? *p : T(0);
+          rb[i][0] = ok ? *gptr(p) : T(0);
         }
       }
     }
@@ -238,27 +238,76 @@ struct GemmKernel {
     }
 
     // Epilogue: lane holds C[m][n] with m = .. + lr, n = .. + row(lk, r).
+    // Per 16-row strip, all old values of C are loaded before any store, so
+    // the strip's loads are in flight together (one memory latency per
+    // strip, not one per element).
     const T alpha = job.alpha, beta = job.beta;
     const int tri = job.tri;
+    // A tile wholly inside C and inside the triangle (nearly every tile of a
+    // big update) stores without per-element branches, which would make
+    // every store wait for the ones before it.
+    const bool whole = m0 + BM <= M && n0 + BN <= N &&
+                       (tri == 0 || (tri == 1 ? m0 + BM - 1 <= n0 : m0 >= n0 + BN - 1));
+    if (whole) {
+      const size_t ldc = job.ldc;
+      auto cbase = gptr(job.C + (size_t)(n0 + wn0) * ldc + m0 + wm0 + lr);
+      if (beta == T(0)) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              cbase[(size_t)(j * 16 + Mfma<T>::row(lk, r)) * ldc + i * 16] = alpha * acc[i][j][r];
+      } else {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          T old[FN][4];
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              old[j][r] = cbase[(size_t)(j * 16 + Mfma<T>::row(lk, r)) * ldc + i * 16];
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              cbase[(size_t)(j * 16 + Mfma<T>::row(lk, r)) * ldc + i * 16] =
+                  alpha * acc[i][j][r] + beta * old[j][r];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
       const int m = m0 + wm0 + i * 16 + lr;
+      T old[FN][4];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
+      for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int n = n0 + wn0 + j * 16 + Mfma<T>::row(lk, r);
-          bool ok = m < M && n < N;
-          if (tri == 1) ok = ok && m <= n;
-          else if (tri == 2) ok = ok && m >= n;
-          if (ok) {
-            T* c = job.C + (size_t)n * job.ldc + m;
+          const bool ok = in_c(m, n, M, N, tri);
+          old[j][r] = (ok && beta != T(0)) ? *gptr(job.C + (size_t)n * job.ldc + m) : T(0);
+        }
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn0 + j * 16 + Mfma<T>::row(lk, r);
+          if (in_c(m, n, M, N, tri)) {
             const T v = alpha * acc[i][j][r];
-            *c = (beta == T(0)) ? v : v + beta * *c;
+            *gptr(job.C + (size_t)n * job.ldc + m) = (beta == T(0)) ? v : v + beta * old[j][r];
           }
         }
-      }
     }
+  }
+  // (m, n) inside C and inside the requested triangle (tri 1 upper, 2 lower)
+  static __device__ __forceinline__ bool in_c(int m, int n, int M, int N, int tri) {
+    bool ok = m < M && n < N;
+    if (tri == 1) ok = ok && m <= n;
+    else if (tri == 2) ok = ok && m >= n;
+    return ok;
   }
 };
 
@@ -327,7 +376,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
         for (int s = 0; s < 4; ++s) {
           const int k = kbase + s;
           const bool ok = i < M && k < K;
-          a[rb][s] = ok ? (TA ? job.A[(size_t)i * job.lda + k] : job.A[(size_t)k * job.lda + i])
+          a[rb][s] = ok ? (TA ? gptr(job.A)[(size_t)i * job.lda + k] : gptr(job.A)[(size_t)k * job.lda + i])
                         : T(0);
         }
       }
@@ -338,7 +387,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
         for (int s = 0; s < 4; ++s) {
           const int k = kbase + s;
           const bool ok = j < N && k < K;
-          x[b][s] = ok ? (TB ? job.B[(size_t)k * job.ldb + j] : job.B[(size_t)j * job.ldb + k])
+          x[b][s] = ok ? (TB ? gptr(job.B)[(size_t)k * job.ldb + j] : gptr(job.B)[(size_t)j * job.ldb + k])
                        : T(0);
         }
       }
@@ -377,7 +426,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
             if (tri == 1) ok = ok && m <= n;
             else if (tri == 2) ok = ok && m >= n;
             if (ok) {
-              T* c = job.C + (size_t)n * job.ldc + m;
+              auto c = gptr(job.C + (size_t)n * job.ldc + m);
               *c = (beta == T(0)) ? alpha * v : alpha * v + beta * *c;
             }
           }

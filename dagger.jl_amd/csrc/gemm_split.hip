@@ -64,8 +64,8 @@ struct SplitGemm {
   static __device__ __forceinline__ f4 load4(const float* __restrict__ X, int ld, int r, int k,
                                              int R, int K) {
     if (r >= R) return f4{0, 0, 0, 0};
-    const float* p = X + (size_t)r * ld + k;
-    if (k + 3 < K) return *(const f4*)p;
+    const auto p = gptr(X + (size_t)r * ld + k);
+    if (k + 3 < K) return *(const DM_GLOBAL_AS f4*)p;
     f4 v{0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -181,6 +181,31 @@ struct SplitGemm {
     // lane holds C[m][n] with m = .. + lr, n = .. + 4*lk + r
     const float alpha = job.alpha, beta = job.beta;
     const int tri = job.tri;
+    // A tile wholly inside C and the triangle stores without per-element
+    // branches; per 16-row strip the old values are loaded before any store.
+    const bool whole = m0 + BM <= job.M && n0 + BN <= job.N &&
+                       (tri == 0 || (tri == 1 ? m0 + BM - 1 <= n0 : m0 >= n0 + BN - 1));
+    if (whole) {
+      const size_t ldc = job.ldc;
+      auto cbase = gptr(job.C + (size_t)(n0 + wn0 + 4 * lk) * ldc + m0 + wm0 + lr);
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        float old[FN][4];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            old[j][r] = beta == 0.0f ? 0.0f : cbase[(size_t)(j * 16 + r) * ldc + i * 16];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float v = alpha * acc[i][j][r];
+            cbase[(size_t)(j * 16 + r) * ldc + i * 16] = beta == 0.0f ? v : v + beta * old[j][r];
+          }
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
       const int m = m0 + wm0 + i * 16 + lr;
@@ -193,7 +218,7 @@ struct SplitGemm {
           if (tri == 1) ok = ok && m <= n;
           else if (tri == 2) ok = ok && m >= n;
           if (ok) {
-            float* c = job.C + (size_t)n * job.ldc + m;
+            auto c = gptr(job.C + (size_t)n * job.ldc + m);
             const float v = alpha * acc[i][j][r];
             *c = (beta == 0.0f) ? v : v + beta * *c;
           }

@@ -36,8 +36,8 @@ __global__ __launch_bounds__(64) void trtri_leaf_kernel(const TrtriLeaf* __restr
   __shared__ T x[64][65];
   if (abort && *abort) return;
   const TrtriLeaf L = leaves[blockIdx.x];
-  const T* __restrict__ src = (const T*)L.src;
-  T* __restrict__ dst = (T*)L.dst;
+  const auto src = gptr((const T*)L.src);
+  const auto dst = gptr((T*)L.dst);
   const int n = L.n, r = threadIdx.x;
   for (int cc = 0; cc < 64; ++cc) {  // thread r reads row r: coalesced per column
     T v = (r == cc) ? T(1) : T(0);

@@ -45,7 +45,8 @@ template <> struct MfmaT<float> {
 
 template <typename T, bool TTRANS>
 __device__ __forceinline__ T tm_at(const T* __restrict__ A, int lda, int i, int k) {
-  return TTRANS ? A[(size_t)i * lda + k] : A[(size_t)k * lda + i];
+  const auto g = gptr(A);
+  return TTRANS ? g[(size_t)i * lda + k] : g[(size_t)k * lda + i];
 }
 
 // inv[b*1024 + c*32 + r] = (Tm_bb^-1)(r, c); rows/cols past L act as identity.
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
   const int j0 = local * TRSM_VPW + wave * 16;  // first vector of this wave
   if (j0 >= NV) continue;                       // waves never synchronise with each other
   const T* __restrict__ A = job.A;
-  T* __restrict__ B = job.B;
+  const auto B = gptr(job.B);
   const int lda = job.lda, ldb = job.ldb;
   const T alpha = job.alpha;
   const int gj = j0 + lr;  // this lane's vector in the MFMA output layout
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(const TrsmJob<T>* __restrict_
       }
     trsm_wave_sync();
     // ---- X_I = inv(Tm_II) bt
-    const T* __restrict__ iv = job.inv + (size_t)b * TRSM_BS * TRSM_BS;
+    const auto iv = gptr(job.inv + (size_t)b * TRSM_BS * TRSM_BS);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       v4 xa = v4{0, 0, 0, 0}, xb = v4{0, 0, 0, 0};

@@ -162,6 +162,7 @@ inline int readlane(int v, int l) {
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)
 #define __builtin_amdgcn_s_setprio(p) ((void)0)
+#define DM_GLOBAL_AS  /* host memory: no device address spaces */
 #define __builtin_amdgcn_readlane(v, l) ::emu::readlane((v), (l))
 #define hipLaunchKernelGGL(K, G, B, SH, ST, ...) \
   ::emu::launch(dim3(G), dim3(B), (size_t)(SH), (ST), [=]() { K(__VA_ARGS__); })

@@ -48,3 +48,4 @@ const char* hipGetErrorString(hipError_t);
 #define __device__
 #define __forceinline__ inline
 #define __builtin_amdgcn_s_setprio(p) ((void)0)
+#define DM_GLOBAL_AS
