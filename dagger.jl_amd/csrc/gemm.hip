@@ -181,6 +181,18 @@ struct GemmKernel {
     return TB ? Bs[k * SNB + n] : Bs[n * SK + k];
   }
 
+  // the MFMA operands of sub-step kk (k = 4kk + lk) for this wave's 16-row
+  // blocks of op(A) and 16-column blocks of op(B)
+  static __device__ __forceinline__ void read_frags(const T* __restrict__ As,
+                                                    const T* __restrict__ Bs, int kk, int wm0,
+                                                    int wn0, int lr, int lk, T (&af)[FM],
+                                                    T (&bf)[FN]) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i) af[i] = frag_a(As, wm0 + i * 16 + lr, kk * 4 + lk);
+#pragma unroll
+    for (int j = 0; j < FN; ++j) bf[j] = frag_b(Bs, kk * 4 + lk, wn0 + j * 16 + lr);
+  }
+
   static __device__ __forceinline__ void run(const GemmJob<T>& job, int tm, int tn, T* smem) {
     T* As0 = smem;
     T* Bs0 = smem + A_ELEMS;
@@ -209,32 +221,50 @@ struct GemmKernel {
     }
     __syncthreads();
     const int lr = lane & 15, lk = lane >> 4;
+    // Software pipeline over the BK/4 sub-steps of a k-step: the fragments
+    // of sub-step kk+1 are read from LDS before the MFMAs of sub-step kk; the
+    // next k-step's tile goes from registers into the spare LDS buffer
+    // halfway through; the one barrier comes before the last sub-step's
+    // MFMAs, so the next k-step's first fragments are read while those run
+    // and the matrix pipe does not wait on LDS latency at every k-step.
+    // LDS hazards: buffer kt&1 is read in k-step kt up to its barrier; buffer
+    // (kt+1)&1 is written in k-step kt after the barrier of k-step kt-1
+    // (which ended every read of it) and read after the barrier of k-step kt.
+    constexpr int KS = BK / 4;
+    T af[FM], bf[FN];
+    if (nk > 0) read_frags(As0, Bs0, 0, wm0, wn0, lr, lk, af, bf);
     for (int kt = 0; kt < nk; ++kt) {
       const bool odd = kt & 1;
       const T* As = odd ? As1 : As0;
       const T* Bs = odd ? Bs1 : Bs0;
+      T* An = odd ? As0 : As1;
+      T* Bn = odd ? Bs0 : Bs1;
       const bool more = kt + 1 < nk;
       if (more) {
         load_a(job.A, job.lda, m0, (kt + 1) * BK, M, K, ra);
         load_b(job.B, job.ldb, n0, (kt + 1) * BK, N, K, rb);
       }
 #pragma unroll
-      for (int kk = 0; kk < BK / 4; ++kk) {
-        T af[FM], bf[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = frag_a(As, wm0 + i * 16 + lr, kk * 4 + lk);
-#pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = frag_b(Bs, kk * 4 + lk, wn0 + j * 16 + lr);
+      for (int kk = 0; kk < KS; ++kk) {
+        T an[FM] = {}, bn[FN] = {};
+        if (kk + 1 < KS) read_frags(As, Bs, kk + 1, wm0, wn0, lr, lk, an, bn);
+        if (kk == KS / 2 && more) {
+          store_a(An, ra);
+          store_b(Bn, rb);
+        }
+        if (kk == KS - 1) {
+          __syncthreads();
+          if (more) read_frags(An, Bn, 0, wm0, wn0, lr, lk, an, bn);
+        }
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mac(bf[j], af[i], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = an[i];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = bn[j];
       }
-      if (more) {
-        store_a(odd ? As0 : As1, ra);
-        store_b(odd ? Bs0 : Bs1, rb);
-      }
-      __syncthreads();
     }
 
     // Epilogue: lane holds C[m][n] with m = .. + lr, n = .. + row(lk, r).
