@@ -353,7 +353,9 @@ int Dag::build_device_tables() {
         }
         G.njobs = (int)jobs.size();
         if (!G.njobs) return 0;
-        G.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, G.njobs);
+        G.vec = true;
+        for (auto& j : jobs) G.vec = G.vec && gemm_job_vec_ok<T>(ta, tb, j);
+        G.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, G.njobs, G.vec);
         if constexpr (std::is_same<T, float>::value) {
           if (opts.f32_update == 3 || opts.f32_update == 6) {
             bool split = true;
@@ -362,8 +364,6 @@ int Dag::build_device_tables() {
           }
         }
         G.total_wg = gemm_prepare_jobs<T>(jobs.data(), G.njobs, G.cfg);
-        G.vec = true;
-        for (auto& j : jobs) G.vec = G.vec && gemm_job_vec_ok<T>(ta, tb, j);
         align(64);
         G.table_off = blob.size();
         const unsigned char* p = (const unsigned char*)jobs.data();
@@ -434,7 +434,9 @@ int Dag::build_device_tables() {
             for (const GemmJob<T>& g : st.gjobs) rec.flops += 2.0 * g.M * g.N * g.K;
             rec.ta = st.ta;
             rec.tb = st.tb;
-            rec.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, rec.njobs);
+            rec.vec = true;
+            for (const GemmJob<T>& g : st.gjobs) rec.vec = rec.vec && gemm_job_vec_ok<T>(st.ta, st.tb, g);
+            rec.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, rec.njobs, rec.vec);
             if constexpr (std::is_same<T, float>::value) {
               // the solve's updates take the same BF16-split arithmetic as
               // the factorization's when their operands allow it
@@ -445,8 +447,6 @@ int Dag::build_device_tables() {
               }
             }
             rec.total_wg = gemm_prepare_jobs<T>(st.gjobs.data(), rec.njobs, rec.cfg);
-            rec.vec = true;
-            for (const GemmJob<T>& g : st.gjobs) rec.vec = rec.vec && gemm_job_vec_ok<T>(st.ta, st.tb, g);
             const unsigned char* p = (const unsigned char*)st.gjobs.data();
             blob.insert(blob.end(), p, p + st.gjobs.size() * sizeof(GemmJob<T>));
           }

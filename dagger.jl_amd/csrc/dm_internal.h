@@ -124,7 +124,7 @@ struct GemmCfgId {
   int bm, bn; // workgroup tile
 };
 template <typename T>
-GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs);
+GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs, bool vec = true);
 
 // Fills tiles_m/tiles_n/wg_begin for jobs under `cfg`; returns total WGs.
 template <typename T>

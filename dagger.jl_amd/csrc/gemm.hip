@@ -61,7 +61,11 @@ struct GemmKernel {
   static constexpr int NT = 64 * WM * WN;
   static constexpr int WTM = BM / WM, WTN = BN / WN;
   static constexpr int FM = WTM / 16, FN = WTN / 16;
-  static constexpr int KP = 2;         // pad of k-contiguous rows
+  // Pad of k-contiguous rows.  A quarter-wave's 8-byte fragment reads hit
+  // rows lr = 0..15 at the same k: a row stride of an odd number of doubles
+  // (2 x odd dwords) puts them on 16 distinct bank pairs of the 32 banks;
+  // an FP32 stride of 18 dwords does the same for 4-byte reads.
+  static constexpr int KP = sizeof(T) == 8 ? 1 : 2;
   static constexpr int SK = BK + KP;   // stride of [mn][k] layout
   static constexpr int SMA = BM + 16;  // stride of [k][m] layout
   static constexpr int SNB = BN + 16;  // stride of [k][n] layout
@@ -115,8 +119,10 @@ struct GemmKernel {
       const int e = threadIdx.x + i * NT;
       if (TA) {
         const int k = (e % (BK / VEC)) * VEC, m = e / (BK / VEC);
-        if (VEC == 2) *(vt*)&As[m * SK + k] = *(const vt*)&ra[i][0];
-        else As[m * SK + k] = ra[i][0];
+        if (VEC == 2 && SK % 2 == 0) *(vt*)&As[m * SK + k] = *(const vt*)&ra[i][0];
+        else
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) As[m * SK + k + q] = ra[i][q];
       } else {
         const int m = (e % (BM / VEC)) * VEC, k = e / (BM / VEC);
         if (VEC == 2) *(vt*)&As[k * SMA + m] = *(const vt*)&ra[i][0];
@@ -163,8 +169,10 @@ struct GemmKernel {
       const int e = threadIdx.x + i * NT;
       if (!TB) {
         const int k = (e % (BK / VEC)) * VEC, n = e / (BK / VEC);
-        if (VEC == 2) *(vt*)&Bs[n * SK + k] = *(const vt*)&rb[i][0];
-        else Bs[n * SK + k] = rb[i][0];
+        if (VEC == 2 && SK % 2 == 0) *(vt*)&Bs[n * SK + k] = *(const vt*)&rb[i][0];
+        else
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) Bs[n * SK + k + q] = rb[i][q];
       } else {
         const int n = (e % (BN / VEC)) * VEC, k = e / (BN / VEC);
         if (VEC == 2) *(vt*)&Bs[k * SNB + n] = *(const vt*)&rb[i][0];
@@ -508,7 +516,7 @@ constexpr int GEMM_CFG_SKINNY = 5;  // 32 rows x bn (16..64) columns, K split ov
 }
 
 template <typename T>
-GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs) {
+GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs, bool vec) {
   (void)K;
   const long nj = njobs > 0 ? njobs : 1;
   // A launch of at most 64 columns too small to fill the device (fewer than
@@ -521,8 +529,10 @@ GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs) {
   // few columns: the K-split skinny kernel (at 64 columns the 64 x 64 tile
   // measured slightly faster in large launches)
   if (N <= 48) return GemmCfgId{GEMM_CFG_SKINNY, 32, 16 * ((N + 15) / 16)};
-  if (sizeof(T) == 8) {
-    // Largest tile whose launch still puts about one workgroup on every CU.
+  if (sizeof(T) == 8 && vec) {
+    // Largest tile whose launch still puts about one workgroup on every CU
+    // (scalar-load launches keep the 128 x 128 tile: the 256 x 128 one does
+    // not fit its registers without spilling inside the k-loop).
     const long t256 = (long)((M + 255) / 256) * ((N + 127) / 128) * nj;
     if (t256 >= 256) return GemmCfgId{GEMM_CFG_256x128, 256, 128};
   }
@@ -666,13 +676,14 @@ int gemm_single(bool ta, bool tb, int M, int N, int K, T alpha, const T* A, int 
   j.lda = lda; j.ldb = ldb; j.ldc = ldc;
   j.alpha = alpha; j.beta = beta; j.tri = tri;
   if (K <= 0 || alpha == T(0)) { j.alpha = T(0); j.K = 0; }
-  GemmCfgId cfg = gemm_pick_cfg<T>(M, N, K, 1);
+  const bool vec = job_vec_ok(ta, tb, j);
+  GemmCfgId cfg = gemm_pick_cfg<T>(M, N, K, 1, vec);
   long total = gemm_prepare_jobs<T>(&j, 1, cfg);
-  return gemm_launch<T>(ta, tb, job_vec_ok(ta, tb, j), cfg, nullptr, 0, j, total, abort, s, 0);
+  return gemm_launch<T>(ta, tb, vec, cfg, nullptr, 0, j, total, abort, s, 0);
 }
 
-template GemmCfgId gemm_pick_cfg<double>(int, int, int, int);
-template GemmCfgId gemm_pick_cfg<float>(int, int, int, int);
+template GemmCfgId gemm_pick_cfg<double>(int, int, int, int, bool);
+template GemmCfgId gemm_pick_cfg<float>(int, int, int, int, bool);
 template long gemm_prepare_jobs<double>(GemmJob<double>*, int, GemmCfgId);
 template long gemm_prepare_jobs<float>(GemmJob<float>*, int, GemmCfgId);
 template int gemm_launch<double>(bool, bool, bool, GemmCfgId, const GemmJob<double>*, int,

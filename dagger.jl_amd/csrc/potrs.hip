@@ -195,10 +195,10 @@ int TrtriPlan<T>::build(bool up, int64_t m_, int nb_, const std::function<const 
       maxM = std::max(maxM, j.M); maxN = std::max(maxN, j.N); maxK = std::max(maxK, j.K);
     }
     L.njobs = (int)jobs.size();
-    L.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, L.njobs);
-    L.total = gemm_prepare_jobs<T>(jobs.data(), L.njobs, L.cfg);
     L.vec = true;
     for (auto& j : jobs) L.vec = L.vec && gemm_job_vec_ok<T>(false, false, j);
+    L.cfg = gemm_pick_cfg<T>(maxM, maxN, maxK, L.njobs, L.vec);
+    L.total = gemm_prepare_jobs<T>(jobs.data(), L.njobs, L.cfg);
     std::memcpy(blob.data() + tab_off[t], jobs.data(), sizeof(GemmJob<T>) * jobs.size());
     L.jobs = (const GemmJob<T>*)(tb + tab_off[t]);
     launches.push_back(L);

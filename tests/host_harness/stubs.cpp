@@ -80,7 +80,7 @@ int exec_streams(hipStream_t, int count, hipStream_t* out) {
 void release_exec_streams(hipStream_t) {}
 int trsm_init_attributes() { return 0; }
 int run_big_stack(std::function<int()> fn) { return fn(); }
-template <typename T> GemmCfgId gemm_pick_cfg(int, int, int, int) { return GemmCfgId{0, 128, 128}; }
+template <typename T> GemmCfgId gemm_pick_cfg(int, int, int, int, bool) { return GemmCfgId{0, 128, 128}; }
 template <typename T> long gemm_prepare_jobs(GemmJob<T>* jobs, int nj, GemmCfgId c) {
   long t = 0; for (int i = 0; i < nj; ++i) { jobs[i].tiles_m = (jobs[i].M + 127) / 128; jobs[i].tiles_n = (jobs[i].N + 127) / 128; jobs[i].wg_begin = (int)t; t += (long)jobs[i].tiles_m * jobs[i].tiles_n; } return t; }
 template <typename T> bool gemm_job_vec_ok(bool, bool, const GemmJob<T>&) { return true; }
@@ -155,7 +155,7 @@ int laswp_tiled(T* base, int64_t Mr, int W, int mb, const int* ipiv, int npiv, i
   return 0;
 }
 #define INST(T) \
-  template GemmCfgId gemm_pick_cfg<T>(int, int, int, int); \
+  template GemmCfgId gemm_pick_cfg<T>(int, int, int, int, bool); \
   template long gemm_prepare_jobs<T>(GemmJob<T>*, int, GemmCfgId); \
   template bool gemm_job_vec_ok<T>(bool, bool, const GemmJob<T>&); \
   template int gemm_launch<T>(bool, bool, bool, GemmCfgId, const GemmJob<T>*, int, const GemmJob<T>&, long, const int*, hipStream_t, long); \
