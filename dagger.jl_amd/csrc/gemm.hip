@@ -403,38 +403,51 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(const GemmJob<T>* __re
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int b = 0; b < NB; ++b) acc[rb][b] = v4{0, 0, 0, 0};
+    // A wave takes the 16-k blocks w, w+4, w+8, ... in that order; the loads
+    // of D consecutive blocks are issued together before their MFMAs, so a
+    // K-long chain waits for memory once per D blocks (the accumulation
+    // order is unchanged).
+    constexpr int D = NB == 1 ? 4 : (NB == 2 ? 2 : 1);
     const int nkb = (K + 15) / 16;
-    for (int kb = w; kb < nkb; kb += 4) {
-      const int kbase = kb * 16 + 4 * lk;
-      T a[2][4], x[NB][4];
+    for (int kb0 = w; kb0 < nkb; kb0 += 4 * D) {
+      T a[D][2][4], x[D][NB][4];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-        const int i = m0 + 16 * rb + lr;
+      for (int d = 0; d < D; ++d) {
+        const int kbase = (kb0 + 4 * d) * 16 + 4 * lk;  // past K when the block is past nkb
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int k = kbase + s;
-          const bool ok = i < M && k < K;
-          a[rb][s] = ok ? (TA ? gptr(job.A)[(size_t)i * job.lda + k] : gptr(job.A)[(size_t)k * job.lda + i])
-                        : T(0);
+        for (int rb = 0; rb < 2; ++rb) {
+          const int i = m0 + 16 * rb + lr;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int k = kbase + s;
+            const bool ok = i < M && k < K;
+            a[d][rb][s] = ok ? (TA ? gptr(job.A)[(size_t)i * job.lda + k]
+                                   : gptr(job.A)[(size_t)k * job.lda + i])
+                             : T(0);
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int j = n0 + 16 * b + lr;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int k = kbase + s;
+            const bool ok = j < N && k < K;
+            x[d][b][s] = ok ? (TB ? gptr(job.B)[(size_t)k * job.ldb + j]
+                                  : gptr(job.B)[(size_t)j * job.ldb + k])
+                            : T(0);
+          }
         }
       }
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int j = n0 + 16 * b + lr;
+      for (int d = 0; d < D; ++d)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int k = kbase + s;
-          const bool ok = j < N && k < K;
-          x[b][s] = ok ? (TB ? gptr(job.B)[(size_t)k * job.ldb + j] : gptr(job.B)[(size_t)j * job.ldb + k])
-                       : T(0);
-        }
-      }
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+          for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-          for (int b = 0; b < NB; ++b) acc[rb][b] = Mfma<T>::mac(a[rb][s], x[b][s], acc[rb][b]);
+            for (int b = 0; b < NB; ++b)
+              acc[rb][b] = Mfma<T>::mac(a[d][rb][s], x[d][b][s], acc[rb][b]);
     }
     // partial sums of waves 1..3 through LDS, added to wave 0's in wave order
     if (w > 0) {
@@ -506,7 +519,8 @@ __global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __
 // Configurations (FP64; measured on the k-contiguous Cholesky update shape):
 //   id 4: 256x128 tile, 16 waves of 64x32, BK 16 (110 KB LDS) -- big launches
 //   id 0: 128x128 tile, 16 waves of 32x32, BK 32 (139 KB LDS)
-//   id 1:  64x64  tile,  4 waves of 32x32, BK 16 -- small launches
+//   id 1:  64x64  tile,  4 waves of 32x32, BK 16 -- small launches and
+//          products of at most 64 columns
 // FP32 keeps 128x128 with 4 waves of 64x64 (id 0) and 64x64 (id 1).
 // 16-wave workgroups put four waves on each SIMD, which hides LDS and
 // global latency that a single 64x64-per-wave workgroup leaves exposed.
@@ -529,6 +543,9 @@ GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs, bool vec) {
   // few columns: the K-split skinny kernel (at 64 columns the 64 x 64 tile
   // measured slightly faster in large launches)
   if (N <= 48) return GemmCfgId{GEMM_CFG_SKINNY, 32, 16 * ((N + 15) / 16)};
+  // at most 64 columns: the 64 x 64 tile (a 128-wide tile would compute
+  // half its columns for nothing)
+  if (N <= 64) return GemmCfgId{1, 64, 64};
   if (sizeof(T) == 8 && vec) {
     // Largest tile whose launch still puts about one workgroup on every CU
     // (scalar-load launches keep the 128 x 128 tile: the 256 x 128 one does
