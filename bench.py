@@ -34,6 +34,7 @@ sys.path.insert(0, os.path.join(ROOT, "dagger.jl_amd"))
 
 METRIC = "Cholesky TFLOP/s N=32k FP64 vs mixed+IR @1e-13 bwd err; served solves/s + p99"
 FP64_MFMA_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix (vendor figure)
+HBM_PEAK_GBS = 8000.0  # GB/s, MI355X HBM3E (vendor figure)
 # accuracy gate of the timed FP64 solve (max column backward error); the
 # generator's kappa is about 5, so a correct factor and solve land near 1e-16
 BERR_LIMIT = 1e-12
@@ -317,30 +318,109 @@ def serving_leg(dm, rates, count, seed, workers=6):
     return out
 
 
-def cpu_baseline(dm, n, nb, seed):
-    """Oracle restatement of Dagger's CPU datadeps Cholesky, threaded."""
+def ldiv_roofline(n, nb, nrhs, ms):
+    """ldiv! with nrhs right-hand sides: algorithmic flops 2 N^2 nrhs (two
+    triangular solves) and bytes 2 x the factor's upper tiles (each sweep
+    reads every tile once) + 4 N nrhs x 8 (each sweep reads and writes B).
+    Both bounds: whichever of MFMA time and HBM time is larger is the
+    roofline; the achieved rates are against both peaks."""
+    mt = (n + nb - 1) // nb
+    flops = 2.0 * n * n * nrhs
+    nbytes = 2.0 * (mt * (mt + 1) / 2) * nb * nb * 8 + 4.0 * n * nrhs * 8
+    t = ms * 1e-3
+    t_mfma, t_hbm = flops / (FP64_MFMA_PEAK * 1e12), nbytes / (HBM_PEAK_GBS * 1e9)
+    return {
+        "bound": "mfma" if t_mfma >= t_hbm else "hbm",
+        "achieved": round(flops / t / 1e12, 3),
+        "peak": FP64_MFMA_PEAK,
+        "unit": "TFLOP/s",
+        "frac": round(flops / t / 1e12 / FP64_MFMA_PEAK, 4),
+        "flops": flops,
+        "bytes": nbytes,
+        "gb_per_s": round(nbytes / t / 1e9, 1),
+        "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline_ms": round(max(t_mfma, t_hbm) * 1e3, 3),
+    }
+
+
+def _step_flops(mt, nb, steps):
+    """Algorithmic flops of the first `steps` k-iterations of the tiled
+    Cholesky (POTRF nb^3/3, TRSM nb^3, SYRK nb^3, GEMM 2nb^3 per task)."""
+    f = 0.0
+    for k in range(min(steps, mt)):
+        r = mt - k - 1
+        f += nb ** 3 / 3 + r * nb ** 3 + r * nb ** 3 + r * (r - 1) / 2 * 2 * nb ** 3
+    return f
+
+
+def cpu_baseline(dm, n, nb, seed, steps, timer):
+    """Oracle restatement of Dagger's CPU datadeps Cholesky, threaded: a
+    bounded sample of the headline workload (the first `steps` k-iterations
+    of the N x N tiled factorization, the bulk of its flops per step), plus
+    the whole CPU-runnable configuration (N=4096, Blocks(512,512), cholesky!
+    and ldiv! with 16 right-hand sides) timed on the host and on the GPU."""
     from oracle import dagger_tiled as O
-    DA = dm.spd_generate(n, dm.Blocks(nb, nb), seed=seed)
-    A = DA.collect()
-    DA.unsafe_free()
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
+    DA = dm.spd_generate(n, dm.Blocks(nb, nb), seed=seed)
+    A = DA.collect()
+    DA.unsafe_free()
+    mt = (n + nb - 1) // nb
     with O.blas_single_thread():
         t0 = time.perf_counter()
-        F, info = O.cholesky(A, nb, "U", threaded=True, workers=cores)
+        _, info = O.cholesky(A, nb, "U", threaded=True, workers=cores, steps=steps)
         dt = time.perf_counter() - t0
+    del A
+    flops = _step_flops(mt, nb, steps)
+    ntasks = sum(1 + 2 * (mt - k - 1) + (mt - k - 1) * (mt - k - 2) // 2
+                 for k in range(min(steps, mt)))
+    # the CPU-runnable configuration, whole
+    n1, nb1, r1 = 4096, 512, 16
+    DA1 = dm.spd_generate(n1, dm.Blocks(nb1, nb1), seed=seed)
+    DB1 = dm.randn(dm.Blocks(nb1, nb1), n1, r1, seed=seed + 1)
+    A1, B1 = DA1.collect(), DB1.collect()
+    with O.blas_single_thread():
+        t0 = time.perf_counter()
+        F1, info1 = O.cholesky(A1, nb1, "U", threaded=True, workers=cores)
+        X1 = O.ldiv(F1, "U", B1, nb1, threaded=True, workers=cores)
+        dt1 = time.perf_counter() - t0
+    res1 = float(np.abs(B1 - A1 @ X1).max() / (np.abs(A1).sum(1).max() * np.abs(X1).max()
+                                                + np.abs(B1).max()))
+    gpu_ms = []
+    for rep in range(3):
+        DA1c, DB1c = DA1.copy(), DB1.copy()
+        dm._lib.call("dm_stream_sync", None)
+        timer.start()
+        F = dm.cholesky_(DA1c)
+        dm.ldiv_(F, DB1c)
+        gpu_ms.append(timer.stop_ms())
+        DA1c.unsafe_free()
+        DB1c.unsafe_free()
+    DA1.unsafe_free()
+    DB1.unsafe_free()
+    fl1 = n1 ** 3 / 3 + 2.0 * n1 * n1 * r1
     return {
-        "value": round(n ** 3 / 3 / dt / 1e12, 4),
+        "value": round(flops / dt / 1e12, 4),
         "unit": "TFLOP/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"full tiled Cholesky (U) at N={n}, Blocks({nb},{nb}) on the host: the "
-                   f"oracle's datadeps tile DAG on {cores} threads, one single-threaded "
-                   f"OpenBLAS call per tile task; {dt:.2f} s"),
+        "sample": (f"the first {steps} of {mt} k-iterations of the tiled Cholesky (U) at N={n}, "
+                   f"Blocks({nb},{nb}) ({ntasks} tile tasks, {flops:.3e} flops) on the host: the "
+                   f"oracle's datadeps tile DAG on {cores} threads, one single-threaded OpenBLAS "
+                   f"call per tile task; {dt:.2f} s"),
         "seconds": round(dt, 3),
         "info": info,
+        "config1": {
+            "workload": f"cholesky!+ldiv! N={n1}, Blocks({nb1},{nb1}), {r1} RHS",
+            "cpu_s": round(dt1, 3),
+            "cpu_tflops": round(fl1 / dt1 / 1e12, 4),
+            "cpu_residual": res1,
+            "cpu_info": info1,
+            "gpu_ms": round(min(gpu_ms), 3),
+            "gpu_tflops": round(fl1 / (min(gpu_ms) * 1e-3) / 1e12, 3),
+        },
     }
 
 
@@ -353,8 +433,10 @@ def main():
     ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-n", type=int, default=12288,
+    ap.add_argument("--cpu-n", type=int, default=32768,
                     help="N of the CPU-baseline sample (0 disables it)")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="k-iterations of the CPU-baseline sample")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=32768,
                     help="N of the mixed-precision leg (0 disables it)")
@@ -450,7 +532,7 @@ def main():
         log(f"batched: {batched_res}")
     cpu = None
     if a.cpu_n > 0:
-        cpu = cpu_baseline(dm, a.cpu_n, nb, a.seed)
+        cpu = cpu_baseline(dm, a.cpu_n, nb, a.seed, a.cpu_steps, Timer(call))
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -475,6 +557,7 @@ def main():
         "cholesky_ms": round(chol_ms, 3),
         "ldiv_ms": round(solve_ms, 3),
         "ldiv_tflops": round(2.0 * n * n * nrhs / (solve_ms * 1e-3) / 1e12, 3),
+        "ldiv_roofline": ldiv_roofline(n, nb, nrhs, solve_ms),
         "backward_error": berr,
         "first_call_s": round(first_s, 3),
         "roofline": roof,

@@ -10,6 +10,25 @@
       kernel (`__amd_rocclr_copyBuffer`) of the largest device-to-device copy
       in the same run, whose true size is BYTES (read BYTES, write BYTES).
 
+  python -m dagger_mi355x.rocprof_summary counters COUNTER_CSV [COUNTER_CSV ...] \
+          --kernel REGEX [--simds 1024] [--xcds 8]
+      per-dispatch means of every counter collected for the kernels matching
+      REGEX (rocprofv3 sums a dispatch's XCD and shader-engine instances into
+      one value; GRBM_* counters are then divided by the XCD count to give
+      one XCD's cycles), and the
+      derived figures the counters allow: MFMA utilisation
+      sum(SQ_VALU_MFMA_BUSY_CYCLES) / (GRBM_GUI_ACTIVE x SIMDs), the
+      average shader clock GRBM_GUI_ACTIVE / dispatch duration, the share of
+      LDS-active cycles lost to bank conflicts, and the wave-cycle shares
+      spent waiting.
+
+  python -m dagger_mi355x.rocprof_summary window KERNEL_TRACE_CSV \
+          --start REGEX --stop REGEX
+      windows of a kernel trace: each opens at a dispatch whose kernel
+      matches --start and closes before the next dispatch matching --stop
+      (or --start); prints every window's span (first start to last end) and
+      dispatch count, and the per-kernel time of the last window.
+
 Prints a short text summary; the `traffic` mode ends with machine-readable
 lines `traffic_bytes_per_launch: <bytes>` and `fetch_scale`/`write_scale`.
 """
@@ -94,6 +113,96 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
     return "\n".join(out)
 
 
+def _dispatch_rows(paths, pat):
+    """{dispatch id: {"name", "dur_ns", counter: [row values]}} of the
+    dispatches whose kernel name matches `pat`, over several CSV files."""
+    out = {}
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"]
+            if not pat.search(name):
+                continue
+            key = (path, int(r["Dispatch_Id"]))
+            d = out.setdefault(key, {"name": name, "dur_ns": None, "vals": defaultdict(list)})
+            if r.get("Start_Timestamp") and r.get("End_Timestamp"):
+                d["dur_ns"] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+            d["vals"][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return out
+
+
+def counters(paths, kernel, simds, xcds):
+    pat = re.compile(kernel)
+    rows = _dispatch_rows(paths, pat)
+    if not rows:
+        raise SystemExit(f"no dispatch matches {kernel!r}")
+    per = defaultdict(list)      # counter -> per-dispatch values
+    durs = defaultdict(list)     # counter -> durations of the dispatches that carry it
+    for d in rows.values():
+        for c, vals in d["vals"].items():
+            v = sum(vals) / xcds if c.startswith("GRBM_") else sum(vals)
+            per[c].append(v)
+            if d["dur_ns"]:
+                durs[c].append(d["dur_ns"])
+    mean = {c: sum(v) / len(v) for c, v in per.items()}
+    names = sorted({d["name"] for d in rows.values()})
+    out = [f"kernel: {', '.join(_short(n, 160) for n in names)}",
+           f"dispatches: {len(rows)} over {len(paths)} run(s)"]
+    for c in sorted(mean):
+        out.append(f"  {c:<32} {mean[c]:>18.1f}  (mean per dispatch of {len(per[c])})")
+    g = mean.get("GRBM_GUI_ACTIVE")
+    if g and "SQ_VALU_MFMA_BUSY_CYCLES" in mean:
+        # ratio of sums over the dispatches that carry both counters
+        out.append(f"mfma_util: {mean['SQ_VALU_MFMA_BUSY_CYCLES'] / (g * simds):.4f}  "
+                   f"(sum SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x {simds} SIMDs))")
+    if g and durs.get("GRBM_GUI_ACTIVE"):
+        ns = sum(durs["GRBM_GUI_ACTIVE"]) / len(durs["GRBM_GUI_ACTIVE"])
+        out.append(f"mean_dispatch_us: {ns / 1e3:.1f}")
+        out.append(f"avg_clock_mhz: {g / ns * 1e3:.0f}  (GRBM_GUI_ACTIVE / dispatch duration)")
+    if mean.get("SQ_LDS_IDX_ACTIVE") and "SQ_LDS_BANK_CONFLICT" in mean:
+        out.append(f"lds_bank_conflict_share: "
+                   f"{mean['SQ_LDS_BANK_CONFLICT'] / mean['SQ_LDS_IDX_ACTIVE']:.4f}  "
+                   f"(SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE)")
+    w = mean.get("SQ_WAVE_CYCLES")
+    if w:
+        for c in ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS",
+                  "SQ_ACTIVE_INST_VALU", "SQ_INST_CYCLES_VMEM_RD"):
+            if c in mean:
+                out.append(f"{c.lower()}_per_wave_cycle: {mean[c] / w:.4f}")
+    return "\n".join(out)
+
+
+def window(path, start, stop):
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    ps, pe = re.compile(start), re.compile(stop)
+    wins, cur = [], None
+    for r in rows:
+        name = r["Kernel_Name"]
+        if ps.search(name):
+            cur = [r]
+            wins.append(cur)
+        elif pe.search(name):
+            cur = None
+        elif cur is not None:
+            cur.append(r)
+    if not wins:
+        raise SystemExit(f"no dispatch matches {start!r}")
+    out = []
+    for i, w in enumerate(wins):
+        t0 = int(w[0]["Start_Timestamp"])
+        t1 = max(int(r["End_Timestamp"]) for r in w)
+        out.append(f"window {i}: span {(t1 - t0) / 1e6:.3f} ms, {len(w)} dispatches")
+    agg = defaultdict(lambda: [0, 0.0])
+    for r in wins[-1]:
+        a = agg[r["Kernel_Name"]]
+        a[0] += 1
+        a[1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    out.append(f"last window, per kernel:")
+    out.append(f"{'calls':>7} {'total ms':>10} {'avg us':>10}  kernel")
+    for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"{c:>7} {t / 1e6:>10.3f} {t / c / 1e3:>10.2f}  {_short(name)}")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -105,9 +214,23 @@ def main(argv=None):
     t.add_argument("write_csv")
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--simds", type=int, default=1024)
+    c.add_argument("--xcds", type=int, default=8)
+    w = sub.add_parser("window")
+    w.add_argument("csv")
+    w.add_argument("--start", required=True)
+    w.add_argument("--stop", required=True)
     a = ap.parse_args(argv)
+    if a.mode == "window":
+        print(window(a.csv, a.start, a.stop))
+        return 0
     if a.mode == "stats":
         print(stats(a.csv, a.top))
+    elif a.mode == "counters":
+        print(counters(a.csv, a.kernel, a.simds, a.xcds))
     else:
         print(traffic(a.fetch_csv, a.write_csv, a.kernel, a.copy_bytes))
     return 0

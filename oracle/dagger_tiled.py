@@ -254,12 +254,15 @@ def blas_single_thread():
 
 
 # ---------------------------------------------------------------- Cholesky
-def chol_region(T, uplo, info):
+def chol_region(T, uplo, info, steps=None):
     """Record the task loop of src/array/cholesky.jl:38-53 ('U') / :83-98 ('L').
 
-    info is the 1-element array filled by potrf_checked! on failure."""
+    info is the 1-element array filled by potrf_checked! on failure; `steps`
+    (default all) records only the first `steps` iterations of the k loop
+    (a bounded sample of the same task list, for timing)."""
     Ac = T.chunks
     mt, nt = Ac.shape
+    kmax = mt if steps is None else min(mt, steps)
     R = Region()
 
     def potrf_checked(uplo_, A, info_arr):
@@ -280,7 +283,7 @@ def chol_region(T, uplo, info):
 
     trsm, syrk, gemm = guarded(tile_trsm), guarded(tile_syrk), guarded(tile_gemm)
     if uplo == "U":
-        for k in range(mt):
+        for k in range(kmax):
             R.spawn("potrf", potrf_checked, ("U", Ac[k, k], info), [(Ac[k, k], INOUT), (info, OUT)])
             for n in range(k + 1, nt):
                 R.spawn("trsm", trsm, ("L", "U", "T", "N", 1.0, Ac[k, k], Ac[k, n]),
@@ -292,7 +295,7 @@ def chol_region(T, uplo, info):
                     R.spawn("gemm", gemm, ("T", "N", -1.0, Ac[k, m], Ac[k, n], 1.0, Ac[m, n]),
                             [(Ac[k, m], IN), (Ac[k, n], IN), (Ac[m, n], INOUT)])
     else:
-        for k in range(mt):
+        for k in range(kmax):
             R.spawn("potrf", potrf_checked, ("L", Ac[k, k], info), [(Ac[k, k], INOUT), (info, OUT)])
             for m in range(k + 1, mt):
                 R.spawn("trsm", trsm, ("R", "L", "T", "N", 1.0, Ac[k, k], Ac[m, k]),
@@ -306,14 +309,15 @@ def chol_region(T, uplo, info):
     return R
 
 
-def cholesky(A, nb, uplo="U", threaded=False, workers=None):
+def cholesky(A, nb, uplo="U", threaded=False, workers=None, steps=None):
     """_chol!(DArray(A, Blocks(nb, nb)), uplo): returns (factor matrix, info).
 
     The factor matrix is the collected DArray: the uplo triangle holds the
-    factor, the other triangle keeps the input (as the reference leaves it)."""
+    factor, the other triangle keeps the input (as the reference leaves it).
+    `steps` limits the k loop (chol_region) to a leading sample."""
     T = TiledMatrix(np.array(A, copy=True), nb, nb)
     info = np.zeros(1, dtype=np.int64)
-    R = chol_region(T, uplo, info)
+    R = chol_region(T, uplo, info, steps)
     try:
         if threaded:
             run_threaded(R, workers)

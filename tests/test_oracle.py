@@ -435,3 +435,21 @@ def test_lu_planted_mistake_is_caught():
     F, ipiv, _ = orig_lu(A, 64)
     L, U, P = O.lu_unpack(F, ipiv)
     assert np.abs(P @ A - L @ U).max() < 1e-12
+
+
+def test_cholesky_leading_steps_sample():
+    """The bounded timing sample (the first k-iterations only): after one
+    step, block row 0 is the full factor's block row 0 and the trailing upper
+    triangle is the Schur complement A22 - U01^T U01 (closed form)."""
+    rng = np.random.default_rng(31)
+    n, nb = 384, 128
+    G = rng.standard_normal((n, n))
+    A = G @ G.T / n + np.eye(n)
+    F1, info = O.cholesky(A, nb, "U", steps=1)
+    Ff, _ = O.cholesky(A, nb, "U")
+    assert info == 0
+    assert np.allclose(np.triu(F1[:nb, :]), np.triu(Ff[:nb, :]), rtol=0, atol=1e-13)
+    U01 = F1[:nb, nb:]
+    S = A[nb:, nb:] - U01.T @ U01
+    assert np.allclose(np.triu(F1[nb:, nb:]), np.triu(S), rtol=0, atol=1e-12)
+    assert np.array_equal(np.tril(F1, -1), np.tril(A, -1))  # lower part untouched
