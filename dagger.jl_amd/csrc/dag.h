@@ -115,6 +115,8 @@ struct Dag {
   int issue(hipStream_t origin_stream);
   template <typename T>
   int launch_group_t(const Group& G, hipStream_t s);
+  template <typename R>
+  int launch_group_cplx(const Group& G, hipStream_t s);
 };
 
 }  // namespace dm

@@ -33,12 +33,15 @@
 #include <vector>
 #include "dm_internal.h"
 #include "dag.h"
+#include "complex_ops.h"
 
 namespace dm {
 
 namespace {
 
-size_t esize(int dtype) { return dtype == DM_F32 ? 4 : 8; }
+size_t esize(int dtype) {
+  return dtype == DM_F32 ? 4 : dtype == DM_C64 ? 16 : 8;  // DM_F64, DM_C32: 8
+}
 
 // Footprint of a column-major rows x cols array with leading dimension ld.
 void mat_span(const void* p, long rows, long cols, long ld, size_t es, uintptr_t& lo,
@@ -53,7 +56,11 @@ void mat_span(const void* p, long rows, long cols, long ld, size_t es, uintptr_t
 int Dag::add(const dm_task& t) {
   if (planned) return arg_fail(1, "dag already planned");
   if (t.op < DM_OP_POTRF || t.op > DM_OP_LASWP) return arg_fail(2, "unknown op");
-  if (t.dtype != DM_F64 && t.dtype != DM_F32) return arg_fail(2, "unknown dtype");
+  if (t.dtype != DM_F64 && t.dtype != DM_F32 && t.dtype != DM_C64 && t.dtype != DM_C32)
+    return arg_fail(2, "unknown dtype");
+  if (is_cplx(t.dtype) && t.op != DM_OP_POTRF && t.op != DM_OP_TRSM && t.op != DM_OP_SYRK &&
+      t.op != DM_OP_GEMM)
+    return arg_fail(2, "complex tasks are POTRF, TRSM, SYRK (herk) and GEMM");
   TaskRec r;
   r.t = t;
   const size_t es = esize(t.dtype);
@@ -213,8 +220,9 @@ int Dag::plan(const dm_plan_opts& o) {
     TaskRec& r = tasks[i];
     const int slack = r.alap - r.level;
     const int crit = (opts.lookahead > 0 && slack <= opts.lookahead) ? 0 : 1;
-    const bool batchable = opts.batch && (r.t.op == DM_OP_GEMM || r.t.op == DM_OP_SYRK ||
-                                          r.t.op == DM_OP_TRSM);
+    // complex tasks launch one by one (complex.hip), real ones batch
+    const bool batchable = opts.batch && !is_cplx(r.t.dtype) &&
+                           (r.t.op == DM_OP_GEMM || r.t.op == DM_OP_SYRK || r.t.op == DM_OP_TRSM);
     const auto key = std::make_tuple(r.level, crit, shape_key(r.t), batchable ? -1 : i);
     auto it = gid.find(key);
     int g;
@@ -318,6 +326,7 @@ int Dag::build_device_tables() {
   std::vector<unsigned char> blob;
   auto align = [&](size_t a) { while (blob.size() % a) blob.push_back(0); };
   for (Group& G : groups) {
+    if (is_cplx(G.dtype)) continue;  // complex tasks need no job tables
     if (G.tasks.size() <= 1 && G.op != DM_OP_TRSM && G.op != DM_OP_GEMM) continue;
     if (G.op == DM_OP_GEMM) {
       const dm_task& t0 = tasks[G.tasks[0]].t;
@@ -477,8 +486,8 @@ int Dag::build_device_tables() {
     ws_bytes = (getrf_bytes + 255) / 256 * 256;
   }
   for (Group& G : groups) {
-    const bool needs = G.op == DM_OP_TRSV ||
-                       (G.op == DM_OP_TRSM && !G.batched_trsm);
+    const bool needs = !is_cplx(G.dtype) &&
+                       (G.op == DM_OP_TRSV || (G.op == DM_OP_TRSM && !G.batched_trsm));
     if (!needs) continue;
     const size_t per = G.dtype == DM_F64 ? trsm_ws_elems<double>() * sizeof(double)
                                          : trsm_ws_elems<float>() * sizeof(float);
@@ -516,6 +525,43 @@ int Dag::build_device_tables() {
   }
   if (!blob.empty()) DM_HIP(hipMemcpy(base + tab_base, blob.data(), blob.size(), hipMemcpyHostToDevice));
   DM_HIP(hipMemset(dev_mem, 0, 256));
+  return 0;
+}
+
+// Complex tasks (each its own group): POTRF, TRSM, SYRK as herk!, GEMM with
+// the real alpha/beta the tiled Cholesky and its solves use
+// (src/array/cholesky.jl:23-26, src/array/trsm.jl:59-192).
+template <typename R>
+int Dag::launch_group_cplx(const Group& G, hipStream_t s) {
+  for (int ti : G.tasks) {
+    const dm_task& t = tasks[ti].t;
+    const R al = (R)t.alpha, be = (R)t.beta;
+    int rc = 0;
+    switch (t.op) {
+      case DM_OP_POTRF:
+        rc = zpotrf_rec<R>(is_upper(t.uplo), t.n, (R*)t.c, t.ldc, t.info, 0, abort_dev, t.k, s);
+        break;
+      case DM_OP_TRSM:
+        rc = ztrsm_call<R>(t.side, t.uplo, t.transa, t.diag, t.m, t.n, al, R(0), (const R*)t.a,
+                           t.lda, (R*)t.c, t.ldc, abort_dev, s);
+        break;
+      case DM_OP_SYRK: {  // herk!(uplo, trans, alpha, A, beta, C)
+        if ((t.alpha == 0.0 || t.k == 0) && t.beta == 1.0) break;  // BLAS quick return
+        const bool tr = !is_notrans(t.transa);
+        rc = zgemm_call<R>(tr ? 'C' : 'N', tr ? 'N' : 'C', t.n, t.n, t.k, al, R(0),
+                           (const R*)t.a, t.lda, (const R*)t.a, t.lda, be, R(0), (R*)t.c, t.ldc,
+                           is_upper(t.uplo) ? 1 : 2, 1, abort_dev, s);
+        break;
+      }
+      case DM_OP_GEMM:
+        rc = zgemm_call<R>(t.transa, t.transb, t.m, t.n, t.k, al, R(0), (const R*)t.a, t.lda,
+                           (const R*)t.b, t.ldb, be, R(0), (R*)t.c, t.ldc, 0, 0, abort_dev, s);
+        break;
+      default:
+        rc = arg_fail(0, "complex task op");
+    }
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -640,7 +686,12 @@ int Dag::issue(hipStream_t o) {
     for (int w : G.waits) DM_HIP(hipStreamWaitEvent(s, events[w], 0));
     const bool prof = opts.profile && !capturing;
     if (prof) DM_HIP(hipEventRecord(prof_start[g], s));
-    rc = G.dtype == DM_F64 ? launch_group_t<double>(G, s) : launch_group_t<float>(G, s);
+    switch (G.dtype) {
+      case DM_F64: rc = launch_group_t<double>(G, s); break;
+      case DM_F32: rc = launch_group_t<float>(G, s); break;
+      case DM_C64: rc = launch_group_cplx<double>(G, s); break;
+      default: rc = launch_group_cplx<float>(G, s); break;
+    }
     if (rc) return rc;
     if (prof) DM_HIP(hipEventRecord(prof_stop[g], s));
     DM_HIP(hipEventRecord(events[g], s));

@@ -65,6 +65,7 @@ void* stream_workspace(hipStream_t s, size_t bytes);
 int exec_streams(hipStream_t caller, int count, hipStream_t* out);
 void release_exec_streams(hipStream_t caller);
 
+inline bool is_cplx(int dtype) { return dtype == DM_C64 || dtype == DM_C32; }
 inline bool is_trans(char t) { return t == 'T' || t == 't' || t == 'C' || t == 'c'; }
 inline bool is_notrans(char t) { return t == 'N' || t == 'n'; }
 inline bool is_upper(char u) { return u == 'U' || u == 'u'; }

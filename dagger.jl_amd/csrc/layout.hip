@@ -42,11 +42,29 @@ static int tile_copy(bool pack, const dm_tiled* t, void* full, int64_t ld, doubl
 
 using namespace dm;
 
+// The real view of a complex tiled matrix: each (re, im) pair is two rows, so
+// an m x n complex Blocks(mb, nb) matrix is, bit for bit, the 2m x n real
+// Blocks(2mb, nb) one (each tile's ld doubles with its rows).
+static dm_tiled real_view(const dm_tiled& t) {
+  dm_tiled r = t;
+  if (is_cplx(t.dtype)) {
+    r.m *= 2;
+    r.mb *= 2;
+    r.dtype = t.dtype == DM_C64 ? DM_F64 : DM_F32;
+  }
+  return r;
+}
+
 extern "C" int dm_tiled_pack(const dm_tiled* dst, const void* src, int64_t ld, double diag_add,
                              dm_stream_t s) {
   if (!dst) return arg_fail(1, "dst is NULL");
   if (dst->mb <= 0 || dst->nb <= 0) return arg_fail(1, "block size must be positive");
   if (ld < (dst->m > 1 ? dst->m : 1)) return arg_fail(3, "ld < max(1,m)");
+  if (is_cplx(dst->dtype)) {  // copies of the real view (no complex diagonal shift)
+    if (diag_add != 0.0) return arg_fail(4, "diag_add is for real element types");
+    const dm_tiled r = real_view(*dst);
+    return dm_tiled_pack(&r, src, 2 * ld, 0.0, s);
+  }
   int rc = ensure_init();
   if (rc) return rc;
   if (dst->dtype == DM_F64) return tile_copy<double>(true, dst, (void*)src, ld, diag_add, stream_of(s));
@@ -58,6 +76,10 @@ extern "C" int dm_tiled_unpack(void* dst, int64_t ld, const dm_tiled* src, dm_st
   if (!src) return arg_fail(3, "src is NULL");
   if (src->mb <= 0 || src->nb <= 0) return arg_fail(3, "block size must be positive");
   if (ld < (src->m > 1 ? src->m : 1)) return arg_fail(2, "ld < max(1,m)");
+  if (is_cplx(src->dtype)) {
+    const dm_tiled r = real_view(*src);
+    return dm_tiled_unpack(dst, 2 * ld, &r, s);
+  }
   int rc = ensure_init();
   if (rc) return rc;
   if (src->dtype == DM_F64) return tile_copy<double>(false, src, dst, ld, 0.0, stream_of(s));
@@ -123,20 +145,23 @@ extern "C" int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_d
 
 // ---------------------------------------------------------------------------
 // ishermitian(::DArray) (src/array/linalg.jl:90-124): diagonal tiles must be
-// exactly symmetric (Julia's ishermitian on a dense block); an off-diagonal
-// pair (i > j) passes when ||A_ij - A_ji^T||_F <= sqrt(eps) *
+// exactly Hermitian (Julia's ishermitian on a dense block); an off-diagonal
+// pair (i > j) passes when ||A_ij - A_ji^H||_F <= sqrt(eps) *
 // max(||A_ij||_F, ||A_ji||_F) (is_cross_hermitian, src/array/linalg.jl:38-48).
 // One workgroup per tile pair; fixed-order reductions.
 namespace dm {
 // One workgroup (16 waves) per tile pair (i >= j), walking the pair in 64 x 64
 // sub-blocks: the sub-block of A_ij is read along its columns, the matching
 // sub-block of A_ji along its columns too and transposed through LDS, so
-// both reads are contiguous (A is streamed from HBM once).
-template <typename T>
+// both reads are contiguous (A is streamed from HBM once).  CPLX: elements
+// are (re, im) pairs of T and the partner is conjugated (A_ji^H), so a
+// diagonal tile must also have an exactly zero imaginary diagonal.
+template <typename T, bool CPLX>
 __global__ __launch_bounds__(1024) void herm_pair_kernel(const T* __restrict__ A, int64_t m,
                                                         int mb, int mt,
                                                         int* __restrict__ ok_pairs) {
-  __shared__ T ts[64 * 65];
+  constexpr int S = CPLX ? 2 : 1;  // T values per element
+  __shared__ T ts[S][64 * 65];
   __shared__ double s0[1024], s1[1024], s2[1024];
   // decode pair index -> (i, j) with i >= j, column by column of the lower triangle
   int p = blockIdx.x, j = 0;
@@ -145,7 +170,9 @@ __global__ __launch_bounds__(1024) void herm_pair_kernel(const T* __restrict__ A
   const int ri = (int)((i * (int64_t)mb + mb <= m) ? mb : m - i * (int64_t)mb);
   const int rj = (int)((j * (int64_t)mb + mb <= m) ? mb : m - j * (int64_t)mb);
   // tile (r, c) origin for a square m x m Blocks(mb, mb) matrix
-  auto tile = [&](int r, int c, int64_t cols) { return A + c * (int64_t)mb * m + r * (int64_t)mb * cols; };
+  auto tile = [&](int r, int c, int64_t cols) {
+    return A + S * (c * (int64_t)mb * m + r * (int64_t)mb * cols);
+  };
   const T* Aij = tile(i, j, rj);  // ri x rj, ld = ri
   const T* Aji = tile(j, i, ri);  // rj x ri, ld = rj
   const int tid = threadIdx.x, lo = tid & 63, hi = tid >> 6;
@@ -157,23 +184,30 @@ __global__ __launch_bounds__(1024) void herm_pair_kernel(const T* __restrict__ A
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int rr = hi + 16 * q, cc = lo;
-        T v = T(0);
-        if (r0 + rr < ri && c0 + cc < rj) v = Aji[(int64_t)(r0 + rr) * rj + c0 + cc];
-        ts[rr * 65 + cc] = v;
+        const bool in = r0 + rr < ri && c0 + cc < rj;
+#pragma unroll
+        for (int h = 0; h < S; ++h)
+          ts[h][rr * 65 + cc] = in ? Aji[S * ((int64_t)(r0 + rr) * rj + c0 + cc) + h] : T(0);
       }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int rr = lo, cc = hi + 16 * q;
         if (r0 + rr < ri && c0 + cc < rj) {
-          const double a = (double)Aij[(int64_t)(c0 + cc) * ri + r0 + rr];
-          const double b = (double)ts[rr * 65 + cc];
+          const int64_t ia = S * ((int64_t)(c0 + cc) * ri + r0 + rr);
+          const double a = (double)Aij[ia];
+          const double b = (double)ts[0][rr * 65 + cc];
+          double ai = 0, bi = 0;
+          if (CPLX) {
+            ai = (double)Aij[ia + 1];
+            bi = -(double)ts[S - 1][rr * 65 + cc];  // conj
+          }
           if (i == j) {
-            d2 += (a != b) ? 1.0 : 0.0;
+            d2 += (a != b || ai != bi) ? 1.0 : 0.0;
           } else {
-            d2 += (a - b) * (a - b);
-            n1 += a * a;
-            n2 += b * b;
+            d2 += (a - b) * (a - b) + (ai - bi) * (ai - bi);
+            n1 += a * a + ai * ai;
+            n2 += b * b + bi * bi;
           }
         }
       }
@@ -227,12 +261,26 @@ extern "C" int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* wor
   const int mt = (int)((A->m + A->mb - 1) / A->mb);
   const int pairs = mt * (mt + 1) / 2;
   if (pairs == 0) return fill_i32_launch(result_dev, 1, 1, st);
-  if (A->dtype == DM_F64)
-    hipLaunchKernelGGL((herm_pair_kernel<double>), dim3(pairs), dim3(1024), 0, st,
-                       (const double*)A->base, A->m, A->mb, mt, work_dev);
-  else
-    hipLaunchKernelGGL((herm_pair_kernel<float>), dim3(pairs), dim3(1024), 0, st,
-                       (const float*)A->base, A->m, A->mb, mt, work_dev);
+  switch (A->dtype) {
+    case DM_F64:
+      hipLaunchKernelGGL((herm_pair_kernel<double, false>), dim3(pairs), dim3(1024), 0, st,
+                         (const double*)A->base, A->m, A->mb, mt, work_dev);
+      break;
+    case DM_F32:
+      hipLaunchKernelGGL((herm_pair_kernel<float, false>), dim3(pairs), dim3(1024), 0, st,
+                         (const float*)A->base, A->m, A->mb, mt, work_dev);
+      break;
+    case DM_C64:
+      hipLaunchKernelGGL((herm_pair_kernel<double, true>), dim3(pairs), dim3(1024), 0, st,
+                         (const double*)A->base, A->m, A->mb, mt, work_dev);
+      break;
+    case DM_C32:
+      hipLaunchKernelGGL((herm_pair_kernel<float, true>), dim3(pairs), dim3(1024), 0, st,
+                         (const float*)A->base, A->m, A->mb, mt, work_dev);
+      break;
+    default:
+      return arg_fail(1, "unknown dtype");
+  }
   DM_LAUNCH_CHECK("herm_pair_kernel");
   hipLaunchKernelGGL(all_ok_kernel, dim3(1), dim3(256), 0, st, work_dev, pairs, result_dev);
   DM_LAUNCH_CHECK("all_ok_kernel");
@@ -364,8 +412,13 @@ extern "C" int dm_tiled_norm2(const dm_tiled* A, int part, double* out_dev, int*
                               double* work_dev, dm_stream_t s) {
   if (!A) return arg_fail(1, "A is NULL");
   if (A->m < 0 || A->n < 0 || A->mb < 1 || A->nb < 1) return arg_fail(1, "bad tiled matrix");
-  if (A->dtype != DM_F64 && A->dtype != DM_F32) return arg_fail(1, "dtype must be FP64 or FP32");
   if (part < 0 || part > 2) return arg_fail(2, "part must be 0 (whole), 1 (upper) or 2 (lower)");
+  if (is_cplx(A->dtype)) {  // sum of |z|^2 = sum over the real view's entries
+    if (part != 0) return arg_fail(2, "complex element types: part must be 0 (whole)");
+    const dm_tiled r = real_view(*A);
+    return dm_tiled_norm2(&r, 0, out_dev, nonfinite_dev, work_dev, s);
+  }
+  if (A->dtype != DM_F64 && A->dtype != DM_F32) return arg_fail(1, "unknown dtype");
   if (!out_dev || !nonfinite_dev || !work_dev) return arg_fail(3, "NULL output/flag/work");
   int rc = ensure_init();
   if (rc) return rc;

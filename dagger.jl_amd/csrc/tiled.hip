@@ -37,7 +37,7 @@ struct TileView {
   int mt, nt;
   explicit TileView(const dm_tiled& t)
       : base((char*)t.base), m(t.m), n(t.n), mb(t.mb), nb(t.nb), dtype(t.dtype),
-        es(t.dtype == DM_F32 ? 4 : 8) {
+        es(t.dtype == DM_F32 ? 4 : t.dtype == DM_C64 ? 16 : 8) {
     mt = (int)((m + mb - 1) / mb);
     nt = (int)((n + nb - 1) / nb);
   }
@@ -54,8 +54,16 @@ int check_tiled(const dm_tiled* A, int argno) {
   if (!A->base && A->m > 0 && A->n > 0) return arg_fail(argno, "tiled base is NULL");
   if (A->m < 0 || A->n < 0) return arg_fail(argno, "negative size");
   if (A->mb <= 0 || A->nb <= 0) return arg_fail(argno, "block size must be positive");
-  if (A->dtype != DM_F64 && A->dtype != DM_F32) return arg_fail(argno, "unknown dtype");
+  if (A->dtype != DM_F64 && A->dtype != DM_F32 && A->dtype != DM_C64 && A->dtype != DM_C32)
+    return arg_fail(argno, "unknown dtype");
   return 0;
+}
+
+// BLAS transpose character of a tiled operation: 'C' stays the conjugate
+// transpose on complex tiles and is 'T' on real ones.
+char trans_char(char t, int dtype) {
+  if (is_notrans(t)) return 'N';
+  return (is_cplx(dtype) && (t == 'C' || t == 'c')) ? 'C' : 'T';
 }
 
 dm_task make(int op, int dtype) {
@@ -124,17 +132,18 @@ void add_gemm(Dag& d, int dtype, char ta, char tb, int m, int n, int k, double a
 void build_cholesky(Dag& d, const TileView& A, char uplo, int* info) {
   const int mt = A.mt, nt = A.nt;
   const int dt = A.dtype;
+  const char tc = is_cplx(dt) ? 'C' : 'T';  // trans = iscomplex ? 'C' : 'T'; SYRK -> herk
   if (is_upper(uplo)) {
     for (int k = 0; k < mt; ++k) {
       add_potrf(d, A, k, 'U', info);
       for (int n = k + 1; n < nt; ++n)
-        add_trsm(d, dt, 'L', 'U', 'T', 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(k, n), A.rows(k),
+        add_trsm(d, dt, 'L', 'U', tc, 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(k, n), A.rows(k),
                  A.cols(n), A.ld(k));
       for (int m = k + 1; m < mt; ++m) {
-        add_syrk(d, dt, 'U', 'T', A.cols(m), A.rows(k), -1.0, A.ptr(k, m), A.ld(k), 1.0,
+        add_syrk(d, dt, 'U', tc, A.cols(m), A.rows(k), -1.0, A.ptr(k, m), A.ld(k), 1.0,
                  A.ptr(m, m), A.ld(m));
         for (int n = m + 1; n < nt; ++n)
-          add_gemm(d, dt, 'T', 'N', A.rows(m), A.cols(n), A.rows(k), -1.0, A.ptr(k, m), A.ld(k),
+          add_gemm(d, dt, tc, 'N', A.rows(m), A.cols(n), A.rows(k), -1.0, A.ptr(k, m), A.ld(k),
                    A.ptr(k, n), A.ld(k), 1.0, A.ptr(m, n), A.ld(m));
       }
     }
@@ -142,13 +151,13 @@ void build_cholesky(Dag& d, const TileView& A, char uplo, int* info) {
     for (int k = 0; k < mt; ++k) {
       add_potrf(d, A, k, 'L', info);
       for (int m = k + 1; m < mt; ++m)
-        add_trsm(d, dt, 'R', 'L', 'T', 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(m, k), A.rows(m),
+        add_trsm(d, dt, 'R', 'L', tc, 'N', 1.0, A.ptr(k, k), A.ld(k), A.ptr(m, k), A.rows(m),
                  A.cols(k), A.ld(m));
       for (int n = k + 1; n < nt; ++n) {
         add_syrk(d, dt, 'L', 'N', A.rows(n), A.cols(k), -1.0, A.ptr(n, k), A.ld(n), 1.0,
                  A.ptr(n, n), A.ld(n));
         for (int m = n + 1; m < mt; ++m)
-          add_gemm(d, dt, 'N', 'T', A.rows(m), A.rows(n), A.cols(k), -1.0, A.ptr(m, k), A.ld(m),
+          add_gemm(d, dt, 'N', tc, A.rows(m), A.rows(n), A.cols(k), -1.0, A.ptr(m, k), A.ld(m),
                    A.ptr(n, k), A.ld(n), 1.0, A.ptr(m, n), A.ld(m));
       }
     }
@@ -219,7 +228,7 @@ void build_trsm(Dag& d, char side, char uplo, char trans, char diag, double alph
                 const TileView& A, const TileView& B) {
   const int dt = B.dtype;
   const int Bmt = B.mt, Bnt = B.nt;
-  const char tr = is_trans(trans) ? 'T' : 'N';
+  const char tr = trans_char(trans, dt);
   auto trsm = [&](double al, int ai, int bi, int bj) {
     add_trsm(d, dt, side, uplo, tr, diag, al, A.ptr(ai, ai), A.ld(ai), B.ptr(bi, bj), B.rows(bi),
              B.cols(bj), B.ld(bi));
@@ -326,7 +335,8 @@ void build_gemm_tiled(Dag& d, char ta, char tb, double alpha, const dm_tiled& A,
         const int ai = tra ? l : i, aj = tra ? i : l;
         const int bi = trb ? j : l, bj = trb ? l : j;
         const int kk = tra ? va.rows(l) : va.cols(l);
-        add_gemm(d, vc.dtype, tra ? 'T' : 'N', trb ? 'T' : 'N', vc.rows(i), vc.cols(j), kk, alpha,
+        add_gemm(d, vc.dtype, trans_char(ta, vc.dtype), trans_char(tb, vc.dtype), vc.rows(i),
+                 vc.cols(j), kk, alpha,
                  va.ptr(ai, aj), va.ld(ai), vb.ptr(bi, bj), vb.ld(bi), l == 0 ? beta : 1.0,
                  vc.ptr(i, j), vc.ld(i));
       }
@@ -518,7 +528,7 @@ int dm_trsm_tiled(char side, char uplo, char trans, char diag, double alpha, con
   TileView ta(*A), tb(*B);
   const char up = is_upper(uplo) ? 'U' : 'L';
   const char sd = is_left(side) ? 'L' : 'R';
-  const char tr = is_trans(trans) ? 'T' : 'N';
+  const char tr = trans_char(trans, A->dtype);
   const char dg = is_unit(diag) ? 'U' : 'N';
   char tag[16];
   snprintf(tag, sizeof tag, "trsm%c%c%c%c", sd, up, tr, dg);
@@ -559,7 +569,13 @@ int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B, const dm_pla
   Dag* d = nullptr;
   rc = get_plan(key, {A->base, B->base}, o,
                 [&](Dag& dag) {
-                  // inverse diagonal tiles, then the two sweeps of GEMM tasks
+                  // inverse diagonal tiles, then the two sweeps of GEMM tasks;
+                  // complex factors: the two trsm! regions (linalg.jl:385-401)
+                  if (is_cplx(A->dtype)) {
+                    build_trsm(dag, 'L', up, up == 'U' ? 'C' : 'N', 'N', 1.0, ta, tb);
+                    build_trsm(dag, 'L', up, up == 'U' ? 'N' : 'C', 'N', 1.0, ta, tb);
+                    return 0;
+                  }
                   return A->dtype == DM_F64 ? setup_potrs_inv<double>(dag, up, ta, tb)
                                             : setup_potrs_inv<float>(dag, up, ta, tb);
                 },
@@ -570,8 +586,8 @@ int dm_potrs_tiled(char uplo, const dm_tiled* A, const dm_tiled* B, const dm_pla
 
 int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A, const dm_tiled* B,
                   double beta, const dm_tiled* Cm, const dm_plan_opts* opts, dm_stream_t s) {
-  if (!is_trans(transa) && !is_notrans(transa)) return arg_fail(1, "transa must be 'N','T'");
-  if (!is_trans(transb) && !is_notrans(transb)) return arg_fail(2, "transb must be 'N','T'");
+  if (!is_trans(transa) && !is_notrans(transa)) return arg_fail(1, "transa must be 'N','T','C'");
+  if (!is_trans(transb) && !is_notrans(transb)) return arg_fail(2, "transb must be 'N','T','C'");
   int rc = check_tiled(A, 4);
   if (rc) return rc;
   rc = check_tiled(B, 5);
@@ -592,10 +608,10 @@ int dm_gemm_tiled(char transa, char transb, double alpha, const dm_tiled* A, con
   if (rc) return rc;
   if (Cm->m == 0 || Cm->n == 0) return 0;
   const dm_plan_opts o = opts_or_default(opts);
-  const char ta = is_trans(transa) ? 'T' : 'N', tb = is_trans(transb) ? 'T' : 'N';
+  const char ta = trans_char(transa, A->dtype), tb = trans_char(transb, A->dtype);
   // A times a block column of at most 64 columns (the residual of
   // refinement): one streaming launch instead of the k-ordered tile DAG
-  if (tiled_skinny_gemm_ok(ta, tb, *A, *B, *Cm))
+  if (!is_cplx(A->dtype) && tiled_skinny_gemm_ok(ta, tb, *A, *B, *Cm))
     return tiled_skinny_gemm_launch(*A, *B, *Cm, alpha, beta, stream_of(s));
   int64_t ab, bb;
   std::memcpy(&ab, &alpha, sizeof ab);
@@ -621,6 +637,7 @@ int dm_getrf_tiled(int pivot, const dm_tiled* A, int* ipiv_dev, int* info_dev,
   if (pivot != 0 && pivot != 1) return arg_fail(1, "pivot must be 0 (NoPivot) or 1 (RowMaximum)");
   int rc = check_tiled(A, 2);
   if (rc) return rc;
+  if (is_cplx(A->dtype)) return arg_fail(2, "LU of complex element types is not built");
   if (A->mb != A->nb) return arg_fail(2, "tiles must be square (Blocks(nb, nb))");
   if (!ipiv_dev && A->m > 0 && A->n > 0) return arg_fail(3, "ipiv is NULL");
   if (!info_dev) return arg_fail(4, "info is NULL");
@@ -648,6 +665,7 @@ int dm_getrf_tiled(int pivot, const dm_tiled* A, int* ipiv_dev, int* info_dev,
 int dm_tiled_laswp(const dm_tiled* B, const int* ipiv_dev, int npiv, dm_stream_t s) {
   int rc = check_tiled(B, 1);
   if (rc) return rc;
+  if (is_cplx(B->dtype)) return arg_fail(1, "row swaps of complex element types are not built");
   if (npiv < 0 || npiv > B->m) return arg_fail(3, "npiv must be in [0, rows of B]");
   if (!ipiv_dev && npiv > 0) return arg_fail(2, "ipiv is NULL");
   rc = ensure_init();
@@ -691,12 +709,13 @@ int dm_dag_build_potrs(dm_dag_t* dag, char uplo, const dm_tiled* A, const dm_til
     return arg_fail(4, "B does not conform to A");
   std::unique_ptr<Dag> d(new Dag());
   TileView ta(*A), tb(*B);
+  const char tc = is_cplx(A->dtype) ? 'C' : 'T';
   if (is_upper(uplo)) {
-    build_trsm(*d, 'L', 'U', 'T', 'N', 1.0, ta, tb);
+    build_trsm(*d, 'L', 'U', tc, 'N', 1.0, ta, tb);
     build_trsm(*d, 'L', 'U', 'N', 'N', 1.0, ta, tb);
   } else {
     build_trsm(*d, 'L', 'L', 'N', 'N', 1.0, ta, tb);
-    build_trsm(*d, 'L', 'L', 'T', 'N', 1.0, ta, tb);
+    build_trsm(*d, 'L', 'L', tc, 'N', 1.0, ta, tb);
   }
   *dag = new dm_dag_s{std::move(d)};
   return 0;

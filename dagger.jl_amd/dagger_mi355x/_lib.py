@@ -11,7 +11,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libdagger_mi355x.so")
 
-DM_F64, DM_F32 = 0, 1
+DM_F64, DM_F32, DM_C64, DM_C32 = 0, 1, 2, 3
 DM_OP_POTRF, DM_OP_TRSM, DM_OP_SYRK, DM_OP_GEMM, DM_OP_TRSV, DM_OP_GEMV = 1, 2, 3, 4, 5, 6
 DM_OP_GETRF, DM_OP_LASWP = 7, 8
 DM_IN, DM_OUT, DM_INOUT = 1, 2, 3
@@ -99,6 +99,14 @@ _PROTOS = {
     "dm_dsyrk": (i32, [c_char, c_char, i32, i32, f64, vp, i32, f64, vp, i32, vp]),
     "dm_ssyrk": (i32, [c_char, c_char, i32, i32, f32, vp, i32, f32, vp, i32, vp]),
     "dm_dgemm": (i32, [c_char, c_char, i32, i32, i32, f64, vp, i32, vp, i32, f64, vp, i32, vp]),
+    "dm_zpotrf": (i32, [c_char, i32, vp, i32, vp, vp]),
+    "dm_cpotrf": (i32, [c_char, i32, vp, i32, vp, vp]),
+    "dm_ztrsm": (i32, [c_char, c_char, c_char, c_char, i32, i32, vp, vp, i32, vp, i32, vp]),
+    "dm_ctrsm": (i32, [c_char, c_char, c_char, c_char, i32, i32, vp, vp, i32, vp, i32, vp]),
+    "dm_zherk": (i32, [c_char, c_char, i32, i32, f64, vp, i32, f64, vp, i32, vp]),
+    "dm_cherk": (i32, [c_char, c_char, i32, i32, f32, vp, i32, f32, vp, i32, vp]),
+    "dm_zgemm": (i32, [c_char, c_char, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp]),
+    "dm_cgemm": (i32, [c_char, c_char, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32, vp]),
     "dm_sgemm": (i32, [c_char, c_char, i32, i32, i32, f32, vp, i32, vp, i32, f32, vp, i32, vp]),
     "dm_sgemm_bf16x": (i32, [i32, c_char, c_char, i32, i32, i32, f32, vp, i32, vp, i32, f32, vp,
                              i32, vp]),

@@ -21,7 +21,8 @@ import numpy as np
 from . import _lib
 from ._lib import call
 
-_DTYPES = {np.dtype(np.float64): _lib.DM_F64, np.dtype(np.float32): _lib.DM_F32}
+_DTYPES = {np.dtype(np.float64): _lib.DM_F64, np.dtype(np.float32): _lib.DM_F32,
+           np.dtype(np.complex128): _lib.DM_C64, np.dtype(np.complex64): _lib.DM_C32}
 
 
 class Blocks:

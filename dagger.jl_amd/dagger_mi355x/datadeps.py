@@ -62,6 +62,7 @@ class _Namespace:
 BLAS = _Namespace(
     gemm=_TileOp("BLAS.gemm!", _lib.DM_OP_GEMM),
     syrk=_TileOp("BLAS.syrk!", _lib.DM_OP_SYRK),
+    herk=_TileOp("BLAS.herk!", _lib.DM_OP_SYRK),  # complex tiles: C := a A^H A + b C
     trsm=_TileOp("BLAS.trsm!", _lib.DM_OP_TRSM),
     gemv=_TileOp("BLAS.gemv!", _lib.DM_OP_GEMV),
     trsv=_TileOp("BLAS.trsv!", _lib.DM_OP_TRSV),
@@ -69,7 +70,8 @@ BLAS = _Namespace(
 LAPACK = _Namespace(potrf=_TileOp("LAPACK.potrf!", _lib.DM_OP_POTRF))
 potrf_checked = _TileOp("potrf_checked!", _lib.DM_OP_POTRF)
 
-_DT = {np.dtype(np.float64): _lib.DM_F64, np.dtype(np.float32): _lib.DM_F32}
+_DT = {np.dtype(np.float64): _lib.DM_F64, np.dtype(np.float32): _lib.DM_F32,
+       np.dtype(np.complex128): _lib.DM_C64, np.dtype(np.complex64): _lib.DM_C32}
 
 
 def _unwrap(a):

@@ -54,12 +54,12 @@ class Cholesky:
     @property
     def U(self):
         F = self.factors.collect()
-        return np.triu(F) if self.uplo == "U" else np.triu(F.T)
+        return np.triu(F) if self.uplo == "U" else np.triu(F.conj().T)
 
     @property
     def L(self):
         F = self.factors.collect()
-        return np.tril(F) if self.uplo == "L" else np.tril(F.T)
+        return np.tril(F) if self.uplo == "L" else np.tril(F.conj().T)
 
     def issuccess(self):
         return self.info == 0
@@ -87,28 +87,32 @@ def _maybe_copy_buffered(pairs, f):
 
 
 def _chol_datadeps(Ac, uplo, info, options):
-    """The loop nest of src/array/cholesky.jl:38-53 (U) / :83-98 (L)."""
+    """The loop nest of src/array/cholesky.jl:38-53 (U) / :83-98 (L); for
+    complex tiles trans = 'C' and herk! in place of syrk! (:27-28, :44-45)."""
     mt, nt = Ac.shape
+    iscomplex = np.issubdtype(Ac[0, 0].dtype, np.complexfloating)
+    trans = "C" if iscomplex else "T"
+    rk = BLAS.herk if iscomplex else BLAS.syrk
     with spawn_datadeps(options=options):
         if uplo == "U":
             for k in range(mt):
                 spawn(potrf_checked, "U", InOut(Ac[k, k]), Out(info))
                 for n in range(k + 1, nt):
-                    spawn(BLAS.trsm, "L", "U", "T", "N", 1.0, In(Ac[k, k]), InOut(Ac[k, n]))
+                    spawn(BLAS.trsm, "L", "U", trans, "N", 1.0, In(Ac[k, k]), InOut(Ac[k, n]))
                 for m in range(k + 1, mt):
-                    spawn(BLAS.syrk, "U", "T", -1.0, In(Ac[k, m]), 1.0, InOut(Ac[m, m]))
+                    spawn(rk, "U", trans, -1.0, In(Ac[k, m]), 1.0, InOut(Ac[m, m]))
                     for n in range(m + 1, nt):
-                        spawn(BLAS.gemm, "T", "N", -1.0, In(Ac[k, m]), In(Ac[k, n]), 1.0,
+                        spawn(BLAS.gemm, trans, "N", -1.0, In(Ac[k, m]), In(Ac[k, n]), 1.0,
                               InOut(Ac[m, n]))
         else:
             for k in range(mt):
                 spawn(potrf_checked, "L", InOut(Ac[k, k]), Out(info))
                 for m in range(k + 1, mt):
-                    spawn(BLAS.trsm, "R", "L", "T", "N", 1.0, In(Ac[k, k]), InOut(Ac[m, k]))
+                    spawn(BLAS.trsm, "R", "L", trans, "N", 1.0, In(Ac[k, k]), InOut(Ac[m, k]))
                 for n in range(k + 1, nt):
-                    spawn(BLAS.syrk, "L", "N", -1.0, In(Ac[n, k]), 1.0, InOut(Ac[n, n]))
+                    spawn(rk, "L", "N", -1.0, In(Ac[n, k]), 1.0, InOut(Ac[n, n]))
                     for m in range(n + 1, mt):
-                        spawn(BLAS.gemm, "N", "T", -1.0, In(Ac[m, k]), In(Ac[n, k]), 1.0,
+                        spawn(BLAS.gemm, "N", trans, -1.0, In(Ac[m, k]), In(Ac[n, k]), 1.0,
                               InOut(Ac[m, n]))
 
 
@@ -382,6 +386,19 @@ def ldiv_(F, B, via="native", options=None):
     pA = F.factors
     bs = min(pA.partitioning.blocksize)
     sq = Blocks(bs, bs)
+    iscomplex = np.issubdtype(pA.dtype, np.complexfloating)
+    tc = "C" if iscomplex else "T"
+    if B.ndim == 1 and iscomplex:
+        # a complex vector solves as the n x 1 matrix over the same storage
+        # (the vector kernels are real); its Blocks(bs) tiles are the column's
+        if B.partitioning != Blocks(bs):
+            Bv = B.reblock(Blocks(bs))
+            ldiv_(F, Bv, via=via, options=options)
+            Bv.copy_into(B)
+            Bv.unsafe_free()
+            return B
+        ldiv_(F, DArray(B.dtype, (B.m, 1), sq, _buffer=B.buffer), via=via, options=options)
+        return B
     if B.ndim == 1:
         partB = Blocks(bs)
 
@@ -404,11 +421,11 @@ def ldiv_(F, B, via="native", options=None):
             call("dm_potrs_tiled", F.uplo.encode(), C.byref(ta), C.byref(tb), C.byref(o), None)
             call("dm_stream_sync", None)
         elif F.uplo == "U":
-            _trsm_datadeps("L", "U", "T", "N", 1.0, pa.chunks, pb.chunks, options)
+            _trsm_datadeps("L", "U", tc, "N", 1.0, pa.chunks, pb.chunks, options)
             _trsm_datadeps("L", "U", "N", "N", 1.0, pa.chunks, pb.chunks, options)
         else:
             _trsm_datadeps("L", "L", "N", "N", 1.0, pa.chunks, pb.chunks, options)
-            _trsm_datadeps("L", "L", "T", "N", 1.0, pa.chunks, pb.chunks, options)
+            _trsm_datadeps("L", "L", tc, "N", 1.0, pa.chunks, pb.chunks, options)
 
     _maybe_copy_buffered([(pA, sq), (B, partB)], mat)
     return B

@@ -46,8 +46,10 @@ typedef struct dm_event_s* dm_event_t;
 typedef struct dm_graph_s* dm_graph_t;
 typedef struct dm_dag_s* dm_dag_t;
 
-/* element types */
-enum { DM_F64 = 0, DM_F32 = 1 };
+/* element types.  Complex elements are (re, im) pairs, Julia's ComplexF64 /
+ * ComplexF32 layout: a complex m x n column-major array is bit-identical to
+ * a real 2m x n one, and a Blocks(mb, nb) tiling to Blocks(2mb, nb). */
+enum { DM_F64 = 0, DM_F32 = 1, DM_C64 = 2, DM_C32 = 3 };
 
 /* ---------------------------------------------------------------- runtime */
 
@@ -155,6 +157,36 @@ int dm_dgemm(char transa, char transb, int m, int n, int k, double alpha,
 int dm_sgemm(char transa, char transb, int m, int n, int k, float alpha,
              const float* A, int lda, const float* B, int ldb, float beta,
              float* C, int ldc, dm_stream_t s);
+/* Complex tile ops (ComplexF64: z, ComplexF32: c).  A, B, C point to
+ * interleaved (re, im) pairs; sizes and leading dimensions count complex
+ * elements.  The BLAS/LAPACK calls _chol! and trsm! make for T <: Complex
+ * (src/array/cholesky.jl:40-50,85-95; src/array/trsm.jl:90-183):
+ *   potrf: A = U^H U ('U') or L L^H ('L'), LAPACK info as dm_dpotrf; the
+ *          imaginary part of A's diagonal is ignored and zeroed;
+ *   trsm : transa 'N', 'T' or 'C' (conjugate transpose); alpha points to a
+ *          host complex scalar {re, im};
+ *   herk : C := alpha op(A) op(A)^H + beta C on the uplo triangle, trans
+ *          'N' (op(A) = A) or 'C' (op(A) = A^H), real alpha and beta, the
+ *          imaginary part of C's diagonal set to zero (BLAS herk!);
+ *   gemm : C := alpha op(A) op(B) + beta C, op 'N', 'T' or 'C'; alpha and
+ *          beta point to host complex scalars. */
+int dm_zpotrf(char uplo, int n, void* A, int lda, int* info_dev, dm_stream_t s);
+int dm_cpotrf(char uplo, int n, void* A, int lda, int* info_dev, dm_stream_t s);
+int dm_ztrsm(char side, char uplo, char transa, char diag, int m, int n,
+             const double* alpha, const void* A, int lda, void* B, int ldb, dm_stream_t s);
+int dm_ctrsm(char side, char uplo, char transa, char diag, int m, int n,
+             const float* alpha, const void* A, int lda, void* B, int ldb, dm_stream_t s);
+int dm_zherk(char uplo, char trans, int n, int k, double alpha, const void* A, int lda,
+             double beta, void* C, int ldc, dm_stream_t s);
+int dm_cherk(char uplo, char trans, int n, int k, float alpha, const void* A, int lda,
+             float beta, void* C, int ldc, dm_stream_t s);
+int dm_zgemm(char transa, char transb, int m, int n, int k, const double* alpha,
+             const void* A, int lda, const void* B, int ldb, const double* beta, void* C,
+             int ldc, dm_stream_t s);
+int dm_cgemm(char transa, char transb, int m, int n, int k, const float* alpha,
+             const void* A, int lda, const void* B, int ldb, const float* beta, void* C,
+             int ldc, dm_stream_t s);
+
 /* FP32 GEMM on the BF16 matrix cores: every FP32 operand is split into
  * `terms` == 6: three BF16 parts (products hh,hm,mh,hl,lh,mm) or
  * `terms` == 3: two parts (hh,hm,mh), accumulated in FP32.  No reference

@@ -98,31 +98,50 @@ class TiledMatrix:
 # Each op mutates its output tile in place, like BLAS.xxx!/LAPACK.potrf!.
 
 def _tr(t):
-    return 1 if t in ("T", "C") else 0
+    """BLAS trans code: 'N' 0, 'T' 1, 'C' 2 (the conjugate transpose; for
+    real element types 'C' is 'T')."""
+    return {"N": 0, "T": 1, "C": 2}[t]
+
+
+_PREFIX = {np.dtype(np.float64): "d", np.dtype(np.float32): "s",
+           np.dtype(np.complex128): "z", np.dtype(np.complex64): "c"}
+
+
+def _blas(name, dtype):
+    return getattr(blas, _PREFIX[np.dtype(dtype)] + name)
 
 
 def tile_potrf(uplo, A):
-    """LAPACK.potrf!(uplo, A) -> info (LAPACK convention)."""
-    f = lapack.dpotrf if A.dtype == np.float64 else lapack.spotrf
+    """LAPACK.potrf!(uplo, A) -> info (LAPACK convention; complex: the
+    Hermitian A = U^H U / L L^H)."""
+    f = getattr(lapack, _PREFIX[A.dtype] + "potrf")
     c, info = f(A, lower=1 if uplo == "L" else 0, clean=0, overwrite_a=0)
     A[...] = c
     return int(info)
 
 
 def tile_trsm(side, uplo, trans, diag, alpha, A, B):
-    f = blas.dtrsm if B.dtype == np.float64 else blas.strsm
-    B[...] = f(alpha, A, B, side=0 if side == "L" else 1, lower=1 if uplo == "L" else 0,
-               trans_a=_tr(trans), diag=1 if diag == "U" else 0)
+    B[...] = _blas("trsm", B.dtype)(alpha, A, B, side=0 if side == "L" else 1,
+                                     lower=1 if uplo == "L" else 0, trans_a=_tr(trans),
+                                     diag=1 if diag == "U" else 0)
 
 
 def tile_syrk(uplo, trans, alpha, A, beta, C):
-    f = blas.dsyrk if C.dtype == np.float64 else blas.ssyrk
-    C[...] = f(alpha, A, beta=beta, c=C, trans=_tr(trans), lower=1 if uplo == "L" else 0)
+    C[...] = _blas("syrk", C.dtype)(alpha, A, beta=beta, c=C, trans=_tr(trans),
+                                    lower=1 if uplo == "L" else 0)
+
+
+def tile_herk(uplo, trans, alpha, A, beta, C):
+    """BLAS.herk!(uplo, trans, alpha, A, beta, C) on complex tiles: trans 'N'
+    C = a A A^H + b C, 'C' C = a A^H A + b C on the uplo triangle, with the
+    imaginary part of the diagonal set to zero."""
+    C[...] = _blas("herk", C.dtype)(alpha, A, beta=beta, c=C, trans=_tr(trans),
+                                    lower=1 if uplo == "L" else 0)
 
 
 def tile_gemm(ta, tb, alpha, A, B, beta, C):
-    f = blas.dgemm if C.dtype == np.float64 else blas.sgemm
-    C[...] = f(alpha, A, B, beta=beta, c=C, trans_a=_tr(ta), trans_b=_tr(tb))
+    C[...] = _blas("gemm", C.dtype)(alpha, A, B, beta=beta, c=C, trans_a=_tr(ta),
+                                    trans_b=_tr(tb))
 
 
 def tile_gemv(trans, alpha, A, x, beta, y):
@@ -281,30 +300,33 @@ def chol_region(T, uplo, info, steps=None):
             fn(*a)
         return g
 
-    trsm, syrk, gemm = guarded(tile_trsm), guarded(tile_syrk), guarded(tile_gemm)
+    iscomplex = np.issubdtype(Ac[0, 0].dtype, np.complexfloating)
+    tr = "C" if iscomplex else "T"  # src/array/cholesky.jl:27-28
+    trsm, gemm = guarded(tile_trsm), guarded(tile_gemm)
+    syrk = guarded(tile_herk if iscomplex else tile_syrk)  # herk! for complex (:44-48)
     if uplo == "U":
         for k in range(kmax):
             R.spawn("potrf", potrf_checked, ("U", Ac[k, k], info), [(Ac[k, k], INOUT), (info, OUT)])
             for n in range(k + 1, nt):
-                R.spawn("trsm", trsm, ("L", "U", "T", "N", 1.0, Ac[k, k], Ac[k, n]),
+                R.spawn("trsm", trsm, ("L", "U", tr, "N", 1.0, Ac[k, k], Ac[k, n]),
                         [(Ac[k, k], IN), (Ac[k, n], INOUT)])
             for m in range(k + 1, mt):
-                R.spawn("syrk", syrk, ("U", "T", -1.0, Ac[k, m], 1.0, Ac[m, m]),
+                R.spawn("syrk", syrk, ("U", tr, -1.0, Ac[k, m], 1.0, Ac[m, m]),
                         [(Ac[k, m], IN), (Ac[m, m], INOUT)])
                 for n in range(m + 1, nt):
-                    R.spawn("gemm", gemm, ("T", "N", -1.0, Ac[k, m], Ac[k, n], 1.0, Ac[m, n]),
+                    R.spawn("gemm", gemm, (tr, "N", -1.0, Ac[k, m], Ac[k, n], 1.0, Ac[m, n]),
                             [(Ac[k, m], IN), (Ac[k, n], IN), (Ac[m, n], INOUT)])
     else:
         for k in range(kmax):
             R.spawn("potrf", potrf_checked, ("L", Ac[k, k], info), [(Ac[k, k], INOUT), (info, OUT)])
             for m in range(k + 1, mt):
-                R.spawn("trsm", trsm, ("R", "L", "T", "N", 1.0, Ac[k, k], Ac[m, k]),
+                R.spawn("trsm", trsm, ("R", "L", tr, "N", 1.0, Ac[k, k], Ac[m, k]),
                         [(Ac[k, k], IN), (Ac[m, k], INOUT)])
             for n in range(k + 1, nt):
                 R.spawn("syrk", syrk, ("L", "N", -1.0, Ac[n, k], 1.0, Ac[n, n]),
                         [(Ac[n, k], IN), (Ac[n, n], INOUT)])
                 for m in range(n + 1, mt):
-                    R.spawn("gemm", gemm, ("N", "T", -1.0, Ac[m, k], Ac[n, k], 1.0, Ac[m, n]),
+                    R.spawn("gemm", gemm, ("N", tr, -1.0, Ac[m, k], Ac[n, k], 1.0, Ac[m, n]),
                             [(Ac[m, k], IN), (Ac[n, k], IN), (Ac[m, n], INOUT)])
     return R
 
@@ -338,7 +360,9 @@ def trsm_region(side, uplo, trans, diag, alpha, TA, TB):
     Ac, Bc = TA.chunks, TB.chunks
     Bmt, Bnt = Bc.shape
     R = Region()
-    tr = "T" if trans in ("T", "C") else "N"
+    # the reference hands trans through to its tile calls ('C' is the
+    # conjugate transpose on complex tiles, the same as 'T' on real ones)
+    tr = trans if np.iscomplexobj(TB.chunks[0, 0]) else ("T" if trans in ("T", "C") else "N")
 
     def trsm(al, a, b):
         R.spawn("trsm", tile_trsm, (side, uplo, tr, diag, al, a, b), [(a, IN), (b, INOUT)])
@@ -511,6 +535,8 @@ def ldiv(F, uplo, B, nb, threaded=False, workers=None):
     B = np.asarray(B)
     TA = TiledMatrix(np.array(F, copy=True), nb, nb)
     run = (lambda R: run_threaded(R, workers)) if threaded else run_sequential
+    if np.iscomplexobj(F) and B.ndim == 1:
+        return ldiv(F, uplo, B.reshape(-1, 1), nb, threaded, workers).ravel()  # n x 1 form
     if B.ndim == 1:
         TB = TiledMatrix(np.array(B, copy=True), nb)
         if uplo == "U":
@@ -520,13 +546,14 @@ def ldiv(F, uplo, B, nb, threaded=False, workers=None):
             run(trsv_region("L", "N", "N", TA, TB))
             run(trsv_region("L", "T", "N", TA, TB))
         return TB.collect().ravel()
+    tc = "C" if np.iscomplexobj(F) else "T"  # Uᴴ / Lᴴ for complex factors
     TB = TiledMatrix(np.array(B, copy=True), nb, nb)
     if uplo == "U":
-        run(trsm_region("L", "U", "T", "N", 1.0, TA, TB))
+        run(trsm_region("L", "U", tc, "N", 1.0, TA, TB))
         run(trsm_region("L", "U", "N", "N", 1.0, TA, TB))
     else:
         run(trsm_region("L", "L", "N", "N", 1.0, TA, TB))
-        run(trsm_region("L", "L", "T", "N", 1.0, TA, TB))
+        run(trsm_region("L", "L", tc, "N", 1.0, TA, TB))
     return TB.collect()
 
 

@@ -545,7 +545,102 @@ def quick():
               ("batched", m))
 
 
-GROUPS = dict(quick=quick, tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
+def complex_ops():
+    """Complex tile ops (ComplexF64 / ComplexF32) on the emulation build:
+    every op/side/uplo/trans form at shapes that cross the 64-wide leaves,
+    against numpy; then a small tiled complex Cholesky and its solve."""
+    rng = np.random.default_rng(9)
+
+    def crand(*shape, dt=np.complex128):
+        return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dt)
+
+    def op(X, t):
+        return X if t == "N" else (X.T if t == "T" else X.conj().T)
+
+    for dt, pre, tol in ((np.complex128, "z", 1e-12), (np.complex64, "c", 1e-4)):
+        R = np.float64 if dt == np.complex128 else np.float32
+        al, be = np.array([0.5, -0.25], R), np.array([1.5, 0.5], R)
+        a, b = complex(0.5, -0.25), complex(1.5, 0.5)
+        for ta, tb in (("N", "N"), ("C", "N"), ("N", "C"), ("T", "C")):
+            m, n, k = 70, 45, 37
+            A = crand(*((m, k) if ta == "N" else (k, m)), dt=dt)
+            B = crand(*((k, n) if tb == "N" else (n, k)), dt=dt)
+            Cm = crand(m, n, dt=dt)
+            dA, dB, dC = Dev(A), Dev(B), Dev(Cm)
+            call(f"dm_{pre}gemm", ta.encode(), tb.encode(), m, n, k, al.ctypes.data_as(C.c_void_p),
+                 dA.p, A.shape[0], dB.p, B.shape[0], be.ctypes.data_as(C.c_void_p), dC.p, m, None)
+            ref = a * op(A, ta).astype(np.complex128) @ op(B, tb) + b * Cm
+            check(relerr(dC.get(), ref) <= tol, (pre, "gemm", ta, tb))
+        for uplo, tr in (("U", "C"), ("L", "N")):
+            n, k = 90, 33
+            A = crand(*((k, n) if tr == "C" else (n, k)), dt=dt)
+            Cm = crand(n, n, dt=dt)
+            dA, dC = Dev(A), Dev(Cm)
+            fn = C.c_double if dt == np.complex128 else C.c_float
+            call(f"dm_{pre}herk", uplo.encode(), tr.encode(), n, k, fn(-1.0), dA.p, A.shape[0],
+                 fn(1.0), dC.p, n, None)
+            P = op(A, tr).astype(np.complex128)
+            full = Cm - P @ P.conj().T
+            tri = np.triu if uplo == "U" else np.tril
+            got = dC.get()
+            want = tri(full) + (np.tril(Cm, -1) if uplo == "U" else np.triu(Cm, 1))
+            np.fill_diagonal(want, np.diag(want).real)  # herk: real diagonal
+            check(relerr(got, want) <= tol, (pre, "herk", uplo, tr))
+        for side in "LR":
+            for uplo in "UL":
+                for tr in "NTC":
+                    m, n = (100, 7) if side == "L" else (9, 100)
+                    ka = m if side == "L" else n
+                    T = crand(ka, ka, dt=dt) * 0.05 + np.diag(np.full(ka, 2.0 + 0.5j)).astype(dt)
+                    T = np.triu(T) if uplo == "U" else np.tril(T)
+                    B = crand(m, n, dt=dt)
+                    dT, dB = Dev(T), Dev(B)
+                    call(f"dm_{pre}trsm", side.encode(), uplo.encode(), tr.encode(), b"N", m, n,
+                         al.ctypes.data_as(C.c_void_p), dT.p, ka, dB.p, m, None)
+                    X = dB.get().astype(np.complex128)
+                    Tm = op(T, tr).astype(np.complex128)
+                    lhs = Tm @ X if side == "L" else X @ Tm
+                    check(relerr(lhs, a * B) <= tol, (pre, "trsm", side, uplo, tr))
+        for uplo in "UL":
+            n = 100
+            G = crand(n, n, dt=np.complex128)
+            A = (G @ G.conj().T / n + np.eye(n)).astype(dt)
+            dA = Dev(A)
+            info = dm.DeviceInt(1)
+            call(f"dm_{pre}potrf", uplo.encode(), n, dA.p, n, C.c_void_p(info.ptr), None)
+            check(int(info.get()[0]) == 0, (pre, "potrf info"))
+            L = np.linalg.cholesky(A.astype(np.complex128))
+            ref = L.conj().T if uplo == "U" else L
+            tri = np.triu if uplo == "U" else np.tril
+            check(relerr(tri(dA.get()), ref) <= tol, (pre, "potrf", uplo))
+            Bad = A.copy()
+            Bad[70, 70] = -1.0  # first failure at column 71
+            dB = Dev(Bad)
+            call(f"dm_{pre}potrf", uplo.encode(), n, dB.p, n, C.c_void_p(info.ptr), None)
+            check(int(info.get()[0]) == 71, (pre, "potrf failed column", int(info.get()[0])))
+    # tiled complex Cholesky (ragged tiles) and ldiv!, native and datadeps
+    n, nb = 150, 48
+    G = crand(n, n)
+    A = G @ G.conj().T / n + np.eye(n)
+    A = 0.5 * (A + A.conj().T)  # exactly Hermitian (a GEMM product is not)
+    X0 = crand(n, 3)
+    for uplo in "UL":
+        for via in ("native", "datadeps"):
+            DA = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+            check(dm.ishermitian(DA), "complex ishermitian")
+            F = dm.cholesky_(DA, uplo, via=via)
+            Lref = np.linalg.cholesky(A)
+            got = F.U if uplo == "U" else F.L
+            ref = Lref.conj().T if uplo == "U" else Lref
+            check(relerr(got, ref) <= 1e-12, ("complex chol", uplo, via))
+            DB = dm.DArray.from_array(A @ X0, dm.Blocks(nb, nb))
+            dm.ldiv_(F, DB, via=via)
+            check(relerr(DB.collect(), X0) <= 1e-10, ("complex ldiv", uplo, via))
+            xv = dm.ldiv(F, (A @ X0)[:, 0]).collect()
+            check(relerr(xv, X0[:, 0]) <= 1e-10, ("complex vector ldiv", uplo, via))
+
+
+GROUPS = dict(quick=quick, complex_ops=complex_ops, tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
               capped=capped)
 
 if __name__ == "__main__":

@@ -6,6 +6,7 @@
 #include <string>
 #include <hip/hip_runtime.h>
 #include "dm_internal.h"
+#include "complex_ops.h"
 #include "potrs.h"
 static int g_streams = 0, g_events = 0;
 // Optional trace of the stream operations (environment DM_HARNESS_TRACE =
@@ -154,7 +155,14 @@ int laswp_tiled(T* base, int64_t Mr, int W, int mb, const int* ipiv, int npiv, i
   ++g_launch;
   return 0;
 }
+// complex tile operations (complex.hip): one traced launch each
+template <typename R> int zgemm_call(char, char, int, int, int, R, R, const R*, int, const R*, int, R, R, R*, int, int, int, const int*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
+template <typename R> int ztrsm_call(char, char, char, char, int, int, R, R, const R*, int, R*, int, const int*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
+template <typename R> int zpotrf_rec(bool, int, R*, int, int*, int, int*, int, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
 #define INST(T) \
+  template int zgemm_call<T>(char, char, int, int, int, T, T, const T*, int, const T*, int, T, T, T*, int, int, int, const int*, hipStream_t); \
+  template int ztrsm_call<T>(char, char, char, char, int, int, T, T, const T*, int, T*, int, const int*, hipStream_t); \
+  template int zpotrf_rec<T>(bool, int, T*, int, int*, int, int*, int, hipStream_t); \
   template GemmCfgId gemm_pick_cfg<T>(int, int, int, int, bool); \
   template long gemm_prepare_jobs<T>(GemmJob<T>*, int, GemmCfgId); \
   template bool gemm_job_vec_ok<T>(bool, bool, const GemmJob<T>&); \

@@ -453,3 +453,58 @@ def test_cholesky_leading_steps_sample():
     S = A[nb:, nb:] - U01.T @ U01
     assert np.allclose(np.triu(F1[nb:, nb:]), np.triu(S), rtol=0, atol=1e-12)
     assert np.array_equal(np.tril(F1, -1), np.tril(A, -1))  # lower part untouched
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.complex64])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_complex_cholesky_matches_dense_lapack(T, uplo):
+    """The restatement for T <: Complex (trans 'C', herk!, ragged tiles)
+    equals the dense factor (numpy's zpotrf route) and reconstructs A."""
+    rng = np.random.default_rng(41)
+    n, nb = 173, 48
+    G = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = G @ G.conj().T / n + np.eye(n)
+    A = (0.5 * (A + A.conj().T)).astype(T)
+    F, info = O.cholesky(A, nb, uplo)
+    assert info == 0
+    L = np.linalg.cholesky(A.astype(np.complex128))
+    ref = L.conj().T if uplo == "U" else L
+    tol = 1e-13 if T == np.complex128 else 2e-6
+    tri = np.triu if uplo == "U" else np.tril
+    assert np.abs(tri(F) - ref).max() <= tol * np.abs(ref).max()
+    U = np.triu(F) if uplo == "U" else np.tril(F).conj().T
+    assert np.abs(U.conj().T @ U - A).max() <= 10 * tol * np.abs(A).max()
+    assert np.all(np.diag(F).imag == 0)
+
+
+def test_complex_herk_zeroes_the_imaginary_diagonal_and_planted_transpose_fails():
+    """herk! on a tile: C - A^H A on the triangle with a real diagonal; and
+    the restatement with 'T' in place of 'C' (a plausible slip) no longer
+    factors A, which the dense comparison catches."""
+    rng = np.random.default_rng(43)
+    A = rng.standard_normal((30, 12)) + 1j * rng.standard_normal((30, 12))
+    Cm = rng.standard_normal((12, 12)) + 1j * rng.standard_normal((12, 12))
+    got = np.asfortranarray(Cm.copy())
+    O.tile_herk("U", "C", -1.0, A, 1.0, got)
+    want = Cm - A.conj().T @ A
+    assert np.allclose(np.triu(got, 1), np.triu(want, 1), atol=1e-13)
+    assert np.allclose(np.diag(got).real, np.diag(want).real, atol=1e-13)
+    assert np.all(np.diag(got).imag == 0)
+    assert np.array_equal(np.tril(got, -1), np.tril(Cm, -1))
+    n, nb = 96, 32
+    G = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = G @ G.conj().T / n + np.eye(n)
+    H = 0.5 * (H + H.conj().T)
+    T = O.TiledMatrix(np.array(H, copy=True), nb, nb)
+    info = np.zeros(1, dtype=np.int64)
+    R = O.chol_region(T, "U", info)
+    for t in R.tasks:  # the planted mistake: plain transposes of complex tiles
+        if t.name in ("trsm", "gemm"):
+            t.args = tuple("T" if isinstance(a, str) and a == "C" else a for a in t.args)
+    for t in R.tasks:
+        try:
+            t.run()
+        except O.PosDefException:
+            break
+    L = np.linalg.cholesky(H)
+    assert np.abs(np.triu(T.collect()) - L.conj().T).max() > 1e-3
