@@ -19,6 +19,7 @@ SCALARS = {"Cint": "int", "Cchar": "char", "Cdouble": "double", "Cfloat": "float
            "Csize_t": "size_t", "UInt64": "uint64_t", "Int64": "int64_t"}
 POINTEES = {"Cint": "int", "Float64": "double", "F64": "double", "Float32": "float",
             "F32": "float", "Cdouble": "double", "Cfloat": "float", "DMTiled": "dm_tiled",
+            "ComplexF64": "double", "ComplexF32": "float",  # {re, im} scalars by reference
             "DMPlanOpts": "dm_plan_opts", "DMTask": "dm_task"}
 
 
@@ -106,8 +107,8 @@ def test_glue_binds_declared_entry_points_with_matching_types():
     assert len(calls) >= 25
     seen = set()
     for name, ret, args, ctx in calls:
-        if name is None:             # the @eval potrf! loop: both names are checked below
-            for fn in ("dm_dpotrf", "dm_spotrf"):
+        if name is None:             # the @eval potrf! loops: every name is checked here
+            for fn in ("dm_dpotrf", "dm_spotrf", "dm_zpotrf", "dm_cpotrf"):
                 assert fn in decls
                 assert len(args) == len(decls[fn][1])
             continue
@@ -125,6 +126,7 @@ def test_glue_binds_declared_entry_points_with_matching_types():
     for must in ("dm_memcpy_h2d", "dm_memcpy_d2h", "dm_memcpy_d2d", "dm_memcpy2d_h2d",
                  "dm_memcpy2d_d2h", "dm_malloc", "dm_free", "dm_fill_i32", "dm_fill_randn_d",
                  "dm_dgemm", "dm_dsyrk", "dm_dtrsm", "dm_dtrsv", "dm_dgemv", "dm_sgemm",
+                 "dm_zgemm", "dm_cgemm", "dm_zherk", "dm_cherk", "dm_ztrsm", "dm_ctrsm",
                  "dm_stream_sync", "dm_device_sync", "dm_init", "dm_stream_create",
                  "dm_cholesky_tiled"):
         assert must in seen, must
