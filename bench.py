@@ -250,12 +250,15 @@ def mixed_leg(dm, n, nb, kappa, nrhs, seed):
 
 def batched_leg(dm, count, seed, reps=3):
     """Batching form: `count` uneven SPD systems A = G G^T/n + I with
-    n ~ LogUniform[32, 2048] and nrhs ~ U{1..8}, factored and solved by one
-    persistent launch (dm_batched_posv), FP64 and FP32, panel width per size
-    bucket from the autotuner.  Timed with HIP events, systems restored from
-    device copies between runs; reports systems/s and (sum n^3/3 + sum
-    2 n^2 nrhs) / time, plus the largest backward error of 8 sampled systems
-    (numpy on the host copies)."""
+    n ~ LogUniform[32, 2048] and nrhs ~ U{1..8}, FP64 and FP32, factored and
+    solved by the tiled level-synchronous plan (BatchedSystems.solve,
+    dm_batched_tiled_*; plan built once, outside the timed runs, like the
+    executor's cached plans), and for comparison by one persistent launch
+    (dm_batched_posv, panel width per size bucket from the autotuner).
+    Timed with HIP events, systems restored from device copies between
+    runs; reports systems/s and (sum n^3/3 + sum 2 n^2 nrhs) / time, plus
+    the largest backward error of 8 sampled systems (numpy on the host
+    copies)."""
     from dagger_mi355x import batched
     call = dm._lib.call
     timer = Timer(call)
@@ -267,15 +270,21 @@ def batched_leg(dm, count, seed, reps=3):
         originals = {m: bs.get_system(m) for m in picks}
         panels, _ = batched.autotune(bs, sample=16, reps=1)
         snap = bs.snapshot()
-        times = []
-        for rep in range(reps + 1):
-            bs.restore(snap)
-            call("dm_stream_sync", None)
-            timer.start()
-            bs.posv(panels=panels)
-            ms = timer.stop_ms()
-            if rep:
-                times.append(ms)
+
+        def timed(run):
+            times = []
+            for rep in range(reps + 1):
+                bs.restore(snap)
+                call("dm_stream_sync", None)
+                timer.start()
+                run()
+                ms = timer.stop_ms()
+                if rep:
+                    times.append(ms)
+            return sum(times) / len(times)
+
+        ms_persistent = timed(lambda: bs.posv(panels=panels))
+        ms = timed(lambda: bs.solve(panels=panels))
         bad = int((bs.infos() != 0).sum())
         berr = 0.0
         for m in picks:
@@ -287,10 +296,14 @@ def batched_leg(dm, count, seed, reps=3):
                 den = np.abs(A64).sum(1).max() * np.abs(X64).max(0) + np.abs(B64).max(0)
                 berr = max(berr, float((np.abs(r).max(0) / den).max()))
         ff, fs = bs.flops()
-        ms = sum(times) / len(times)
         out[name] = {"ms": round(ms, 3), "systems_per_s": round(count / (ms * 1e-3), 1),
                      "tflops": round((ff + fs) / (ms * 1e-3) / 1e12, 3), "failed": bad,
-                     "max_sampled_berr": berr, "panels": {str(k): v for k, v in panels.items()}}
+                     "max_sampled_berr": berr, "path": "tiled level-synchronous",
+                     "tile_nb": batched.TILED_NB,
+                     "persistent_ms": round(ms_persistent, 3),
+                     "persistent_tflops": round((ff + fs) / (ms_persistent * 1e-3) / 1e12, 3),
+                     "panels": {str(k): v for k, v in panels.items()}}
+        bs.close()
         for buf in (*snap, bs.A, bs.B):
             buf.free()
     return out

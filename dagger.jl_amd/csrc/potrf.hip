@@ -40,65 +40,18 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(T* __restrict__ A, int n
 
 // Panel row beside a factored full PW x PW block: U12 := U11^-T A12 (U) or
 // A21 := A21 L11^-T (L).  One wave per 64 columns of A12 (rows of A21), one
-// column (row) per lane; the factored block sits in LDS as its upper view
-// with row stride PW, so reading row p past column PW-1 lands in the
-// (zero) strictly lower part of row p+1, followed by PW zeros.
+// column (row) per lane (potrf_row_stage / potrf_row_solve, potrf_diag.h).
 template <typename T, bool UPPER, int PW>
 __global__ __launch_bounds__(64) void potrf_row_kernel(T* __restrict__ A, int n, int lda, int j0,
                                                        const int* __restrict__ info,
                                                        const int* __restrict__ abort) {
-  static_assert(PW == 32 || PW == 64, "panel width");
   __shared__ T Us[PW * PW + PW];
   __shared__ T invd[PW];
   if (abort && *abort) return;
   if (info && *info) return;
   raise_wave_priority();
-  const int t = threadIdx.x;
-  if (t < PW) {
-    // lane = column c; U: element (r, c) = A[j0+r, j0+c]; L: A[j0+c, j0+r]
-    for (int r = 0; r < PW; ++r) {
-      T v = T(0);
-      if (r <= t) v = UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t];
-      Us[r * PW + t] = v;
-    }
-    Us[PW * PW + t] = T(0);
-  }
-  __syncthreads();
-  if (t < PW) invd[t] = T(1) / Us[t * PW + t];
-  __syncthreads();
-  const int c_begin = j0 + PW, rest = n - c_begin;
-  const int c0 = blockIdx.x * 64;
-  if (c0 + t >= rest) return;  // no barrier follows
-  const int cc = c_begin + c0 + t;
-  const size_t step = UPPER ? 1 : (size_t)lda;
-  T* const base = UPPER ? A + (size_t)cc * lda + j0 : A + (size_t)j0 * lda + cc;
-  T x[PW];
-  const T* px = base;
-#pragma unroll
-  for (int r = 0; r < PW; ++r) {
-    x[r] = *px;
-    px += step;
-    if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
-  }
-  // U11^T x = b, forward over the rows of the block; x[i] holds row p + i
-  auto phase = [&](auto len, int ps) {
-    constexpr int L = decltype(len)::value;
-#pragma unroll 1
-    for (int p = ps; p < ps + 16; ++p) {
-      const T xp = x[0] * invd[p];
-      base[p * step] = xp;
-      const T* u = Us + p * PW + p + 1;
-#pragma unroll
-      for (int i = 0; i < L - 1; ++i) x[i] = x[i + 1] - u[i] * xp;
-      x[L - 1] = T(0);
-    }
-  };
-  phase(std::integral_constant<int, PW>{}, 0);
-  phase(std::integral_constant<int, PW - 16>{}, 16);
-  if constexpr (PW == 64) {
-    phase(std::integral_constant<int, 32>{}, 32);
-    phase(std::integral_constant<int, 16>{}, 48);
-  }
+  potrf_row_stage<T, UPPER, PW>(A, lda, j0, Us, invd);
+  potrf_row_solve<T, UPPER, PW>(A, n, lda, j0, blockIdx.x * 64, Us, invd);
 }
 
 namespace {

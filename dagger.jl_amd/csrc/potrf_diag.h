@@ -1,7 +1,9 @@
-// The diagonal-block step of the tile POTRF (one wave), shared by
-// potrf_diag_kernel (potrf.hip) and the fused trailing-update kernel that
-// factors the next diagonal block (gemm.hip).
+// The diagonal-block and panel-row steps of the tile POTRF (one wave each),
+// shared by potrf_diag_kernel / potrf_row_kernel (potrf.hip), the fused
+// trailing-update kernel that factors the next diagonal block (gemm.hip) and
+// the batched level-synchronous factorization (batched.hip).
 #pragma once
+#include <type_traits>
 #include "dm_internal.h"
 
 namespace dm {
@@ -143,6 +145,93 @@ __device__ __forceinline__ void potrf_diag_block(T* __restrict__ A, int n, int l
     if (info) *info = j0 + fail;
     if (abort) *abort = global_base + j0 + fail;
   }
+}
+
+// Panel row beside a factored full PW x PW block at (j0, j0): U12 :=
+// U11^-T A12 (U) or A21 := A21 L11^-T (L), one column of A12 (row of A21)
+// per lane.  potrf_row_stage (every thread of a one-wave workgroup) puts
+// the block in LDS as its upper view with row stride PW -- so reading row p
+// past column PW-1 lands in the (zero) strictly lower part of row p+1,
+// followed by PW zeros -- and the reciprocals of its diagonal in invd.
+// Us: PW * PW + PW elements, invd: PW.
+template <typename T, bool UPPER, int PW>
+__device__ __forceinline__ void potrf_row_stage(const T* __restrict__ A, int lda, int j0,
+                                                T* Us, T* invd) {
+  static_assert(PW == 32 || PW == 64, "panel width");
+  const int t = threadIdx.x;
+  if (t < PW) {
+    // lane = column c; U: element (r, c) = A[j0+r, j0+c]; L: A[j0+c, j0+r]
+    for (int r = 0; r < PW; ++r) {
+      T v = T(0);
+      if (r <= t) v = UPPER ? A[(size_t)(j0 + t) * lda + j0 + r] : A[(size_t)(j0 + r) * lda + j0 + t];
+      Us[r * PW + t] = v;
+    }
+    Us[PW * PW + t] = T(0);
+  }
+  __syncthreads();
+  if (t < PW) invd[t] = T(1) / Us[t * PW + t];
+  __syncthreads();
+}
+
+// U11^T x = b for one column per lane, forward over the rows of the staged
+// block; x[i] holds row p + i at step p, and row p's result goes to
+// base[p * step].
+template <typename T, int PW>
+__device__ __forceinline__ void potrf_row_phases(T (&x)[PW], T* base, size_t step, const T* Us,
+                                                 const T* invd) {
+  auto phase = [&](auto len, int ps) {
+    constexpr int L = decltype(len)::value;
+#pragma unroll 1
+    for (int p = ps; p < ps + 16; ++p) {
+      const T xp = x[0] * invd[p];
+      base[p * step] = xp;
+      const T* u = Us + p * PW + p + 1;
+#pragma unroll
+      for (int i = 0; i < L - 1; ++i) x[i] = x[i + 1] - u[i] * xp;
+      x[L - 1] = T(0);
+    }
+  };
+  phase(std::integral_constant<int, PW>{}, 0);
+  phase(std::integral_constant<int, PW - 16>{}, 16);
+  if constexpr (PW == 64) {
+    phase(std::integral_constant<int, 32>{}, 32);
+    phase(std::integral_constant<int, 16>{}, 48);
+  }
+}
+
+// Columns c0 + lane of A12 (rows of A21), counted from j0 + PW, after
+// potrf_row_stage.  Lanes past the panel return (no barrier follows).
+template <typename T, bool UPPER, int PW>
+__device__ __forceinline__ void potrf_row_solve(T* __restrict__ A, int n, int lda, int j0, int c0,
+                                                const T* Us, const T* invd) {
+  const int t = threadIdx.x;
+  const int c_begin = j0 + PW, rest = n - c_begin;
+  if (c0 + t >= rest) return;
+  const int cc = c_begin + c0 + t;
+  const size_t step = UPPER ? 1 : (size_t)lda;
+  T* const base = UPPER ? A + (size_t)cc * lda + j0 : A + (size_t)j0 * lda + cc;
+  T x[PW];
+  const T* px = base;
+#pragma unroll
+  for (int r = 0; r < PW; ++r) {
+    x[r] = *px;
+    px += step;
+    if ((r & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // bound the loads in flight
+  }
+  potrf_row_phases<T, PW>(x, base, step, Us, invd);
+}
+
+// W = U11^-T (PW x PW, column-major, leading dimension ldw) of the block
+// staged by potrf_row_stage (upper view): lane c solves U11^T w = e_c.
+template <typename T, int PW>
+__device__ __forceinline__ void potrf_row_inverse(T* __restrict__ W, int ldw, const T* Us,
+                                                  const T* invd) {
+  const int t = threadIdx.x;
+  if (t >= PW) return;
+  T x[PW];
+#pragma unroll
+  for (int r = 0; r < PW; ++r) x[r] = (r == t) ? T(1) : T(0);
+  potrf_row_phases<T, PW>(x, W + (size_t)t * ldw, 1, Us, invd);
 }
 
 }  // namespace dm

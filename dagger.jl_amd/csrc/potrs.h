@@ -35,11 +35,17 @@ struct TrtriPlan {
   };
   std::vector<Launch> launches;  // two per level, in order
 
+  std::vector<int> rowsv;  // rows of tile k
+
   T* inv(int k) const { return (T*)mem + (size_t)k * nb * nb; }
-  int rows(int k) const { return (int)std::min<int64_t>(nb, m - (int64_t)k * nb); }
+  int rows(int k) const { return rowsv[k]; }
   ~TrtriPlan();
   // tile(k): origin of the k-th diagonal tile of the factor (ld = rows(k))
   int build(bool up, int64_t m_, int nb_, const std::function<const T*(int)>& tile);
+  // count tiles of at most nb_ rows: tile k at tile(k), rows(k) x rows(k)
+  // with leading dimension ld(k) (the inverse tiles keep ld = rows(k))
+  int build_list(bool up, int count, int nb_, const std::function<const T*(int)>& tile,
+                 const std::function<int(int)>& rows_of, const std::function<int(int)>& ld_of);
   // Inverse tiles on stream s (a no-op once *abort is set).
   int run(hipStream_t s, const int* abort) const;
 };

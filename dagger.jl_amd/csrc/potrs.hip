@@ -77,20 +77,38 @@ TrtriPlan<T>::~TrtriPlan() {
 
 template <typename T>
 int TrtriPlan<T>::build(bool up, int64_t m_, int nb_, const std::function<const T*(int)>& tile) {
-  upper = up;
+  const int count = (int)((m_ + nb_ - 1) / nb_);
+  auto rws = [m_, nb_](int k) { return (int)std::min<int64_t>(nb_, m_ - (int64_t)k * nb_); };
+  int rc = build_list(up, count, nb_, tile, rws, rws);
   m = m_;
+  return rc;
+}
+
+template <typename T>
+int TrtriPlan<T>::build_list(bool up, int count, int nb_, const std::function<const T*(int)>& tile,
+                             const std::function<int(int)>& rows_of,
+                             const std::function<int(int)>& ld_of) {
+  upper = up;
   nb = nb_;
-  mt = (int)((m + nb - 1) / nb);
+  mt = count;
+  m = 0;
+  rowsv.assign(mt, 0);
+  for (int k = 0; k < mt; ++k) {
+    rowsv[k] = rows_of(k);
+    if (rowsv[k] < 0 || rowsv[k] > nb || ld_of(k) < rowsv[k])
+      return arg_fail(0, "trtri: tile rows outside [0, nb] or ld < rows");
+    m += rowsv[k];
+  }
   inv_bytes = align_up<T>(sizeof(T) * (size_t)mt * nb * nb, 256);
   std::vector<TrtriLeaf> lv;
   std::vector<size_t> leaf_off;  // element offset of each leaf in the inverse tiles
   for (int k = 0; k < mt; ++k) {
-    const int r = rows(k);
+    const int r = rows(k), lt = ld_of(k);
     for (int l0 = 0; l0 < r; l0 += 64) {
       TrtriLeaf L{};
-      L.src = tile(k) + (size_t)l0 * r + l0;
+      L.src = tile(k) + (size_t)l0 * lt + l0;
       L.dst = nullptr;  // patched below (offset into the inverse tiles)
-      L.lds = r;
+      L.lds = lt;
       L.ldd = r;
       L.n = std::min(64, r - l0);
       leaf_off.push_back((size_t)k * nb * nb + (size_t)l0 * r + l0);
@@ -127,9 +145,10 @@ int TrtriPlan<T>::build(bool up, int64_t m_, int nb_, const std::function<const 
       const size_t d11 = base + (size_t)p.p0 * r + p.p0;
       const size_t d22 = base + (size_t)(p.p0 + p.s) * r + p.p0 + p.s;
       GemmJob<T> a{}, b{};
+      const int lt = ld_of(p.k);
       if (upper) {
         // S = U12 X22 (s x s2); X12 = -X11 S
-        a.A = tile(p.k) + (size_t)(p.p0 + p.s) * r + p.p0; a.lda = r;
+        a.A = tile(p.k) + (size_t)(p.p0 + p.s) * lt + p.p0; a.lda = lt;
         a.B = (const T*)(uintptr_t)(d22 + 1); a.ldb = r;   // inverse-tile offsets + 1
         a.M = p.s; a.N = p.s2; a.K = p.s2;
         b.A = (const T*)(uintptr_t)(d11 + 1); b.lda = r;
@@ -137,7 +156,7 @@ int TrtriPlan<T>::build(bool up, int64_t m_, int nb_, const std::function<const 
         b.C = (T*)(uintptr_t)(base + (size_t)(p.p0 + p.s) * r + p.p0 + 1); b.ldc = r;
       } else {
         // S = L21 X11 (s2 x s); X21 = -X22 S
-        a.A = tile(p.k) + (size_t)p.p0 * r + p.p0 + p.s; a.lda = r;
+        a.A = tile(p.k) + (size_t)p.p0 * lt + p.p0 + p.s; a.lda = lt;
         a.B = (const T*)(uintptr_t)(d11 + 1); a.ldb = r;
         a.M = p.s2; a.N = p.s; a.K = p.s;
         b.A = (const T*)(uintptr_t)(d22 + 1); b.lda = r;

@@ -446,11 +446,35 @@ int dm_forget_matrix(const void* base);
  * workgroups: persistent grid (0 = one 512-thread workgroup per CU). */
 typedef struct dm_batch_item {
     int64_t a_off, b_off;
-    int n, nrhs, panel, pad_;
+    int n, nrhs, panel;
+    int lda;       /* A_m's leading dimension, >= n (0: n, the packed layout) */
 } dm_batch_item;
 int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_base,
                     const void* items_host, int count, const int* order_host, int* info_dev,
                     int factor_only, int workgroups, dm_stream_t s);
+
+/* The same factorization and solve for the larger systems of a batch
+ * (no reference counterpart), tiled and level-synchronous: every listed
+ * system advances through one blocked right-looking Cholesky in lockstep,
+ * each step one launch over all systems still active at it -- per 64-column
+ * block the diagonal block (one wave per system), its panel row over the
+ * system's full width and the update of the rows left in the current
+ * outer tile of nb columns; after each outer tile the trailing update with
+ * K = nb as one batched MFMA GEMM -- then both triangular sweeps by 64-row
+ * blocks (batched in-register solves of the diagonal blocks and batched
+ * GEMM updates) on a working copy of the right-hand sides.  Items and order
+ * as for dm_batched_posv (each item's `panel` is ignored); nb a multiple of
+ * 64 in [64, 1024].  The plan holds its job tables and an n x nrhs working
+ * slab per system; run it any number of times (on any stream, one run at a
+ * time), each run factoring A in place and, unless factor_only, overwriting
+ * B_m with the solution of every system whose info is 0 (a failed system's
+ * B_m is left as it was).  info_dev: as for dm_batched_posv. */
+typedef struct dm_batch_plan_s* dm_batch_plan_t;
+int dm_batched_tiled_create(dm_batch_plan_t* plan, int dtype, int num_items, void* A_base,
+                            void* B_base, const void* items_host, int count,
+                            const int* order_host, int nb);
+int dm_batched_tiled_run(dm_batch_plan_t plan, int* info_dev, int factor_only, dm_stream_t s);
+int dm_batched_tiled_destroy(dm_batch_plan_t plan);
 
 /* ---------------------------------------------------------- generators
  * Seeded counter-based normal variates: Philox4x32-10 keyed by `seed`,

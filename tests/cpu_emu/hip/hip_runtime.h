@@ -129,6 +129,14 @@ inline T shfl_down(T v, unsigned d, int width) {
   const unsigned base = lane - lane % w;
   return (lane % w) + d < w ? all[base + (lane % w) + d] : v;
 }
+template <typename T>
+inline T shfl(T v, int src, int width) {
+  T all[64];
+  wave_exchange(&v, sizeof(T), all);
+  const unsigned lane = thread_idx().x & 63;
+  const unsigned w = (unsigned)width;
+  return all[lane - lane % w + (unsigned)src % w];
+}
 inline int readlane(int v, int l) {
   int all[64];
   wave_exchange(&v, sizeof(int), all);
@@ -158,6 +166,7 @@ inline int readlane(int v, int l) {
 #define __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, x, y, z) \
   ::emu::mfma_f32_16x16x32_bf16((a), (b), (c))
 #define __shfl_down(v, d, w) ::emu::shfl_down((v), (d), (w))
+#define __shfl(v, l, w) ::emu::shfl((v), (l), (w))
 #define __builtin_amdgcn_wave_barrier() ::emu::wave_exchange(nullptr, 0, nullptr)
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_sched_barrier(mask) ((void)0)

@@ -359,6 +359,62 @@ def batched():
         pass
 
 
+def batched_tiled():
+    """Tiled level-synchronous path of the batching form (outer tiles of
+    128 columns over 64-column blocks, so the larger systems take several of
+    each, with ragged last blocks), mixed with
+    the persistent kernel by BatchedSystems.solve; a non-positive pivot in a
+    later tile of one system gives its LAPACK info and leaves its B as it
+    was, while the other systems are solved."""
+    from dagger_mi355x import batched as Bt
+    for dtype in (np.float64, np.float32):
+        sizes = np.array([200, 64, 130, 40, 65, 129, 96])
+        nrhs = np.array([2, 1, 0, 3, 1, 8, 2])
+        bs = Bt.BatchedSystems(sizes, nrhs, dtype).generate(seed=11)
+        if dtype == np.float64:  # system 5: pivot 100 (second tile) not positive
+            A5, _ = bs.get_system(5)
+            A5[99, 99] = -1.0
+            bs.put_system(5, A=A5)
+        A0s = [bs.get_system(m) for m in range(bs.count)]
+        bs.solve(panels={}, tiled_min=64, nb=128)
+        infos = bs.infos()
+        tol = 10 * np.finfo(dtype).eps
+        for m in range(bs.count):
+            A0, B0 = A0s[m]
+            F, X = bs.get_system(m)
+            ref = np.array(A0, dtype=np.float64, order="F")
+            want = O.tile_potrf("U", ref)
+            check(infos[m] == want, ("batched tiled info", dtype, m, infos[m], want))
+            if want:
+                check(np.array_equal(X, B0), ("failed system's B changed", dtype, m))
+                continue
+            rtol = 1e-12 if dtype == np.float64 else 5e-5
+            check(O.rel_factor_diff(F.astype(np.float64), ref, "U") <= rtol,
+                  ("batched tiled factor", dtype, m))
+            if B0.shape[1]:
+                be = O.backward_error(A0.astype(np.float64), X.astype(np.float64),
+                                      B0.astype(np.float64))
+                check(be <= tol * A0.shape[0], ("batched tiled solve", dtype, m, be))
+        # a second run of the cached plan on restored data gives the same bits
+        if dtype == np.float64:
+            F1 = [bs.get_system(m) for m in range(bs.count)]
+            for m in range(bs.count):
+                bs.put_system(m, A=A0s[m][0], B=A0s[m][1])
+            bs.solve(panels={}, tiled_min=64, nb=128)
+            for m in range(bs.count):
+                F2, X2 = bs.get_system(m)
+                check(np.array_equal(F2, F1[m][0]) and np.array_equal(X2, F1[m][1]),
+                      ("batched tiled rerun", m))
+        bs.close()
+    for bad in (32, 96 + 1, 2048):  # nb not a multiple of 64 in [64, 1024]
+        try:
+            bs = Bt.BatchedSystems([70], [1]).generate(seed=1)
+            Bt.TiledPlan(bs, [0], nb=bad)
+            raise AssertionError(("bad tiled plan accepted", bad))
+        except dm.DMError:
+            pass
+
+
 # ---------------------------------------------------------------- serving
 def serving():
     from dagger_mi355x import serving as S
@@ -640,7 +696,7 @@ def complex_ops():
             check(relerr(xv, X0[:, 0]) <= 1e-10, ("complex vector ldiv", uplo, via))
 
 
-GROUPS = dict(quick=quick, complex_ops=complex_ops, tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
+GROUPS = dict(quick=quick, complex_ops=complex_ops, batched_tiled=batched_tiled, tiles=tiles, tiled=tiled, mixed=mixed, batched=batched, serving=serving,
               capped=capped)
 
 if __name__ == "__main__":

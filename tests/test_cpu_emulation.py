@@ -5,8 +5,8 @@ driven through the C ABI by tests/cpu_emu/scenarios.py.  An out-of-bounds
 access aborts with an ASan report; a barrier that not every work-item
 reaches aborts as a deadlock; results are checked against the oracle.
 
-The default CPU run executes the `quick` and `complex_ops` groups (about two
-minutes including the build).  DM_EMU_GROUPS="tiles tiled mixed batched
+The default CPU run executes the `quick`, `complex_ops` and `batched_tiled`
+groups (a few minutes including the build).  DM_EMU_GROUPS="tiles tiled mixed batched
 serving capped" selects the long groups (tens of minutes)."""
 import os
 import subprocess
@@ -34,7 +34,7 @@ def test_kernels_under_cpu_emulation():
     env["DM_TEST_EMU_LIB"] = lib
     env["LD_PRELOAD"] = build_emu.asan_runtime()
     env["ASAN_OPTIONS"] = "detect_leaks=0"
-    groups = os.environ.get("DM_EMU_GROUPS", "quick complex_ops").split()
+    groups = os.environ.get("DM_EMU_GROUPS", "quick complex_ops batched_tiled").split()
     r = subprocess.run([sys.executable, os.path.join(EMU, "scenarios.py"), *groups], env=env,
                        capture_output=True, text=True, timeout=3000)
     out = r.stdout + r.stderr

@@ -57,15 +57,13 @@ def test_batched_panels_agree_and_autotune(dm):
 
 
 def test_batched_non_pd_system_is_isolated(dm):
-    import ctypes as C
     from dagger_mi355x import batched
     sizes = np.array([100, 200, 150])
     nrhs = np.array([2, 2, 2])
     bs = batched.BatchedSystems(sizes, nrhs).generate(seed=9)
     A1, _ = bs.get_system(1)
     A1[120, 120] = -1.0
-    dm._lib.call("dm_memcpy_h2d", C.c_void_p(bs.a_ptr(1)), A1.ctypes.data_as(C.c_void_p),
-                 C.c_size_t(A1.nbytes), None)
+    bs.put_system(1, A=A1)
     A0s, B0s = zip(*[bs.get_system(m) for m in range(bs.count)])
     bs.posv(panels={})
     infos = bs.infos()
@@ -75,3 +73,44 @@ def test_batched_non_pd_system_is_isolated(dm):
     for m in (0, 2):
         F, X = bs.get_system(m)
         assert O.backward_error(A0s[m], X, B0s[m]) <= 10 * sizes[m] * np.finfo(float).eps
+
+
+@pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-12), (np.float32, 5e-5)])
+@pytest.mark.parametrize("nb", [64, 128, 256])
+def test_batched_tiled_hybrid(dm, dtype, rtol, nb):
+    """Systems of at least 300 rows through the tiled level-synchronous
+    plan (ragged last tiles at every nb), the rest through the persistent
+    kernel; each factor against LAPACK and each solve's backward error."""
+    from dagger_mi355x import batched
+    sizes, nrhs = batched.log_uniform_sizes(40, 32, 1100, seed=31)
+    sizes[:4] = [1100, 300, nb + 1, 2 * nb]
+    bs = batched.BatchedSystems(sizes, nrhs, dtype).generate(seed=32)
+    A0s, B0s = zip(*[bs.get_system(m) for m in range(bs.count)])
+    bs.solve(panels={}, tiled_min=300, nb=nb)
+    _check(bs, A0s, B0s, rtol)
+    bs.close()
+
+
+def test_batched_tiled_failed_system_keeps_b(dm):
+    """A non-positive pivot in a later tile of one tiled system: LAPACK's
+    info for it, its B left as it was, every other system solved."""
+    import ctypes as C
+    from dagger_mi355x import batched
+    sizes = np.array([700, 520, 900, 64])
+    nrhs = np.array([3, 2, 8, 1])
+    bs = batched.BatchedSystems(sizes, nrhs).generate(seed=33)
+    A1, _ = bs.get_system(1)
+    A1[400, 400] = -2.0
+    bs.put_system(1, A=A1)
+    A0s, B0s = zip(*[bs.get_system(m) for m in range(bs.count)])
+    bs.solve(panels={}, tiled_min=500, nb=128)
+    infos = bs.infos()
+    ref = np.array(A1, order="F")
+    assert infos[1] == O.tile_potrf("U", ref) == 401
+    _, X1 = bs.get_system(1)
+    assert np.array_equal(X1, B0s[1])
+    for m in (0, 2, 3):
+        assert infos[m] == 0
+        _, X = bs.get_system(m)
+        assert O.backward_error(A0s[m], X, B0s[m]) <= 10 * sizes[m] * np.finfo(float).eps
+    bs.close()

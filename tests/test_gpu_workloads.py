@@ -9,9 +9,11 @@ checked against closed forms and per-system LAPACK:
   (||A||_inf ||x||_inf + ||b||_inf) is computed on the host from the
   generator's algebra (A x = s x - U W^T x), never from the device;
 * batched: 4096 systems G G^T / n + I with n log-uniform in [32, 2048] and
-  1-8 right-hand sides, FP64 and FP32, one persistent launch; every
-  system's backward error <= 10 n eps, and the factor of one system per
-  floor(log2 n) bucket against LAPACK potrf;
+  1-8 right-hand sides, FP64 and FP32, systems of n >= 512 through the
+  tiled level-synchronous plan and the rest in one persistent launch (and
+  FP64 all in the persistent launch); every system's backward error
+  <= 10 n eps, and the factor of one system per floor(log2 n) bucket
+  against LAPACK potrf;
 * serving: a Poisson trace of 64 requests, n log-uniform in [64, 8192],
   1-32 right-hand sides, tolerances 1e-8 and 1e-13, 30% reuse, 200 GB
   budget, requests on both the FP32-factor and the FP64 paths; every
@@ -101,7 +103,7 @@ def test_mixed_kappa1e8_reports_method_and_meets_tolerance(dm):
     assert info["shift"] in (0.0, 4.0, 32.0, 256.0)
 
 
-def _batched_check(dm, dtype, count=4096, seed=2024):
+def _batched_check(dm, dtype, count=4096, seed=2024, mode="hybrid"):
     from dagger_mi355x import batched
     sizes, nrhs = batched.log_uniform_sizes(count, 32, 2048, seed)
     bs = batched.BatchedSystems(sizes, nrhs, dtype).generate(seed=seed + 1)
@@ -113,7 +115,11 @@ def _batched_check(dm, dtype, count=4096, seed=2024):
     dm._lib.call("dm_memcpy_d2h", B0.ctypes.data_as(C.c_void_p), C.c_void_p(bs.B.ptr),
                  C.c_size_t(bs.b_elems * es), None)
     panels, _ = batched.autotune(bs, sample=8, reps=1)
-    bs.posv(panels=panels)
+    if mode == "hybrid":
+        bs.solve(panels=panels)
+        bs.close()
+    else:
+        bs.posv(panels=panels)
     infos = bs.infos()
     assert (infos == 0).all(), np.nonzero(infos)[0][:10]
     X = np.empty(bs.b_elems, dtype=dtype)
@@ -123,8 +129,8 @@ def _batched_check(dm, dtype, count=4096, seed=2024):
     worst = 0.0
     for m in range(count):
         n, r = int(sizes[m]), int(nrhs[m])
-        a0, b0 = int(bs.a_off[m]), int(bs.b_off[m])
-        A = A0[a0:a0 + n * n].reshape(n, n, order="F").astype(np.float64)
+        b0 = int(bs.b_off[m])
+        A = bs.host_matrix(A0, m).astype(np.float64)
         Bm = B0[b0:b0 + n * r].reshape(n, r, order="F").astype(np.float64)
         Xm = X[b0:b0 + n * r].reshape(n, r, order="F").astype(np.float64)
         be = O.backward_error(A, Xm, Bm)
@@ -139,8 +145,7 @@ def _batched_check(dm, dtype, count=4096, seed=2024):
             continue
         seen.add(bkt)
         n = int(sizes[m])
-        a0 = int(bs.a_off[m])
-        ref = np.array(A0[a0:a0 + n * n].reshape(n, n, order="F"), dtype=np.float64, order="F")
+        ref = np.array(bs.host_matrix(A0, m), dtype=np.float64, order="F")
         assert O.tile_potrf("U", ref) == 0
         F, _ = bs.get_system(m)
         assert O.rel_factor_diff(F.astype(np.float64), ref, "U") <= rtol, (m, n)
@@ -150,9 +155,12 @@ def _batched_check(dm, dtype, count=4096, seed=2024):
     return worst
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_batched_4096_uneven_systems(dm, dtype):
-    worst = _batched_check(dm, dtype)
+@pytest.mark.parametrize("dtype,mode", [(np.float64, "hybrid"), (np.float32, "hybrid"),
+                                        (np.float64, "persistent")])
+def test_batched_4096_uneven_systems(dm, dtype, mode):
+    """hybrid: n >= 512 through the tiled level-synchronous plan, the rest
+    in one persistent launch; persistent: every system in that launch."""
+    worst = _batched_check(dm, dtype, mode=mode)
     assert worst <= 10
 
 
