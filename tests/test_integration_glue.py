@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "dagger_mi355x.h")
 DOC = os.path.join(ROOT, "INTEGRATION.md")
 
 # pointer typedefs of the header (opaque handles)
-HANDLES = {"dm_stream_t", "dm_event_t", "dm_graph_t", "dm_dag_t"}
+HANDLES = {"dm_stream_t", "dm_event_t", "dm_graph_t", "dm_dag_t", "dm_batch_plan_t"}
 SCALARS = {"Cint": "int", "Cchar": "char", "Cdouble": "double", "Cfloat": "float",
            "Csize_t": "size_t", "UInt64": "uint64_t", "Int64": "int64_t"}
 POINTEES = {"Cint": "int", "Float64": "double", "F64": "double", "Float32": "float",
