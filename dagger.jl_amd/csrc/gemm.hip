@@ -527,11 +527,11 @@ __global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __
 namespace {
 constexpr int GEMM_CFG_256x128 = 4;
 constexpr int GEMM_CFG_SKINNY = 5;  // 32 rows x bn (16..64) columns, K split over 4 waves
+constexpr int GEMM_CFG_64x64_K32 = 6;  // 64 x 64 tiles, 32-deep k-blocks
 }
 
 template <typename T>
 GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs, bool vec) {
-  (void)K;
   const long nj = njobs > 0 ? njobs : 1;
   // A launch of at most 64 columns too small to fill the device (fewer than
   // 64 tiles of 64 x 64, e.g. the critical-path products of the ldiv! sweeps)
@@ -544,8 +544,11 @@ GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs, bool vec) {
   // measured slightly faster in large launches)
   if (N <= 48) return GemmCfgId{GEMM_CFG_SKINNY, 32, 16 * ((N + 15) / 16)};
   // at most 64 columns: the 64 x 64 tile (a 128-wide tile would compute
-  // half its columns for nothing)
-  if (N <= 64) return GemmCfgId{1, 64, 64};
+  // half its columns for nothing).  With a long K a workgroup's k-loop is a
+  // chain of dependent memory latencies, so it moves 32-deep k-blocks: the
+  // ldiv! sweeps' bulk updates (K = 1024, 64 columns) took 127 us instead of
+  // about 140 us per launch (64-deep blocks were slower again).
+  if (N <= 64) return GemmCfgId{K >= 512 ? GEMM_CFG_64x64_K32 : 1, 64, 64};
   if (sizeof(T) == 8 && vec) {
     // Largest tile whose launch still puts about one workgroup on every CU
     // (scalar-load launches keep the 128 x 128 tile: the 256 x 128 one does
@@ -601,6 +604,8 @@ int launch_cfg(int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& sing
   }
   if (id == 1)
     return launch_one<T, 64, 64, 16, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
+  if (id == GEMM_CFG_64x64_K32)
+    return launch_one<T, 64, 64, 32, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
   return arg_fail(0, "gemm: unknown configuration");
 }
 template <typename T, bool TA, bool TB>
