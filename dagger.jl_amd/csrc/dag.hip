@@ -496,6 +496,15 @@ int Dag::build_device_tables() {
       ws_bytes += per;
     }
   }
+  // FP32 POTRF tasks factored in FP64 (opts.f64_diag): an n x n FP64 tile
+  // each, so concurrent POTRFs of independent matrices never share one
+  if (opts.f64_diag)
+    for (size_t i = 0; i < tasks.size(); ++i) {
+      const dm_task& t = tasks[i].t;
+      if (t.op != DM_OP_POTRF || t.dtype != DM_F32) continue;
+      task_ws_off[i] = 256 + inv_bytes + ws_bytes;
+      ws_bytes += ((size_t)t.n * t.n * sizeof(double) + 255) / 256 * 256;
+    }
   // device memory: [abort int (256 B)] [inverse blocks] [workspaces] [job tables]
   const size_t tab_base = 256 + inv_bytes + ws_bytes;
   const size_t total = tab_base + blob.size();
@@ -613,6 +622,13 @@ int Dag::launch_group_t(const Group& G, hipStream_t s) {
     switch (t.op) {
       case DM_OP_POTRF: {
         const int gofs = t.k;  // global row offset of the tile (set by the builders; 0 otherwise)
+        if constexpr (std::is_same<T, float>::value) {
+          if (opts.f64_diag) {  // ws: the task's FP64 tile
+            rc = potrf_f64_of_f32(is_upper(t.uplo), t.n, (float*)t.c, t.ldc, (double*)ws, t.info,
+                                  abort_dev, gofs, s);
+            break;
+          }
+        }
         rc = potrf_full<T>(is_upper(t.uplo), t.n, (T*)t.c, t.ldc, t.info, abort_dev, gofs, ws, s);
         break;
       }

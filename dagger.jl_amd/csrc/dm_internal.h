@@ -256,6 +256,10 @@ int tiled_skinny_gemm_launch(const dm_tiled& A, const dm_tiled& X, const dm_tile
 template <typename T>
 int potrf_full(bool upper, int n, T* A, int lda, int* info, int* abort,
                int global_offset, T* ws, hipStream_t s);
+// FP64 POTRF of an FP32 tile through the FP64 workspace W (n x n elements):
+// convert, potrf_full<double>, round the factor back (mixed.hip).
+int potrf_f64_of_f32(bool upper, int n, float* A, int lda, double* W, int* info, int* abort,
+                     int global_base, hipStream_t s);
 
 // ------------------------------------------------------------ LU (lu.hip)
 // Workspace bytes of one panel factorization (contiguous panel, TRSM

@@ -30,6 +30,35 @@ __global__ void convert_s2d_kernel(double* __restrict__ dst, const float* __rest
   if (i < count) dst[i] = (double)src[i];
 }
 
+// dst (n x n, leading dimension ldd) = src (leading dimension lds), one
+// column per blockIdx.y, converted to D.
+template <typename D, typename S>
+__global__ __launch_bounds__(256) void convert_tile_kernel(D* __restrict__ dst, int ldd,
+                                                           const S* __restrict__ src, int lds,
+                                                           int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (i < n) dst[(size_t)j * ldd + i] = (D)src[(size_t)j * lds + i];
+}
+
+// The precision form's FP64 diagonal POTRF of an FP32 tile: the tile goes
+// to the FP64 workspace W (n x n), is factored there in FP64 (potrf_full:
+// same info / abort semantics), and the factor is rounded back to FP32.  The
+// triangle POTRF leaves alone makes an exact round trip.
+int potrf_f64_of_f32(bool upper, int n, float* A, int lda, double* W, int* info, int* abort,
+                     int global_base, hipStream_t s) {
+  if (n <= 0) return 0;
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+  hipLaunchKernelGGL((convert_tile_kernel<double, float>), grid, dim3(256), 0, s, W, n,
+                     (const float*)A, lda, n);
+  DM_LAUNCH_CHECK("convert_tile_kernel");
+  int rc = potrf_full<double>(upper, n, W, n, info, abort, global_base, nullptr, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL((convert_tile_kernel<float, double>), grid, dim3(256), 0, s, A, lda,
+                     (const double*)W, n, n);
+  DM_LAUNCH_CHECK("convert_tile_kernel");
+  return 0;
+}
+
 // A[i,i] *= factor for i < n on a square tiled matrix with square blocks
 // (tile (t,t) of a Blocks(nb, nb) m x m slab starts at t*nb*m + t*nb*cols(t)
 // and has ld = rows(t) = cols(t)).

@@ -431,13 +431,13 @@ std::string key_of(const char* kind, std::initializer_list<const void*> ptrs,
   for (const void* p : ptrs) os << ':' << p;
   for (int64_t v : nums) os << ':' << v;
   os << ':' << o.num_streams << ',' << o.batch << ',' << o.lookahead << ',' << o.use_graph << ','
-     << o.profile << ',' << o.f32_update;
+     << o.profile << ',' << o.f32_update << ',' << o.f64_diag;
   return os.str();
 }
 
 dm_plan_opts opts_or_default(const dm_plan_opts* o) {
   if (o) return *o;
-  return dm_plan_opts{2, 1, 1, 1, 0, 0};
+  return dm_plan_opts{2, 1, 1, 1, 0, 0, 0};
 }
 
 // Looks up or builds (build(d) fills tasks) and plans a DAG.

@@ -43,7 +43,7 @@ class dm_task(C.Structure):
 
 class dm_plan_opts(C.Structure):
     _fields_ = [("num_streams", i32), ("batch", i32), ("lookahead", i32), ("use_graph", i32),
-                ("profile", i32), ("f32_update", i32)]
+                ("profile", i32), ("f32_update", i32), ("f64_diag", i32)]
 
 
 class dm_group_profile(C.Structure):

@@ -187,26 +187,30 @@ def make_task(f, args):
 
 
 class PlanOptions:
-    """Executor knobs: streams, batching, look-ahead depth, hipGraph, and the
+    """Executor knobs: streams, batching, look-ahead depth, hipGraph, the
     arithmetic of FP32 GEMM/SYRK groups (f32_update: 0 FP32 matrix cores,
-    6 or 3 BF16 matrix cores with 3- or 2-term operand splitting)."""
+    6 or 3 BF16 matrix cores with 3- or 2-term operand splitting), and
+    whether FP32 plans factor their diagonal tiles in FP64 (f64_diag)."""
 
     def __init__(self, num_streams=2, batch=True, lookahead=1, use_graph=False, profile=False,
-                 f32_update=0):
+                 f32_update=0, f64_diag=False):
         if f32_update not in (0, 3, 6):
             raise ValueError("f32_update must be 0, 3 or 6")
         self.num_streams, self.batch = num_streams, batch
         self.lookahead, self.use_graph, self.profile = lookahead, use_graph, profile
         self.f32_update = f32_update
+        self.f64_diag = bool(f64_diag)
 
     def c(self):
         return _lib.dm_plan_opts(int(self.num_streams), int(bool(self.batch)),
                                  int(self.lookahead), int(bool(self.use_graph)),
-                                 int(bool(self.profile)), int(self.f32_update))
+                                 int(bool(self.profile)), int(self.f32_update),
+                                 int(self.f64_diag))
 
     def replace(self, **kw):
         d = dict(num_streams=self.num_streams, batch=self.batch, lookahead=self.lookahead,
-                 use_graph=self.use_graph, profile=self.profile, f32_update=self.f32_update)
+                 use_graph=self.use_graph, profile=self.profile, f32_update=self.f32_update,
+                 f64_diag=self.f64_diag)
         d.update(kw)
         return PlanOptions(**d)
 

@@ -32,7 +32,8 @@ from .linalg import PosDefException
 
 # Trailing updates of the FP32 factor on the BF16 matrix cores with 3-term
 # operand splitting (FP32-accurate, several times the FP32 MFMA rate).
-_OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True, f32_update=6)
+_OPTS = PlanOptions(num_streams=2, batch=True, lookahead=1, use_graph=True, f32_update=6,
+                    f64_diag=True)
 
 
 def _square(t):

@@ -269,6 +269,11 @@ typedef struct dm_plan_opts {
                             operand splitting (6 or 3 products, FP32
                             accumulation) where operands are k-contiguous
                             (the upper variant), FP32 cores elsewhere */
+    int f64_diag;        /* 1: FP32 plans factor their diagonal tiles in
+                            FP64 (the tile converted to an FP64 workspace of
+                            the plan, FP64 POTRF, the factor rounded back to
+                            FP32) -- the precision form's FP64 diagonal
+                            POTRF; FP64 plans ignore it */
 } dm_plan_opts;
 
 /* Per-launch-group timing of the last profiled run (dm_plan_opts.profile):

@@ -450,6 +450,28 @@ def capped():
 def quick():
     """A few seconds of every kernel family: the CPU suite's default group."""
     rng = np.random.default_rng(2)
+    # an FP32 plan with FP64 diagonal POTRF: the untouched first diagonal
+    # tile is the FP64 Cholesky of the FP32 tile, rounded once to FP32
+    from dagger_mi355x.datadeps import PlanOptions
+    n, nb = 160, 64
+    G = rng.standard_normal((n, n))
+    A32 = (G @ G.T / n + np.eye(n)).astype(np.float32)
+    for uplo in ("U", "L"):
+        D = dm.DArray.from_array(A32, dm.Blocks(nb, nb))
+        info, ginfo = Dev(np.zeros(1, np.int32)), Dev(np.zeros(1, np.int32))
+        o = PlanOptions(num_streams=2, batch=True, lookahead=1, f64_diag=True).c()
+        t = D.tiled()
+        call("dm_cholesky_tiled", uplo.encode(), C.byref(t), info.p, ginfo.p, C.byref(o), None)
+        check(int(info.get()[0]) == 0, ("f64 diag info", uplo))
+        F = D.collect().astype(np.float64)
+        L0 = np.linalg.cholesky(A32[:nb, :nb].astype(np.float64))
+        ref0 = L0.T if uplo == "U" else L0
+        got0 = np.triu(F[:nb, :nb]) if uplo == "U" else np.tril(F[:nb, :nb])
+        check(np.all(np.abs(got0 - ref0) <= (2.0 ** -24 + 1e-12) * np.abs(ref0) + 1e-30),
+              ("f64 diag tile", uplo))
+        ref = np.array(A32, dtype=np.float64, order="F")
+        O.tile_potrf(uplo, ref)
+        check(O.rel_factor_diff(F, ref, uplo) <= 5e-5, ("f64 diag factor", uplo))
     # the allocation-free TRSM: static scratch, bitwise equal to dm_dtrsm,
     # two calls in a row (the second smaller), a solve longer than one leaf
     for (side, uplo, tr, m, n) in (("L", "U", "T", 300, 20), ("R", "L", "N", 17, 70)):

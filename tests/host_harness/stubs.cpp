@@ -93,6 +93,7 @@ template <typename T> int trinv_blocks_launch(const TrsmShape&, const T*, int, i
 template <typename T> int trsm_launch(const TrsmShape&, const TrsmJob<T>*, int, const TrsmJob<T>&, long, const int*, hipStream_t s, long max_wg) { if (max_wg < 0) abort(); trace_launch(s); ++g_launch; return 0; }
 template <typename T> int trsm_full(char, char, char, char, int, int, T, const T*, int, T*, int, T* ws, const int*, hipStream_t s) { if (!ws) abort(); trace_launch(s); ++g_launch; return 0; }
 template <typename T> int potrf_full(bool, int, T*, int, int*, int*, int, T*, hipStream_t s) { trace_launch(s); ++g_launch; return 0; }
+int potrf_f64_of_f32(bool, int, float*, int, double* W, int*, int*, int, hipStream_t s) { if (!W) abort(); trace_launch(s); ++g_launch; return 0; }
 bool gemm_split_ok(bool ta, bool tb, const GemmJob<float>& j) {
   return ta && !tb && !((uintptr_t)j.A & 15) && !((uintptr_t)j.B & 15) && !(j.lda & 3) &&
          !(j.ldb & 3);

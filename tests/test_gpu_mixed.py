@@ -103,3 +103,35 @@ def test_mixed_matches_plain_fp64_solution(dm):
     dm.ldiv_(F64, DB)
     assert np.allclose(X.collect(), DB.collect(), rtol=1e-12, atol=1e-13)
     assert info["berr"].max() <= 1e-14
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_fp64_diagonal_potrf_of_fp32_plan(dm, uplo):
+    """f64_diag: an FP32 plan factors its diagonal tiles in FP64.  The first
+    diagonal tile has no updates, so its FP32 factor is the FP64 Cholesky of
+    the FP32 tile rounded to FP32 (within one rounding of it); the whole
+    factor still meets the FP32 parity bar against LAPACK in FP64."""
+    import ctypes as C
+    from dagger_mi355x.datadeps import PlanOptions
+    from dagger_mi355x.darray import DeviceInt
+    rng = np.random.default_rng(41)
+    n, nb = 1024, 256
+    G = rng.standard_normal((n, n))
+    A32 = (G @ G.T / n + np.eye(n)).astype(np.float32)
+    D = dm.DArray.from_array(A32, dm.Blocks(nb, nb))
+    info, ginfo = DeviceInt(1), DeviceInt(1)
+    o = PlanOptions(num_streams=2, batch=True, lookahead=1, f64_diag=True).c()
+    t = D.tiled()
+    dm._lib.call("dm_cholesky_tiled", uplo.encode(), C.byref(t), C.c_void_p(info.ptr),
+                 C.c_void_p(ginfo.ptr), C.byref(o), None)
+    assert int(info.get()[0]) == 0
+    F = D.collect().astype(np.float64)
+    T0 = A32[:nb, :nb].astype(np.float64)
+    L0 = np.linalg.cholesky(T0)
+    ref0 = L0.T if uplo == "U" else L0
+    got0 = np.triu(F[:nb, :nb]) if uplo == "U" else np.tril(F[:nb, :nb])
+    # one FP32 rounding of each entry (relative 2^-24, plus tiny entries)
+    assert np.all(np.abs(got0 - ref0) <= (2.0 ** -24 + 1e-12) * np.abs(ref0) + 1e-30)
+    ref = np.array(A32, dtype=np.float64, order="F")
+    assert O.tile_potrf(uplo, ref) == 0
+    assert O.rel_factor_diff(F, ref, uplo) <= 5e-5
