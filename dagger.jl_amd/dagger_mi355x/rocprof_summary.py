@@ -23,11 +23,12 @@
       spent waiting.
 
   python -m dagger_mi355x.rocprof_summary window KERNEL_TRACE_CSV \
-          --start REGEX --stop REGEX
+          --start REGEX --stop REGEX [--index I]
       windows of a kernel trace: each opens at a dispatch whose kernel
       matches --start and closes before the next dispatch matching --stop
       (or --start); prints every window's span (first start to last end) and
-      dispatch count, and the per-kernel time of the last window.
+      dispatch count, and the per-kernel time of window I (default: the
+      last).
 
 Prints a short text summary; the `traffic` mode ends with machine-readable
 lines `traffic_bytes_per_launch: <bytes>` and `fetch_scale`/`write_scale`.
@@ -171,7 +172,7 @@ def counters(paths, kernel, simds, xcds):
     return "\n".join(out)
 
 
-def window(path, start, stop):
+def window(path, start, stop, index=-1):
     rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
     ps, pe = re.compile(start), re.compile(stop)
     wins, cur = [], None
@@ -192,11 +193,11 @@ def window(path, start, stop):
         t1 = max(int(r["End_Timestamp"]) for r in w)
         out.append(f"window {i}: span {(t1 - t0) / 1e6:.3f} ms, {len(w)} dispatches")
     agg = defaultdict(lambda: [0, 0.0])
-    for r in wins[-1]:
+    for r in wins[index]:
         a = agg[r["Kernel_Name"]]
         a[0] += 1
         a[1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
-    out.append(f"last window, per kernel:")
+    out.append(f"window {index % len(wins)}, per kernel:")
     out.append(f"{'calls':>7} {'total ms':>10} {'avg us':>10}  kernel")
     for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
         out.append(f"{c:>7} {t / 1e6:>10.3f} {t / c / 1e3:>10.2f}  {_short(name)}")
@@ -223,9 +224,10 @@ def main(argv=None):
     w.add_argument("csv")
     w.add_argument("--start", required=True)
     w.add_argument("--stop", required=True)
+    w.add_argument("--index", type=int, default=-1, help="window to break down (default: last)")
     a = ap.parse_args(argv)
     if a.mode == "window":
-        print(window(a.csv, a.start, a.stop))
+        print(window(a.csv, a.start, a.stop, a.index))
         return 0
     if a.mode == "stats":
         print(stats(a.csv, a.top))

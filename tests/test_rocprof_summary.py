@@ -79,5 +79,10 @@ def test_window_splits_a_trace(tmp_path):
     out = R.window(str(p), "trtri_leaf", "potrf_").splitlines()
     assert out[0] == "window 0: span 0.003 ms, 3 dispatches"
     assert out[1] == "window 1: span 0.002 ms, 2 dispatches"
+    assert out[2] == "window 1, per kernel:"
+    first = R.window(str(p), "trtri_leaf", "potrf_", index=0).splitlines()
+    assert first[2] == "window 0, per kernel:"
+    assert any("gemm_b" in line for line in first[3:])
+    assert not any("gemm_b" in line for line in out[3:])
     with pytest.raises(SystemExit):
         R.window(str(p), "nothing_matches", "potrf_")
