@@ -30,6 +30,7 @@ baseline replay a trace with Python's cyclic garbage collector paused
 inside the loop would free device memory, and hipFree waits for the whole
 device.
 """
+import contextlib
 import ctypes as C
 import gc
 import threading
@@ -130,8 +131,15 @@ class Server:
     the refinement's extra launches and synchronizations."""
 
     def __init__(self, workers=4, budget_bytes=200e9, cache=True, options=None,
-                 mixed_min_n=4096):
+                 mixed_min_n=4096, blas_threads=1):
         self.nworkers = workers
+        # Host BLAS threads while serving (None: leave the library's pool).
+        # The workers' host algebra (the generator's compact-WY products, the
+        # direct path's residual check) runs on numpy from every worker at
+        # once; with a 16-thread OpenBLAS pool per call the six workers
+        # oversubscribed the host 16-fold: one thread each measured 587
+        # solves/s against 234 (p99 205 ms against 971) at 1000 req/s.
+        self.blas_threads = blas_threads
         self.budget = float(budget_bytes)
         self.cache_enabled = cache
         self.mixed_min_n = mixed_min_n
@@ -331,7 +339,7 @@ class Server:
         with self.lock:
             self.stop = False
             self.done = []
-        with _gc_paused():
+        with _gc_paused(), blas_limited(self.blas_threads):
             return self._replay(trace, time_scale)
 
     def _replay(self, trace, time_scale):
@@ -407,15 +415,27 @@ class _gc_paused:
         return False
 
 
-def serve_one_at_a_time(trace, options=None):
+@contextlib.contextmanager
+def blas_limited(threads):
+    """Host BLAS pools limited to `threads` for the block (None: unchanged)."""
+    if threads is None:
+        yield
+        return
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=int(threads)):
+        yield
+
+
+def serve_one_at_a_time(trace, options=None, blas_threads=None):
     """Baseline: requests strictly one at a time in arrival order, each a
     plain FP64 tiled cholesky! + ldiv! of its own matrix.  Like the server
     it keeps its device buffers (one matrix and right-hand-side array per
     padded size, staging scratch), so cached plans and captured graphs are
     reused and no request allocates or frees device memory once warm; it
     does not cache factors, overlap requests or use low precision.  Each
-    result's backward error is measured like the server's direct path."""
-    with _gc_paused():
+    result's backward error is measured like the server's direct path.
+    blas_threads: host BLAS pool size for its run (None: the default pool)."""
+    with _gc_paused(), blas_limited(blas_threads):
         return _one_at_a_time(trace, options)
 
 
