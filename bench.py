@@ -252,8 +252,9 @@ def batched_leg(dm, count, seed, reps=3):
     """Batching form: `count` uneven SPD systems A = G G^T/n + I with
     n ~ LogUniform[32, 2048] and nrhs ~ U{1..8}, FP64 and FP32, factored and
     solved by the tiled level-synchronous plan (BatchedSystems.solve,
-    dm_batched_tiled_*; plan built once, outside the timed runs, like the
-    executor's cached plans), and for comparison by one persistent launch
+    dm_batched_tiled_*, outer tile per size bucket from the tile autotuner;
+    plan built once, outside the timed runs, like the executor's cached
+    plans), and for comparison by one persistent launch
     (dm_batched_posv, panel width per size bucket from the autotuner).
     Timed with HIP events, systems restored from device copies between
     runs; reports systems/s and (sum n^3/3 + sum 2 n^2 nrhs) / time, plus
@@ -269,6 +270,7 @@ def batched_leg(dm, count, seed, reps=3):
         picks = [int(i) for i in np.linspace(0, count - 1, 8)]
         originals = {m: bs.get_system(m) for m in picks}
         panels, _ = batched.autotune(bs, sample=16, reps=1)
+        tiles, _ = batched.autotune_tiled(bs, sample=32, reps=1)
         snap = bs.snapshot()
 
         def timed(run):
@@ -284,7 +286,7 @@ def batched_leg(dm, count, seed, reps=3):
             return sum(times) / len(times)
 
         ms_persistent = timed(lambda: bs.posv(panels=panels))
-        ms = timed(lambda: bs.solve(panels=panels))
+        ms = timed(lambda: bs.solve(panels=panels, tile_nb=tiles))
         bad = int((bs.infos() != 0).sum())
         berr = 0.0
         for m in picks:
@@ -299,7 +301,8 @@ def batched_leg(dm, count, seed, reps=3):
         out[name] = {"ms": round(ms, 3), "systems_per_s": round(count / (ms * 1e-3), 1),
                      "tflops": round((ff + fs) / (ms * 1e-3) / 1e12, 3), "failed": bad,
                      "max_sampled_berr": berr, "path": "tiled level-synchronous",
-                     "tile_nb": batched.TILED_NB,
+                     "solve_tile": batched.TILED_NB,
+                     "factor_tiles": {str(k): v for k, v in tiles.items()},
                      "persistent_ms": round(ms_persistent, 3),
                      "persistent_tflops": round((ff + fs) / (ms_persistent * 1e-3) / 1e12, 3),
                      "panels": {str(k): v for k, v in panels.items()}}

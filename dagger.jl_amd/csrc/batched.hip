@@ -387,7 +387,9 @@ extern "C" int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_b
 //        block's update (a batched SYRK-shaped GEMM with K = 64);
 //   after an outer tile's last block, the trailing matrix below it gets the
 //   whole tile's update A22 -= U12^T U12 in one batched GEMM with K = nb
-//   (the bulk of the flops, at the matrix cores' K-blocked rate).
+//   (the bulk of the flops, at the matrix cores' K-blocked rate).  Each
+//   system may have its own outer tile (item_nb: picked per size bucket by
+//   the host autotuner); the solves use the plan's nb for every system.
 // Then U^T y = b and U x = y by 64-row blocks, right-looking, in place on a
 // working copy of B: a batched in-register triangular solve of each
 // system's diagonal block, then one batched GEMM subtracting the solved
@@ -429,6 +431,8 @@ struct TiledSys {
   int64_t z_off;  // its right-hand sides in the plan's working slab (ld = n)
   int n, ld, nrhs;
   int sys;        // index into the batch (and into info)
+  int tnb;        // the system's outer tile of the factorization (multiple of 64)
+  int pad_;
 };
 
 __global__ void zero_info_kernel(const TiledSys* __restrict__ d, int count,
@@ -639,7 +643,8 @@ struct BatchTiled : BatchTiledBase {
     return copy(x_copy, info, s);  // gated by each system's info
   }
 
-  int build(T* A_base, T* B_base, const BatchItem* items, const int* order, int count, int nb_) {
+  int build(T* A_base, T* B_base, const BatchItem* items, const int* order, int count, int nb_,
+            const int* item_nb) {
     A = A_base;
     nb = nb_;
     // systems largest first (stable), so the systems active at a block are a prefix
@@ -658,6 +663,7 @@ struct BatchTiled : BatchTiledBase {
       t.ld = it.lda > 0 ? it.lda : it.n;
       t.nrhs = it.nrhs;
       t.sys = sys[i];
+      t.tnb = item_nb && item_nb[sys[i]] > 0 ? item_nb[sys[i]] : nb;
       if (it.nrhs > 0 && it.n > 0) {
         t.z_off = z_elems;
         z_elems += (int64_t)it.n * it.nrhs;
@@ -678,14 +684,15 @@ struct BatchTiled : BatchTiledBase {
     for (int b = 0; b < nblk; ++b) {
       Step& S = steps[b];
       const int jj = 64 * b;
-      const int64_t c0 = (int64_t)(jj / nb) * nb;   // the outer tile holding this block
-      const bool last_in_tile = jj + 64 == c0 + nb;
       while (S.count < nsys && td[S.count].n > jj) ++S.count;
       S.inner.ta = S.trail.ta = true;
       for (int i = 0; i < S.count; ++i) {
         const TiledSys& t = td[i];
         const int nrem = t.n - jj - 64;
         if (nrem <= 0) continue;
+        const int tnb = t.tnb;
+        const int64_t c0 = (int64_t)(jj / tnb) * tnb;  // the outer tile holding this block
+        const bool last_in_tile = jj + 64 == c0 + tnb;
         {  // in place: one workgroup holds all 64 rows of its columns and
            // reads them (K = 64) before it stores (beta = 0: C is not read)
           GemmJob<T> g{};
@@ -696,7 +703,7 @@ struct BatchTiled : BatchTiledBase {
           g.alpha = T(1); g.beta = T(0);
           S.row.jobs.push_back(g);
         }
-        const int c_end = (int)std::min<int64_t>(c0 + nb, t.n);
+        const int c_end = (int)std::min<int64_t>(c0 + tnb, t.n);
         const int rows = c_end - jj - 64;
         if (rows > 0) {  // rows of this outer tile below the block, K = 64:
           // their upper triangle (a square triangular job) ...
@@ -717,14 +724,14 @@ struct BatchTiled : BatchTiledBase {
             S.inner.jobs.push_back(g);
           }
         }
-        if (last_in_tile && t.n > c0 + nb) {  // the tile's update of A22: K = nb
-          const int m = (int)(t.n - c0 - nb);
+        if (last_in_tile && t.n > c0 + tnb) {  // the tile's update of A22: K = tnb
+          const int m = (int)(t.n - c0 - tnb);
           GemmJob<T> g{};
-          g.A = g.B = a_at(t, c0, c0 + nb);
+          g.A = g.B = a_at(t, c0, c0 + tnb);
           g.lda = g.ldb = t.ld;
-          g.C = a_at(t, c0 + nb, c0 + nb);
+          g.C = a_at(t, c0 + tnb, c0 + tnb);
           g.ldc = t.ld;
-          g.M = m; g.N = m; g.K = nb;
+          g.M = m; g.N = m; g.K = tnb;
           g.alpha = T(-1); g.beta = T(1);
           g.tri = 1;
           S.trail.jobs.push_back(g);
@@ -848,7 +855,8 @@ struct dm_batch_plan_s {
 
 extern "C" int dm_batched_tiled_create(dm_batch_plan_t* plan, int dtype, int num_items,
                                        void* A_base, void* B_base, const void* items_host,
-                                       int count, const int* order_host, int nb) {
+                                       int count, const int* order_host, int nb,
+                                       const int* item_nb) {
   if (!plan) return arg_fail(1, "plan is NULL");
   *plan = nullptr;
   if (dtype != DM_F64 && dtype != DM_F32) return arg_fail(2, "dtype must be DM_F64 or DM_F32");
@@ -867,15 +875,19 @@ extern "C" int dm_batched_tiled_create(dm_batch_plan_t* plan, int dtype, int num
   for (int i = 0; i < count; ++i)
     if (order_host[i] < 0 || order_host[i] >= num_items)
       return arg_fail(8, "order entry outside [0, num_items)");
+  if (item_nb)
+    for (int i = 0; i < num_items; ++i)
+      if (item_nb[i] != 0 && (item_nb[i] < 64 || item_nb[i] > 1024 || item_nb[i] % 64))
+        return arg_fail(10, "item_nb entry not 0 or a multiple of 64 in [64, 1024]");
   std::unique_ptr<dm_batch_plan_s> h(new dm_batch_plan_s());
   if (dtype == DM_F64) {
     auto* t = new BatchTiled<double>();
     h->p.reset(t);
-    rc = t->build((double*)A_base, (double*)B_base, items, order_host, count, nb);
+    rc = t->build((double*)A_base, (double*)B_base, items, order_host, count, nb, item_nb);
   } else {
     auto* t = new BatchTiled<float>();
     h->p.reset(t);
-    rc = t->build((float*)A_base, (float*)B_base, items, order_host, count, nb);
+    rc = t->build((float*)A_base, (float*)B_base, items, order_host, count, nb, item_nb);
   }
   if (rc) return rc;
   *plan = h.release();

@@ -132,7 +132,7 @@ _PROTOS = {
     "dm_ishermitian_tiled": (i32, [C.POINTER(dm_tiled), vp, vp, vp]),
     "dm_tiled_norm2": (i32, [C.POINTER(dm_tiled), i32, vp, vp, vp, vp]),
     "dm_batched_posv": (i32, [i32, i32, vp, vp, vp, i32, vp, vp, i32, i32, vp]),
-    "dm_batched_tiled_create": (i32, [C.POINTER(vp), i32, i32, vp, vp, vp, i32, vp, i32]),
+    "dm_batched_tiled_create": (i32, [C.POINTER(vp), i32, i32, vp, vp, vp, i32, vp, i32, vp]),
     "dm_batched_tiled_run": (i32, [vp, vp, i32, vp]),
     "dm_batched_tiled_destroy": (i32, [vp]),
     "dm_gemm_tiled": (i32, [c_char, c_char, f64, C.POINTER(dm_tiled), C.POINTER(dm_tiled), f64,

@@ -208,21 +208,23 @@ class BatchedSystems:
         large = self.sizes >= tiled_min
         return idx[~large], idx[large]
 
-    def tiled_plan(self, subset, nb=TILED_NB):
-        """Tiled level-synchronous plan over `subset` (cached per subset and
-        nb: the plan holds device tables and a working slab for them)."""
-        key = (tuple(int(i) for i in subset), nb)
+    def tiled_plan(self, subset, nb=TILED_NB, tile_nb=None):
+        """Tiled level-synchronous plan over `subset` (cached per subset, nb
+        and per-bucket tiles: the plan holds device tables and a working
+        slab for them)."""
+        key = (tuple(int(i) for i in subset), nb, tuple(sorted((tile_nb or {}).items())))
         cache = self.__dict__.setdefault("_plans", {})
         if key not in cache:
-            cache[key] = TiledPlan(self, subset, nb)
+            cache[key] = TiledPlan(self, subset, nb, tile_nb)
         return cache[key]
 
     def solve(self, panels=None, factor_only=False, tiled_min=TILED_MIN_N, nb=TILED_NB,
-              workgroups=0, overlap=True):
+              workgroups=0, overlap=True, tile_nb=None):
         """Hybrid: systems with n < tiled_min in one persistent launch (panel
         width per size bucket), the rest through the tiled plan.  overlap:
         the persistent launch runs on a second stream beside the plan's
-        launches (forked from and joined back into the default stream)."""
+        launches (forked from and joined back into the default stream).
+        tile_nb: per-bucket outer tiles of the tiled plan (autotune_tiled)."""
         small, large = self.split(tiled_min)
         if len(small) and len(large) and overlap:
             side, fork, join = self._side()
@@ -230,7 +232,7 @@ class BatchedSystems:
             call("dm_stream_wait_event", side, fork)
             self.posv(panels=panels, factor_only=factor_only, subset=small,
                       workgroups=workgroups, stream=side)
-            self.tiled_plan(large, nb).run(factor_only)
+            self.tiled_plan(large, nb, tile_nb).run(factor_only)
             call("dm_event_record", join, side)
             call("dm_stream_wait_event", None, join)
             return self
@@ -238,7 +240,7 @@ class BatchedSystems:
             self.posv(panels=panels, factor_only=factor_only, subset=small,
                       workgroups=workgroups)
         if len(large):
-            self.tiled_plan(large, nb).run(factor_only)
+            self.tiled_plan(large, nb, tile_nb).run(factor_only)
         return self
 
     def _side(self):
@@ -271,16 +273,23 @@ class BatchedSystems:
 
 
 class TiledPlan:
-    """dm_batched_tiled_* over a subset of a BatchedSystems batch."""
+    """dm_batched_tiled_* over a subset of a BatchedSystems batch.  tile_nb:
+    dict size bucket -> outer tile of the factorization for that bucket's
+    systems (others: nb, which is also the solves' tile)."""
 
-    def __init__(self, systems, subset, nb=TILED_NB):
+    def __init__(self, systems, subset, nb=TILED_NB, tile_nb=None):
         self.systems = systems
         it = systems.items({})
         order = systems.order(subset)
+        tile_nb = tile_nb or {}
+        per = np.array([int(tile_nb.get(bucket_of(int(n)), 0)) for n in systems.sizes],
+                       dtype=np.int32)
+        self.item_nb = np.ascontiguousarray(per if len(per) else np.zeros(1, np.int32))
         self.handle = C.c_void_p()
         call("dm_batched_tiled_create", C.byref(self.handle), _DT[systems.dtype], systems.count,
              C.c_void_p(systems.A.ptr), C.c_void_p(systems.B.ptr), C.cast(it, C.c_void_p),
-             len(order), order.ctypes.data_as(C.c_void_p), int(nb))
+             len(order), order.ctypes.data_as(C.c_void_p), int(nb),
+             self.item_nb.ctypes.data_as(C.c_void_p))
 
     def run(self, factor_only=False, stream=None):
         call("dm_batched_tiled_run", self.handle, C.c_void_p(self.systems.info.ptr),
@@ -329,6 +338,56 @@ def autotune(systems, sample=24, reps=2):
     systems.restore(snap)
     call("dm_event_destroy", e0)
     call("dm_event_destroy", e1)
+    return choice, table
+
+
+TILE_CANDIDATES = (128, 192, 256, 320, 384, 512)
+
+
+def autotune_tiled(systems, candidates=TILE_CANDIDATES, sample=64, reps=2):
+    """Pick the tiled plan's outer tile per size bucket: for each bucket,
+    time the factorization of up to `sample` of its systems (a tiled plan
+    over just them, HIP events, systems restored between trials) with each
+    candidate no larger than twice the bucket's largest n, and keep the
+    fastest.  Returns dict bucket -> tile and the timing table."""
+    snap = systems.snapshot()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    call("dm_event_create", C.byref(e0))
+    call("dm_event_create", C.byref(e1))
+    buckets = {}
+    for m in range(systems.count):
+        buckets.setdefault(bucket_of(int(systems.sizes[m])), []).append(m)
+    choice, table = {}, {}
+    try:
+        for b, members in sorted(buckets.items()):
+            sub = members[:sample]
+            nmax = int(systems.sizes[sub].max())
+            best = None
+            for nb in candidates:
+                if nb > max(2 * nmax, candidates[0]):
+                    continue
+                plan = TiledPlan(systems, sub, nb)
+                ms_best = None
+                for _ in range(reps):
+                    systems.restore(snap)
+                    call("dm_event_record", e0, None)
+                    plan.run(factor_only=True)
+                    call("dm_event_record", e1, None)
+                    call("dm_event_sync", e1)
+                    ms = C.c_float()
+                    call("dm_event_elapsed_ms", C.byref(ms), e0, e1)
+                    ms_best = ms.value if ms_best is None else min(ms_best, ms.value)
+                plan.close()
+                table[(b, nb)] = ms_best
+                if best is None or ms_best < best[1]:
+                    best = (nb, ms_best)
+            choice[b] = best[0]
+    finally:
+        systems.restore(snap)
+        for buf in snap:
+            buf.free()
+        call("dm_event_destroy", e0)
+        call("dm_event_destroy", e1)
     return choice, table
 
 

@@ -469,7 +469,10 @@ int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_base,
  * blocks (batched in-register solves of the diagonal blocks and batched
  * GEMM updates) on a working copy of the right-hand sides.  Items and order
  * as for dm_batched_posv (each item's `panel` is ignored); nb a multiple of
- * 64 in [64, 1024].  The plan holds its job tables and an n x nrhs working
+ * 64 in [64, 1024]: the solves' tile, and the factorization's outer tile
+ * of every item whose item_nb entry is 0 (item_nb: num_items entries, or
+ * NULL; a nonzero entry, a multiple of 64 in [64, 1024], is that system's
+ * outer tile -- the autotuner picks it per size bucket).  The plan holds its job tables and an n x nrhs working
  * slab per system; run it any number of times (on any stream, one run at a
  * time), each run factoring A in place and, unless factor_only, overwriting
  * B_m with the solution of every system whose info is 0 (a failed system's
@@ -477,7 +480,7 @@ int dm_batched_posv(int dtype, int num_items, void* A_base, void* B_base,
 typedef struct dm_batch_plan_s* dm_batch_plan_t;
 int dm_batched_tiled_create(dm_batch_plan_t* plan, int dtype, int num_items, void* A_base,
                             void* B_base, const void* items_host, int count,
-                            const int* order_host, int nb);
+                            const int* order_host, int nb, const int* item_nb);
 int dm_batched_tiled_run(dm_batch_plan_t plan, int* info_dev, int factor_only, dm_stream_t s);
 int dm_batched_tiled_destroy(dm_batch_plan_t plan);
 

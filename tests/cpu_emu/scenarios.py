@@ -376,7 +376,9 @@ def batched_tiled():
             A5[99, 99] = -1.0
             bs.put_system(5, A=A5)
         A0s = [bs.get_system(m) for m in range(bs.count)]
-        bs.solve(panels={}, tiled_min=64, nb=128)
+        # per-bucket outer tiles: 64 for n in [64, 128), 192 for [128, 256)
+        tile_nb = {6: 64, 7: 192} if dtype == np.float64 else None
+        bs.solve(panels={}, tiled_min=64, nb=128, tile_nb=tile_nb)
         infos = bs.infos()
         tol = 10 * np.finfo(dtype).eps
         for m in range(bs.count):
@@ -400,7 +402,7 @@ def batched_tiled():
             F1 = [bs.get_system(m) for m in range(bs.count)]
             for m in range(bs.count):
                 bs.put_system(m, A=A0s[m][0], B=A0s[m][1])
-            bs.solve(panels={}, tiled_min=64, nb=128)
+            bs.solve(panels={}, tiled_min=64, nb=128, tile_nb=tile_nb)
             for m in range(bs.count):
                 F2, X2 = bs.get_system(m)
                 check(np.array_equal(F2, F1[m][0]) and np.array_equal(X2, F1[m][1]),
@@ -413,6 +415,12 @@ def batched_tiled():
             raise AssertionError(("bad tiled plan accepted", bad))
         except dm.DMError:
             pass
+    try:  # a per-item tile that is not a multiple of 64
+        bs = Bt.BatchedSystems([70], [1]).generate(seed=1)
+        Bt.TiledPlan(bs, [0], nb=128, tile_nb={6: 100})
+        raise AssertionError("bad item tile accepted")
+    except dm.DMError:
+        pass
 
 
 # ---------------------------------------------------------------- serving

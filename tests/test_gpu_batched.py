@@ -114,3 +114,24 @@ def test_batched_tiled_failed_system_keeps_b(dm):
         _, X = bs.get_system(m)
         assert O.backward_error(A0s[m], X, B0s[m]) <= 10 * sizes[m] * np.finfo(float).eps
     bs.close()
+
+
+def test_batched_tiled_per_bucket_tiles_and_autotune(dm):
+    """Per-bucket outer tiles (one plan mixing 64, 192 and 384), checked
+    per system against LAPACK; the tile autotuner returns one candidate per
+    bucket and the plan it picks solves the batch."""
+    from dagger_mi355x import batched
+    sizes, nrhs = batched.log_uniform_sizes(60, 32, 1500, seed=71)
+    bs = batched.BatchedSystems(sizes, nrhs).generate(seed=72)
+    A0s, B0s = zip(*[bs.get_system(m) for m in range(bs.count)])
+    tiles = {b: (64, 192, 384)[b % 3] for b in range(5, 11)}
+    bs.solve(panels={}, tile_nb=tiles)
+    _check(bs, A0s, B0s, 1e-12)
+    choice, table = batched.autotune_tiled(bs, sample=6, reps=1)
+    assert set(choice) == {batched.bucket_of(int(n)) for n in sizes}
+    assert all(v in batched.TILE_CANDIDATES for v in choice.values())
+    for m in range(bs.count):
+        bs.put_system(m, A=A0s[m], B=B0s[m])
+    bs.solve(panels={}, tile_nb=choice)
+    _check(bs, A0s, B0s, 1e-12)
+    bs.close()
