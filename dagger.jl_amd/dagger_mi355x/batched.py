@@ -92,6 +92,8 @@ class BatchedSystems:
         es = self.dtype.itemsize
         for m in range(self.count):
             n = int(self.sizes[m])
+            if n == 0:  # an empty system: nothing to generate
+                continue
             call("dm_fill_randn_d", C.c_void_p(G.ptr), C.c_size_t(n * n), C.c_uint64(seed + m),
                  C.c_uint64(0), None)
             call("dm_dgemm", b"N", b"T", n, n, n, 1.0 / n, C.c_void_p(G.ptr), n,

@@ -408,6 +408,28 @@ def batched_tiled():
                 check(np.array_equal(F2, F1[m][0]) and np.array_equal(X2, F1[m][1]),
                       ("batched tiled rerun", m))
         bs.close()
+    # edge sizes in one plan: 1, 2, ragged around the 64-wide block, and an
+    # empty system (its info must read 0); ld padded to a multiple of 8
+    sizes = np.array([1, 2, 63, 64, 65, 0, 130])
+    nrhs = np.array([1, 2, 3, 0, 8, 1, 2])
+    bs = Bt.BatchedSystems(sizes, nrhs, np.float64, ld_pad=8).generate(seed=13)
+    A0s = [bs.get_system(m) for m in range(bs.count)]
+    call("dm_fill_i32", C.c_void_p(bs.info.ptr), 7, C.c_size_t(bs.count), None)
+    bs.solve(panels={}, nb=64)
+    infos = bs.infos()
+    check(np.all(infos == 0), ("edge sizes info", infos))
+    for m in range(bs.count):
+        A0, B0 = A0s[m]
+        if sizes[m] == 0:
+            continue
+        F, X = bs.get_system(m)
+        ref = np.array(A0, order="F")
+        O.tile_potrf("U", ref)
+        check(O.rel_factor_diff(F, ref, "U") <= 1e-12, ("edge sizes factor", m))
+        if B0.shape[1]:
+            check(O.backward_error(A0, X, B0) <= 10 * max(sizes[m], 1) * np.finfo(float).eps,
+                  ("edge sizes solve", m))
+    bs.close()
     for bad in (32, 96 + 1, 2048):  # nb not a multiple of 64 in [64, 1024]
         try:
             bs = Bt.BatchedSystems([70], [1]).generate(seed=1)
