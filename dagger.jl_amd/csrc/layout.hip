@@ -150,19 +150,24 @@ extern "C" int dm_add_diag_d(double* A, int64_t ld, int64_t n, const double* d_d
 // max(||A_ij||_F, ||A_ji||_F) (is_cross_hermitian, src/array/linalg.jl:38-48).
 // One workgroup per tile pair; fixed-order reductions.
 namespace dm {
-// One workgroup (16 waves) per tile pair (i >= j), walking the pair in 64 x 64
+// One workgroup (8 waves) per tile pair (i >= j), walking the pair in 64 x 64
 // sub-blocks: the sub-block of A_ij is read along its columns, the matching
 // sub-block of A_ji along its columns too and transposed through LDS, so
 // both reads are contiguous (A is streamed from HBM once).  CPLX: elements
 // are (re, im) pairs of T and the partner is conjugated (A_ji^H), so a
 // diagonal tile must also have an exactly zero imaginary diagonal.
+// The sub-blocks are software-pipelined: both tiles' loads of the next
+// sub-block are issued before the current one is compared, so two
+// sub-blocks' reads are in flight per workgroup.
+constexpr int HERM_NT = 1024;
 template <typename T, bool CPLX>
-__global__ __launch_bounds__(1024) void herm_pair_kernel(const T* __restrict__ A, int64_t m,
-                                                        int mb, int mt,
-                                                        int* __restrict__ ok_pairs) {
+__global__ __launch_bounds__(HERM_NT) void herm_pair_kernel(const T* __restrict__ A, int64_t m,
+                                                           int mb, int mt,
+                                                           int* __restrict__ ok_pairs) {
   constexpr int S = CPLX ? 2 : 1;  // T values per element
+  constexpr int NW = HERM_NT / 64;  // waves: rows hi, hi + NW, ... of a sub-block
   __shared__ T ts[S][64 * 65];
-  __shared__ double s0[1024], s1[1024], s2[1024];
+  __shared__ double s0[HERM_NT], s1[HERM_NT], s2[HERM_NT];
   // decode pair index -> (i, j) with i >= j, column by column of the lower triangle
   int p = blockIdx.x, j = 0;
   while (p >= mt - j) { p -= mt - j; ++j; }
@@ -177,45 +182,71 @@ __global__ __launch_bounds__(1024) void herm_pair_kernel(const T* __restrict__ A
   const T* Aji = tile(j, i, ri);  // rj x ri, ld = rj
   const int tid = threadIdx.x, lo = tid & 63, hi = tid >> 6;
   double d2 = 0, n1 = 0, n2 = 0;
-  for (int r0 = 0; r0 < ri; r0 += 64) {
-    for (int c0 = 0; c0 < rj; c0 += 64) {
-      // ts[rr * 65 + cc] = A_ji(c0 + cc, r0 + rr) = (A_ji^T)(r0 + rr, c0 + cc)
-      __syncthreads();  // the previous sub-block's reads are done
+  constexpr int Q = 64 / NW;
+  const int ncb = (rj + 63) / 64, nsub = ((ri + 63) / 64) * ncb;
+  // registers of one sub-block: A_ji elements (transposed into LDS) and the
+  // matching A_ij elements (compared against LDS)
+  T xji[Q][S], xij[Q][S];
+  auto load = [&](int sb) {
+    const int r0 = (sb / ncb) * 64, c0 = (sb % ncb) * 64;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rr = hi + 16 * q, cc = lo;
+    for (int q = 0; q < Q; ++q) {
+      {  // A_ji(c0 + lo, r0 + rr): column r0 + rr of A_ji, rows along lanes
+        const int rr = hi + NW * q, cc = lo;
         const bool in = r0 + rr < ri && c0 + cc < rj;
 #pragma unroll
         for (int h = 0; h < S; ++h)
-          ts[h][rr * 65 + cc] = in ? Aji[S * ((int64_t)(r0 + rr) * rj + c0 + cc) + h] : T(0);
+          xji[q][h] = in ? Aji[S * ((int64_t)(r0 + rr) * rj + c0 + cc) + h] : T(0);
       }
-      __syncthreads();
+      {  // A_ij(r0 + lo, c0 + cc): column c0 + cc of A_ij, rows along lanes
+        const int rr = lo, cc = hi + NW * q;
+        const bool in = r0 + rr < ri && c0 + cc < rj;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rr = lo, cc = hi + 16 * q;
-        if (r0 + rr < ri && c0 + cc < rj) {
-          const int64_t ia = S * ((int64_t)(c0 + cc) * ri + r0 + rr);
-          const double a = (double)Aij[ia];
-          const double b = (double)ts[0][rr * 65 + cc];
-          double ai = 0, bi = 0;
-          if (CPLX) {
-            ai = (double)Aij[ia + 1];
-            bi = -(double)ts[S - 1][rr * 65 + cc];  // conj
-          }
-          if (i == j) {
-            d2 += (a != b || ai != bi) ? 1.0 : 0.0;
-          } else {
-            d2 += (a - b) * (a - b) + (ai - bi) * (ai - bi);
-            n1 += a * a + ai * ai;
-            n2 += b * b + bi * bi;
-          }
+        for (int h = 0; h < S; ++h)
+          xij[q][h] = in ? Aij[S * ((int64_t)(c0 + cc) * ri + r0 + rr) + h] : T(0);
+      }
+    }
+  };
+  if (nsub > 0) load(0);
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int r0 = (sb / ncb) * 64, c0 = (sb % ncb) * 64;
+    // ts[rr * 65 + cc] = A_ji(c0 + cc, r0 + rr) = (A_ji^T)(r0 + rr, c0 + cc)
+    __syncthreads();  // the previous sub-block's LDS reads are done
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int h = 0; h < S; ++h) ts[h][(hi + NW * q) * 65 + lo] = xji[q][h];
+    T cur[Q][S];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int h = 0; h < S; ++h) cur[q][h] = xij[q][h];
+    if (sb + 1 < nsub) load(sb + 1);  // in flight during the comparison below
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int rr = lo, cc = hi + NW * q;
+      if (r0 + rr < ri && c0 + cc < rj) {
+        const double a = (double)cur[q][0];
+        const double b = (double)ts[0][rr * 65 + cc];
+        double ai = 0, bi = 0;
+        if (CPLX) {
+          ai = (double)cur[q][S - 1];
+          bi = -(double)ts[S - 1][rr * 65 + cc];  // conj
+        }
+        if (i == j) {
+          d2 += (a != b || ai != bi) ? 1.0 : 0.0;
+        } else {
+          d2 += (a - b) * (a - b) + (ai - bi) * (ai - bi);
+          n1 += a * a + ai * ai;
+          n2 += b * b + bi * bi;
         }
       }
     }
   }
   s0[tid] = d2; s1[tid] = n1; s2[tid] = n2;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
+  for (int w = HERM_NT / 2; w > 0; w >>= 1) {
     if (tid < w) {
       s0[tid] += s0[tid + w];
       s1[tid] += s1[tid + w];
@@ -263,19 +294,19 @@ extern "C" int dm_ishermitian_tiled(const dm_tiled* A, int* result_dev, int* wor
   if (pairs == 0) return fill_i32_launch(result_dev, 1, 1, st);
   switch (A->dtype) {
     case DM_F64:
-      hipLaunchKernelGGL((herm_pair_kernel<double, false>), dim3(pairs), dim3(1024), 0, st,
+      hipLaunchKernelGGL((herm_pair_kernel<double, false>), dim3(pairs), dim3(HERM_NT), 0, st,
                          (const double*)A->base, A->m, A->mb, mt, work_dev);
       break;
     case DM_F32:
-      hipLaunchKernelGGL((herm_pair_kernel<float, false>), dim3(pairs), dim3(1024), 0, st,
+      hipLaunchKernelGGL((herm_pair_kernel<float, false>), dim3(pairs), dim3(HERM_NT), 0, st,
                          (const float*)A->base, A->m, A->mb, mt, work_dev);
       break;
     case DM_C64:
-      hipLaunchKernelGGL((herm_pair_kernel<double, true>), dim3(pairs), dim3(1024), 0, st,
+      hipLaunchKernelGGL((herm_pair_kernel<double, true>), dim3(pairs), dim3(HERM_NT), 0, st,
                          (const double*)A->base, A->m, A->mb, mt, work_dev);
       break;
     case DM_C32:
-      hipLaunchKernelGGL((herm_pair_kernel<float, true>), dim3(pairs), dim3(1024), 0, st,
+      hipLaunchKernelGGL((herm_pair_kernel<float, true>), dim3(pairs), dim3(HERM_NT), 0, st,
                          (const float*)A->base, A->m, A->mb, mt, work_dev);
       break;
     default:

@@ -502,6 +502,18 @@ def quick():
         ref = np.array(A32, dtype=np.float64, order="F")
         O.tile_potrf(uplo, ref)
         check(O.rel_factor_diff(F, ref, uplo) <= 5e-5, ("f64 diag factor", uplo))
+    # ishermitian with tiles wider than one 64 x 64 sub-block (ragged
+    # 150 = 2 x 64 + 22): the pair check walks several sub-blocks per pair
+    Ah = gen.spd(300, 6)
+    check(dm.ishermitian(dm.DArray.from_array(Ah, dm.Blocks(150, 150))), "ishermitian wide")
+    Ab = Ah.copy()
+    Ab[290, 140] += 1e-3  # tile pair (1, 0), last sub-block row, ragged column
+    check(not dm.ishermitian(dm.DArray.from_array(Ab, dm.Blocks(150, 150))),
+          "not hermitian wide")
+    Ab = Ah.copy()
+    Ab[70, 71] += 1.0  # diagonal tile 0, sub-block (1, 1): exact comparison
+    check(not dm.ishermitian(dm.DArray.from_array(Ab, dm.Blocks(150, 150))),
+          "not hermitian wide diagonal")
     # the allocation-free TRSM: static scratch, bitwise equal to dm_dtrsm,
     # two calls in a row (the second smaller), a solve longer than one leaf
     for (side, uplo, tr, m, n) in (("L", "U", "T", 300, 20), ("R", "L", "N", 17, 70)):
