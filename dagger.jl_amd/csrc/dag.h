@@ -78,6 +78,9 @@ struct Dag {
 
   bool planned = false;
   dm_plan_opts opts{1, 0, 0, 0};
+  // the critical stream's launches of a look-ahead schedule raise their
+  // waves' priority over the bulk launches sharing the CUs
+  bool crit_prio = true;
   int num_levels = 0;
   std::vector<Group> groups;
   std::vector<int> order;

@@ -578,8 +578,9 @@ template <typename T>
 int Dag::launch_group_t(const Group& G, hipStream_t s) {
   const int* ab = abort_dev;
   // the critical stream's launches of a look-ahead schedule raise their
-  // waves' priority over the bulk update sharing the CUs
-  const int prio = (opts.lookahead > 0 && opts.num_streams >= 2 && G.crit == 0) ? 1 : 0;
+  // waves' priority over the bulk update sharing the CUs (crit_prio)
+  const int prio =
+      (crit_prio && opts.lookahead > 0 && opts.num_streams >= 2 && G.crit == 0) ? 1 : 0;
   if (G.op == DM_OP_GEMM) {
     if (G.njobs == 0) return 0;
     GemmJob<T> dummy{};

@@ -56,7 +56,11 @@ template <> struct VecT<double, 2> { typedef double t __attribute__((ext_vector_
 template <> struct VecT<float, 1> { typedef float t; };
 template <> struct VecT<float, 2> { typedef float t __attribute__((ext_vector_type(2))); };
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
+// PF: tiles of operands in registers ahead of the one being multiplied (1:
+// the next tile is loaded at the start of a k-step and goes into LDS
+// halfway through it; 2: see kloop2).
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC,
+          int PF = 1>
 struct GemmKernel {
   static constexpr int NT = 64 * WM * WN;
   static constexpr int WTM = BM / WM, WTN = BN / WN;
@@ -80,8 +84,27 @@ struct GemmKernel {
   typedef typename Mfma<T>::v4 v4;
   typedef typename VecT<T, VEC>::t vt;
 
+  // VEC elements at p (always a valid address); bit i of *okm records
+  // whether they lie inside the operand (they are zeroed when stored to LDS,
+  // so no instruction reads the loaded registers before that store)
+  static __device__ __forceinline__ void clamped_load(const T* p, bool ok, int i, T (&r)[VEC],
+                                                      unsigned* okm) {
+    vt v = *gptr((const vt*)p);
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) r[q] = reinterpret_cast<const T*>(&v)[q];
+    *okm |= (ok ? 1u : 0u) << i;
+  }
+
+  // CLAMP: a lane outside the operand loads the operand's first element(s)
+  // instead and flags them in *okm (zeroed by store_a/store_b), so the loads
+  // are unconditional: no branch around each load, and the compiler can wait
+  // for one tile's loads while a later tile's are in flight.  M, K >= 1
+  // whenever a tile is loaded, and 2-wide loads imply an even extent along
+  // the contiguous axis.
+  template <bool CLAMP = false>
   static __device__ __forceinline__ void load_a(const T* __restrict__ A, int lda, int m0, int k0,
-                                                int M, int K, T (&ra)[LA][VEC]) {
+                                                int M, int K, T (&ra)[LA][VEC],
+                                                unsigned* okm = nullptr) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int e = threadIdx.x + i * NT;
@@ -91,7 +114,9 @@ struct GemmKernel {
         gm = m0 + e / (BK / VEC);
         const bool ok = gm < M && gk < K;
         const T* p = A + (size_t)gm * lda + gk;
-        if (VEC == 2) {
+        if (CLAMP) {
+          clamped_load(ok ? p : A, ok, i, ra[i], okm);
+        } else if (VEC == 2) {
           vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) ra[i][q] = reinterpret_cast<const T*>(&v)[q];
@@ -103,7 +128,9 @@ struct GemmKernel {
         gk = k0 + e / (BM / VEC);
         const bool ok = gm < M && gk < K;
         const T* p = A + (size_t)gk * lda + gm;
-        if (VEC == 2) {
+        if (CLAMP) {
+          clamped_load(ok ? p : A, ok, i, ra[i], okm);
+        } else if (VEC == 2) {
           vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) ra[i][q] = reinterpret_cast<const T*>(&v)[q];
@@ -113,25 +140,32 @@ struct GemmKernel {
       }
     }
   }
-  static __device__ __forceinline__ void store_a(T* __restrict__ As, const T (&ra)[LA][VEC]) {
+  // okm: in-range flags of clamped loads (out-of-range elements store 0)
+  static __device__ __forceinline__ void store_a(T* __restrict__ As, const T (&ra)[LA][VEC],
+                                                 unsigned okm = ~0u) {
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int e = threadIdx.x + i * NT;
+      T v[VEC];
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) v[q] = ((okm >> i) & 1u) ? ra[i][q] : T(0);
       if (TA) {
         const int k = (e % (BK / VEC)) * VEC, m = e / (BK / VEC);
-        if (VEC == 2 && SK % 2 == 0) *(vt*)&As[m * SK + k] = *(const vt*)&ra[i][0];
+        if (VEC == 2 && SK % 2 == 0) *(vt*)&As[m * SK + k] = *(const vt*)&v[0];
         else
 #pragma unroll
-          for (int q = 0; q < VEC; ++q) As[m * SK + k + q] = ra[i][q];
+          for (int q = 0; q < VEC; ++q) As[m * SK + k + q] = v[q];
       } else {
         const int m = (e % (BM / VEC)) * VEC, k = e / (BM / VEC);
-        if (VEC == 2) *(vt*)&As[k * SMA + m] = *(const vt*)&ra[i][0];
-        else As[k * SMA + m] = ra[i][0];
+        if (VEC == 2) *(vt*)&As[k * SMA + m] = *(const vt*)&v[0];
+        else As[k * SMA + m] = v[0];
       }
     }
   }
+  template <bool CLAMP = false>
   static __device__ __forceinline__ void load_b(const T* __restrict__ B, int ldb, int n0, int k0,
-                                                int N, int K, T (&rb)[LB][VEC]) {
+                                                int N, int K, T (&rb)[LB][VEC],
+                                                unsigned* okm = nullptr) {
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
       const int e = threadIdx.x + i * NT;
@@ -141,7 +175,9 @@ struct GemmKernel {
         gn = n0 + e / (BK / VEC);
         const bool ok = gn < N && gk < K;
         const T* p = B + (size_t)gn * ldb + gk;
-        if (VEC == 2) {
+        if (CLAMP) {
+          clamped_load(ok ? p : B, ok, i, rb[i], okm);
+        } else if (VEC == 2) {
           vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) rb[i][q] = reinterpret_cast<const T*>(&v)[q];
@@ -153,7 +189,9 @@ struct GemmKernel {
         gk = k0 + e / (BN / VEC);
         const bool ok = gn < N && gk < K;
         const T* p = B + (size_t)gk * ldb + gn;
-        if (VEC == 2) {
+        if (CLAMP) {
+          clamped_load(ok ? p : B, ok, i, rb[i], okm);
+        } else if (VEC == 2) {
           vt v = ok ? *gptr((const vt*)p) : vt{};
 #pragma unroll
           for (int q = 0; q < VEC; ++q) rb[i][q] = reinterpret_cast<const T*>(&v)[q];
@@ -163,20 +201,24 @@ struct GemmKernel {
       }
     }
   }
-  static __device__ __forceinline__ void store_b(T* __restrict__ Bs, const T (&rb)[LB][VEC]) {
+  static __device__ __forceinline__ void store_b(T* __restrict__ Bs, const T (&rb)[LB][VEC],
+                                                 unsigned okm = ~0u) {
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
       const int e = threadIdx.x + i * NT;
+      T v[VEC];
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) v[q] = ((okm >> i) & 1u) ? rb[i][q] : T(0);
       if (!TB) {
         const int k = (e % (BK / VEC)) * VEC, n = e / (BK / VEC);
-        if (VEC == 2 && SK % 2 == 0) *(vt*)&Bs[n * SK + k] = *(const vt*)&rb[i][0];
+        if (VEC == 2 && SK % 2 == 0) *(vt*)&Bs[n * SK + k] = *(const vt*)&v[0];
         else
 #pragma unroll
-          for (int q = 0; q < VEC; ++q) Bs[n * SK + k + q] = rb[i][q];
+          for (int q = 0; q < VEC; ++q) Bs[n * SK + k + q] = v[q];
       } else {
         const int n = (e % (BN / VEC)) * VEC, k = e / (BN / VEC);
-        if (VEC == 2) *(vt*)&Bs[k * SNB + n] = *(const vt*)&rb[i][0];
-        else Bs[k * SNB + n] = rb[i][0];
+        if (VEC == 2) *(vt*)&Bs[k * SNB + n] = *(const vt*)&v[0];
+        else Bs[k * SNB + n] = v[0];
       }
     }
   }
@@ -201,6 +243,84 @@ struct GemmKernel {
     for (int j = 0; j < FN; ++j) bf[j] = frag_b(Bs, kk * 4 + lk, wn0 + j * 16 + lr);
   }
 
+  // K-loop with two tiles of operands in registers: the loads of tile kt+2
+  // are issued at the start of k-step kt and tile kt+1 goes into LDS at the
+  // end of it, so a load has about two k-steps of MFMAs to arrive.  For
+  // launches of one or two workgroups per CU whose k-steps wait on memory
+  // (the 64-column bulk updates of a blocked substitution).
+  // Register sets: tile 0 and even tiles in set 0, odd tiles in set 1; the
+  // set of tile kt is free once tile kt is in LDS (end of k-step kt-1).
+  // LDS: buffer kt&1 is read in k-step kt up to its barrier; buffer
+  // (kt+1)&1 is written in k-step kt before that barrier, after the barrier
+  // of k-step kt-1 that ended every read of it.
+  static __device__ __forceinline__ void kloop2(const GemmJob<T>& job, int m0, int n0, int wm0,
+                                                int wn0, int lr, int lk, T* smem,
+                                                v4 (&acc)[FM][FN]) {
+    T* As0 = smem;
+    T* Bs0 = smem + A_ELEMS;
+    T* As1 = smem + A_ELEMS + B_ELEMS;
+    T* Bs1 = As1 + A_ELEMS;
+    const int M = job.M, N = job.N, K = job.K;
+    const int nk = (K + BK - 1) / BK;
+    static_assert(LA <= 32 && LB <= 32, "in-range flags");
+    T ra0[LA][VEC], rb0[LB][VEC], ra1[LA][VEC], rb1[LB][VEC];
+    unsigned ma0 = 0, mb0 = 0, ma1 = 0, mb1 = 0;
+    if (nk > 0) {
+      load_a<true>(job.A, job.lda, m0, 0, M, K, ra0, &ma0);
+      load_b<true>(job.B, job.ldb, n0, 0, N, K, rb0, &mb0);
+      load_a<true>(job.A, job.lda, m0, BK, M, K, ra1, &ma1);
+      load_b<true>(job.B, job.ldb, n0, BK, N, K, rb1, &mb1);
+      store_a(As0, ra0, ma0);
+      store_b(Bs0, rb0, mb0);
+    }
+    __syncthreads();
+    constexpr int KS = BK / 4;
+    T af[FM], bf[FN];
+    if (nk > 0) read_frags(As0, Bs0, 0, wm0, wn0, lr, lk, af, bf);
+    // k-step kt: tile kt+2 is loaded into (la, lb), tile kt+1 stored from (sa, sb)
+    auto step = [&](int kt, T (&la)[LA][VEC], T (&lb)[LB][VEC], unsigned& mla, unsigned& mlb,
+                    const T (&sa)[LA][VEC], const T (&sb)[LB][VEC], unsigned msa,
+                    unsigned msb) {
+      const bool odd = kt & 1;
+      const T* As = odd ? As1 : As0;
+      const T* Bs = odd ? Bs1 : Bs0;
+      T* An = odd ? As0 : As1;
+      T* Bn = odd ? Bs0 : Bs1;
+      // past the last tile every lane loads the operand's first element(s)
+      mla = 0;
+      mlb = 0;
+      load_a<true>(job.A, job.lda, m0, (kt + 2) * BK, M, K, la, &mla);
+      load_b<true>(job.B, job.ldb, n0, (kt + 2) * BK, N, K, lb, &mlb);
+      if (kt >= nk) return;  // the odd step after the last tile: loads only
+      const bool more = kt + 1 < nk;
+#pragma unroll
+      for (int kk = 0; kk < KS; ++kk) {
+        T an[FM] = {}, bn[FN] = {};
+        if (kk + 1 < KS) read_frags(As, Bs, kk + 1, wm0, wn0, lr, lk, an, bn);
+        if (kk == KS - 1) {
+          if (more) {
+            store_a(An, sa, msa);
+            store_b(Bn, sb, msb);
+          }
+          __syncthreads();
+          if (more) read_frags(An, Bn, 0, wm0, wn0, lr, lk, an, bn);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mac(bf[j], af[i], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = an[i];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = bn[j];
+      }
+    };
+    for (int kt = 0; kt < nk; kt += 2) {
+      step(kt, ra0, rb0, ma0, mb0, ra1, rb1, ma1, mb1);
+      step(kt + 1, ra1, rb1, ma1, mb1, ra0, rb0, ma0, mb0);
+    }
+  }
+
   static __device__ __forceinline__ void run(const GemmJob<T>& job, int tm, int tn, T* smem) {
     T* As0 = smem;
     T* Bs0 = smem + A_ELEMS;
@@ -219,6 +339,10 @@ struct GemmKernel {
 #pragma unroll
       for (int j = 0; j < FN; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
+    const int lr = lane & 15, lk = lane >> 4;
+    if constexpr (PF == 2) {
+      kloop2(job, m0, n0, wm0, wn0, lr, lk, smem, acc);
+    } else {
     T ra[LA][VEC], rb[LB][VEC];
     const int nk = (K + BK - 1) / BK;
     if (nk > 0) {
@@ -228,7 +352,6 @@ struct GemmKernel {
       store_b(Bs0, rb);
     }
     __syncthreads();
-    const int lr = lane & 15, lk = lane >> 4;
     // Software pipeline over the BK/4 sub-steps of a k-step: the fragments
     // of sub-step kk+1 are read from LDS before the MFMAs of sub-step kk; the
     // next k-step's tile goes from registers into the spare LDS buffer
@@ -274,6 +397,7 @@ struct GemmKernel {
         for (int j = 0; j < FN; ++j) bf[j] = bn[j];
       }
     }
+    }  // PF == 1
 
     // Epilogue: lane holds C[m][n] with m = .. + lr, n = .. + row(lk, r).
     // Per 16-row strip, all old values of C are loaded before any store, so
@@ -349,12 +473,13 @@ struct GemmKernel {
   }
 };
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC,
+          int PF = 1>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_kernel(const GemmJob<T>* __restrict__ jobs,
                                                             int njobs, GemmJob<T> single,
                                                             int total,
                                                             const int* __restrict__ abort) {
-  typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC> G;
+  typedef GemmKernel<T, BM, BN, BK, WM, WN, TA, TB, VEC, PF> G;
   __shared__ T smem[2 * (G::A_ELEMS + G::B_ELEMS)];
   if (abort && *abort) return;
   if (single.prio) raise_wave_priority();
@@ -521,13 +646,15 @@ __global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __
 //   id 0: 128x128 tile, 16 waves of 32x32, BK 32 (139 KB LDS)
 //   id 1:  64x64  tile,  4 waves of 32x32, BK 16 -- small launches and
 //          products of at most 64 columns
+//   id 6:  64x64  tile,  4 waves of 32x32, BK 32, two tiles of operands in
+//          registers ahead -- products of at most 64 columns with K >= 512
 // FP32 keeps 128x128 with 4 waves of 64x64 (id 0) and 64x64 (id 1).
 // 16-wave workgroups put four waves on each SIMD, which hides LDS and
 // global latency that a single 64x64-per-wave workgroup leaves exposed.
 namespace {
 constexpr int GEMM_CFG_256x128 = 4;
 constexpr int GEMM_CFG_SKINNY = 5;  // 32 rows x bn (16..64) columns, K split over 4 waves
-constexpr int GEMM_CFG_64x64_K32 = 6;  // 64 x 64 tiles, 32-deep k-blocks
+constexpr int GEMM_CFG_64x64_K32 = 6;  // 64 x 64 tiles, 32-deep k-blocks, 2-deep prefetch
 }
 
 template <typename T>
@@ -579,10 +706,11 @@ long gemm_prepare_jobs(GemmJob<T>* jobs, int njobs, GemmCfgId cfg) {
 }
 
 namespace {
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC,
+          int PF = 1>
 int launch_one(const GemmJob<T>* jobs, int njobs, const GemmJob<T>& single, long total, long grid,
                const int* abort, hipStream_t s) {
-  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TA, TB, VEC>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TA, TB, VEC, PF>), dim3((unsigned)grid),
                      dim3(64 * WM * WN), 0, s, jobs, njobs, single, (int)total, abort);
   DM_LAUNCH_CHECK("gemm_kernel");
   return 0;
@@ -605,7 +733,8 @@ int launch_cfg(int id, const GemmJob<T>* jobs, int njobs, const GemmJob<T>& sing
   if (id == 1)
     return launch_one<T, 64, 64, 16, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
   if (id == GEMM_CFG_64x64_K32)
-    return launch_one<T, 64, 64, 32, 2, 2, TA, TB, VEC>(jobs, njobs, single, total, grid, abort, s);
+    return launch_one<T, 64, 64, 32, 2, 2, TA, TB, VEC, 2>(jobs, njobs, single, total, grid, abort,
+                                                          s);
   return arg_fail(0, "gemm: unknown configuration");
 }
 template <typename T, bool TA, bool TB>

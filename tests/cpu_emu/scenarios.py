@@ -55,8 +55,24 @@ def check(cond, what):
 
 
 # ---------------------------------------------------------------- tile ops
+def prefetch_gemms(rng, cases):
+    """dm_dgemm on products of at most 64 columns with K >= 512 in launches
+    of >= 64 tiles of 64 x 64 (the 64 x 64 tile with two-deep prefetch)."""
+    for (ta, tb, m, n, k) in cases:
+        A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+        B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+        Cm = rng.standard_normal((m, n))
+        dA, dB, dC = Dev(A), Dev(B), Dev(Cm)
+        call("dm_dgemm", ta.encode(), tb.encode(), m, n, k, -1.0, dA.p, A.shape[0], dB.p,
+             B.shape[0], 1.0, dC.p, m, None)
+        ref = Cm - (A.T if ta == "T" else A) @ (B.T if tb == "T" else B)
+        check(relerr(dC.get(), ref) < 1e-13, ("dgemm 64-column prefetch", ta, tb, m, n, k))
+
+
 def tiles():
     rng = np.random.default_rng(1)
+    prefetch_gemms(rng, (("N", "N", 4096, 64, 544), ("N", "T", 4100, 50, 1000),
+                         ("T", "T", 4096, 64, 512)))
     for ta in "NT":
         for tb in "NT":
             for (m, n, k) in [(200, 136, 77), (1, 1, 1), (64, 33, 0)]:
@@ -502,6 +518,10 @@ def quick():
         ref = np.array(A32, dtype=np.float64, order="F")
         O.tile_potrf(uplo, ref)
         check(O.rel_factor_diff(F, ref, uplo) <= 5e-5, ("f64 diag factor", uplo))
+    # a product of 64 columns with K >= 512 in a launch of >= 64 tiles (the
+    # two-deep prefetch k-loop): ragged M, an odd count of 32-deep k-steps,
+    # a ragged last k-step (the other transpositions: group "tiles")
+    prefetch_gemms(rng, (("T", "N", 4100, 64, 528),))
     # ishermitian with tiles wider than one 64 x 64 sub-block (ragged
     # 150 = 2 x 64 + 22): the pair check walks several sub-blocks per pair
     Ah = gen.spd(300, 6)
