@@ -58,7 +58,7 @@ template <> struct VecT<float, 2> { typedef float t __attribute__((ext_vector_ty
 
 // PF: tiles of operands in registers ahead of the one being multiplied (1:
 // the next tile is loaded at the start of a k-step and goes into LDS
-// halfway through it; 2: see kloop2).
+// halfway through it; >= 2: see kloop_pf).
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool TA, bool TB, int VEC,
           int PF = 1>
 struct GemmKernel {
@@ -243,41 +243,48 @@ struct GemmKernel {
     for (int j = 0; j < FN; ++j) bf[j] = frag_b(Bs, kk * 4 + lk, wn0 + j * 16 + lr);
   }
 
-  // K-loop with two tiles of operands in registers: the loads of tile kt+2
-  // are issued at the start of k-step kt and tile kt+1 goes into LDS at the
-  // end of it, so a load has about two k-steps of MFMAs to arrive.  For
-  // launches of one or two workgroups per CU whose k-steps wait on memory
-  // (the 64-column bulk updates of a blocked substitution).
-  // Register sets: tile 0 and even tiles in set 0, odd tiles in set 1; the
-  // set of tile kt is free once tile kt is in LDS (end of k-step kt-1).
+  // K-loop with PF (>= 2) tiles of operands in registers: the loads of tile
+  // kt+PF are issued at the start of k-step kt and tile kt+1 goes into LDS
+  // at the end of it, so a load has about PF k-steps of MFMAs to arrive.
+  // For launches of one or two workgroups per CU whose k-steps wait on
+  // memory (the 64-column bulk updates of a blocked substitution).
+  // Register sets: tile t in set t % PF; the set of tile kt is free once
+  // tile kt is in LDS (end of k-step kt-1).  The k-loop is unrolled PF
+  // times so that every set index is a compile-time constant.
   // LDS: buffer kt&1 is read in k-step kt up to its barrier; buffer
   // (kt+1)&1 is written in k-step kt before that barrier, after the barrier
   // of k-step kt-1 that ended every read of it.
-  static __device__ __forceinline__ void kloop2(const GemmJob<T>& job, int m0, int n0, int wm0,
-                                                int wn0, int lr, int lk, T* smem,
-                                                v4 (&acc)[FM][FN]) {
+  static __device__ __forceinline__ void kloop_pf(const GemmJob<T>& job, int m0, int n0, int wm0,
+                                                  int wn0, int lr, int lk, T* smem,
+                                                  v4 (&acc)[FM][FN]) {
+    static_assert(PF >= 2, "kloop_pf");
+    static_assert(LA <= 32 && LB <= 32, "in-range flags");
     T* As0 = smem;
     T* Bs0 = smem + A_ELEMS;
     T* As1 = smem + A_ELEMS + B_ELEMS;
     T* Bs1 = As1 + A_ELEMS;
     const int M = job.M, N = job.N, K = job.K;
     const int nk = (K + BK - 1) / BK;
-    static_assert(LA <= 32 && LB <= 32, "in-range flags");
-    T ra0[LA][VEC], rb0[LB][VEC], ra1[LA][VEC], rb1[LB][VEC];
-    unsigned ma0 = 0, mb0 = 0, ma1 = 0, mb1 = 0;
+    T ra[PF][LA][VEC], rb[PF][LB][VEC];
+    unsigned ma[PF], mb[PF];
     if (nk > 0) {
-      load_a<true>(job.A, job.lda, m0, 0, M, K, ra0, &ma0);
-      load_b<true>(job.B, job.ldb, n0, 0, N, K, rb0, &mb0);
-      load_a<true>(job.A, job.lda, m0, BK, M, K, ra1, &ma1);
-      load_b<true>(job.B, job.ldb, n0, BK, N, K, rb1, &mb1);
-      store_a(As0, ra0, ma0);
-      store_b(Bs0, rb0, mb0);
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        ma[u] = 0;
+        mb[u] = 0;
+        load_a<true>(job.A, job.lda, m0, u * BK, M, K, ra[u], &ma[u]);
+        load_b<true>(job.B, job.ldb, n0, u * BK, N, K, rb[u], &mb[u]);
+      }
+      store_a(As0, ra[0], ma[0]);
+      store_b(Bs0, rb[0], mb[0]);
     }
     __syncthreads();
     constexpr int KS = BK / 4;
     T af[FM], bf[FN];
     if (nk > 0) read_frags(As0, Bs0, 0, wm0, wn0, lr, lk, af, bf);
-    // k-step kt: tile kt+2 is loaded into (la, lb), tile kt+1 stored from (sa, sb)
+    // k-step kt: tile kt+PF is loaded into set (la, lb), tile kt+1 stored
+    // from set (sa, sb); past the last tile every lane loads the operand's
+    // first element(s), and a k-step past the last one only loads
     auto step = [&](int kt, T (&la)[LA][VEC], T (&lb)[LB][VEC], unsigned& mla, unsigned& mlb,
                     const T (&sa)[LA][VEC], const T (&sb)[LB][VEC], unsigned msa,
                     unsigned msb) {
@@ -286,12 +293,11 @@ struct GemmKernel {
       const T* Bs = odd ? Bs1 : Bs0;
       T* An = odd ? As0 : As1;
       T* Bn = odd ? Bs0 : Bs1;
-      // past the last tile every lane loads the operand's first element(s)
       mla = 0;
       mlb = 0;
-      load_a<true>(job.A, job.lda, m0, (kt + 2) * BK, M, K, la, &mla);
-      load_b<true>(job.B, job.ldb, n0, (kt + 2) * BK, N, K, lb, &mlb);
-      if (kt >= nk) return;  // the odd step after the last tile: loads only
+      load_a<true>(job.A, job.lda, m0, (kt + PF) * BK, M, K, la, &mla);
+      load_b<true>(job.B, job.ldb, n0, (kt + PF) * BK, N, K, lb, &mlb);
+      if (kt >= nk) return;
       const bool more = kt + 1 < nk;
 #pragma unroll
       for (int kk = 0; kk < KS; ++kk) {
@@ -315,9 +321,12 @@ struct GemmKernel {
         for (int j = 0; j < FN; ++j) bf[j] = bn[j];
       }
     };
-    for (int kt = 0; kt < nk; kt += 2) {
-      step(kt, ra0, rb0, ma0, mb0, ra1, rb1, ma1, mb1);
-      step(kt + 1, ra1, rb1, ma1, mb1, ra0, rb0, ma0, mb0);
+    for (int kt = 0; kt < nk; kt += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        step(kt + u, ra[u], rb[u], ma[u], mb[u], ra[(u + 1) % PF], rb[(u + 1) % PF],
+             ma[(u + 1) % PF], mb[(u + 1) % PF]);
+      }
     }
   }
 
@@ -340,8 +349,8 @@ struct GemmKernel {
       for (int j = 0; j < FN; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
     const int lr = lane & 15, lk = lane >> 4;
-    if constexpr (PF == 2) {
-      kloop2(job, m0, n0, wm0, wn0, lr, lk, smem, acc);
+    if constexpr (PF >= 2) {
+      kloop_pf(job, m0, n0, wm0, wn0, lr, lk, smem, acc);
     } else {
     T ra[LA][VEC], rb[LB][VEC];
     const int nk = (K + BK - 1) / BK;
@@ -647,7 +656,8 @@ __global__ __launch_bounds__(256) void potrf_update_kernel(GemmJob<T> job, T* __
 //   id 1:  64x64  tile,  4 waves of 32x32, BK 16 -- small launches and
 //          products of at most 64 columns
 //   id 6:  64x64  tile,  4 waves of 32x32, BK 32, two tiles of operands in
-//          registers ahead -- products of at most 64 columns with K >= 512
+//          registers ahead -- products of 17..64 columns with K >= 512 in
+//          launches of >= 64 such tiles
 // FP32 keeps 128x128 with 4 waves of 64x64 (id 0) and 64x64 (id 1).
 // 16-wave workgroups put four waves on each SIMD, which hides LDS and
 // global latency that a single 64x64-per-wave workgroup leaves exposed.
@@ -667,8 +677,15 @@ GemmCfgId gemm_pick_cfg(int M, int N, int K, int njobs, bool vec) {
   // does not depend on the column split.
   if (N <= 64 && (long)((M + 63) / 64) * ((N + 63) / 64) * nj < 64)
     return GemmCfgId{GEMM_CFG_SKINNY, 32, 16};
+  // 17..48 columns with a long K in a large launch: the 64 x 64 tile with
+  // two-deep prefetch, although it computes up to half its columns for
+  // nothing (the 32-column bulk updates of the refinement solves of an FP32
+  // factor at N = 32768: the mixed solve 126.7 -> 122.7 ms)
+  if (N > 16 && N <= 48 && K >= 512) return GemmCfgId{GEMM_CFG_64x64_K32, 64, 64};
   // few columns: the K-split skinny kernel (at 64 columns the 64 x 64 tile
-  // measured slightly faster in large launches)
+  // is faster in large launches: ldiv! with 64 right-hand sides at N = 32768
+  // took 9.2 ms with the skinny kernel for its bulk updates, 7.0 ms with
+  // the tile)
   if (N <= 48) return GemmCfgId{GEMM_CFG_SKINNY, 32, 16 * ((N + 15) / 16)};
   // at most 64 columns: the 64 x 64 tile (a 128-wide tile would compute
   // half its columns for nothing).  With a long K a workgroup's k-loop is a

@@ -72,7 +72,7 @@ def prefetch_gemms(rng, cases):
 def tiles():
     rng = np.random.default_rng(1)
     prefetch_gemms(rng, (("N", "N", 4096, 64, 544), ("N", "T", 4100, 50, 1000),
-                         ("T", "T", 4096, 64, 512)))
+                         ("T", "T", 4096, 64, 512), ("T", "N", 4100, 32, 600)))
     for ta in "NT":
         for tb in "NT":
             for (m, n, k) in [(200, 136, 77), (1, 1, 1), (64, 33, 0)]:
