@@ -197,11 +197,13 @@ def committed_traffic(variant):
     return int(val.group(1))
 
 
-def mixed_leg(dm, n, nb, kappa, nrhs, seed):
+def mixed_leg(dm, n, nb, kappa, nrhs, seed, updates=(0, 6)):
     """Precision form at one size: plain FP64 cholesky!+ldiv! vs FP32
     factor + FP64 refinement to 1e-13, on A = H diag(s) H^T (kappa exact).
     Both timed with HIP events on the default stream after a warm-up call
-    that builds the plans/graphs."""
+    that builds the plans/graphs.  `updates`: the FP32 factor's trailing
+    update arithmetics timed (0: FP32 MFMA, 6: BF16x6, the default; the
+    last one is reported as mixed_ms)."""
     from dagger_mi355x import mixed
     call = dm._lib.call
     timer = Timer(call)
@@ -222,7 +224,7 @@ def mixed_leg(dm, n, nb, kappa, nrhs, seed):
     # mixed: FP32 factor + IR; trailing updates on FP32 MFMA (update 0) and
     # on BF16 MFMA with 3-term operand splitting (update 6, the default)
     Ff = dm.DArray(np.float32, (n, n), blocks)
-    for update in (0, 6):
+    for update in updates:
         opts = mixed._OPTS.replace(f32_update=update)
         for rep in range(2):
             timer.start()
@@ -230,18 +232,20 @@ def mixed_leg(dm, n, nb, kappa, nrhs, seed):
             X, info = M.solve(B0, tol=1e-13, max_iter=30)
             ms = timer.stop_ms()
         res[f"mixed_ms_update{update}"] = ms
-    res["mixed_ms"] = ms  # the default (update 6) run
+    res["mixed_ms"] = ms  # the last (default: update 6) run
     Xm = X.collect()
     # backward error against the closed form A x = s*x - U (W^T x), host side
     b = B0.collect()
     r = b - spec.matvec(Xm)
     closed_berr = float((np.abs(r).max(0) / (M.normA * np.abs(Xm).max(0) +
                                              np.abs(b).max(0))).max())
+    if "mixed_ms_update0" in res:
+        res["speedup_fp32_update"] = round(res["fp64_ms"] / res["mixed_ms_update0"], 3)
     res.update(kappa=kappa, n=n, nb=nb, nrhs=nrhs, closed_form_berr=closed_berr,
                speedup=round(res["fp64_ms"] / res["mixed_ms"], 3),
-               speedup_fp32_update=round(res["fp64_ms"] / res["mixed_ms_update0"], 3),
                mixed_berr=float(np.max(info["berr"])), mixed_iters=info["iterations"],
-               method=info["method"],
+               method=info["method"], shift=info.get("shift"),
+               escalated=bool(info.get("escalated")),
                max_rel_diff_vs_fp64=float(np.abs(Xm - X64).max() / np.abs(X64).max()))
     for D in (A0, B0, DA, DB, Ff):
         D.unsafe_free()
@@ -465,6 +469,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=32768,
                     help="N of the mixed-precision leg (0 disables it)")
+    ap.add_argument("--mixed-kappa8-n", type=int, default=16384,
+                    help="N of the kappa = 1e8 mixed-precision leg, Blocks(512, 512), "
+                         "1 RHS (runs with the mixed leg; 0 disables it)")
     ap.add_argument("--serving-requests", type=int, default=500,
                     help="requests per arrival rate in the serving leg (0 disables it)")
     ap.add_argument("--serving-rates", type=str, default="200,500,1000,2000",
@@ -547,6 +554,12 @@ def main():
         DB.unsafe_free()
         mixed_res = mixed_leg(dm, a.mixed_n, nb, 1e6, 32, a.seed + 7)
         log(f"mixed: {mixed_res}")
+    mixed8_res = None
+    if a.mixed_n > 0 and a.mixed_kappa8_n > 0:
+        # kappa = 1e8 (kappa u32 about 6): the shifted FP32 factor and, when
+        # refinement stalls, GMRES-IR; reported with its method and berr
+        mixed8_res = mixed_leg(dm, a.mixed_kappa8_n, 512, 1e8, 1, a.seed + 8, updates=(6,))
+        log(f"mixed kappa 1e8: {mixed8_res}")
     serving_res = None
     if a.serving_requests > 0:
         rates = [float(x) for x in a.serving_rates.split(",") if x]
@@ -589,6 +602,7 @@ def main():
         "groups_by_op": by_op,
         "norm2_pass": norm_res,
         "mixed_ir": mixed_res,
+        "mixed_ir_kappa1e8": mixed8_res,
         "serving": serving_res,
         "batched": batched_res,
         "cpu_baseline": cpu,

@@ -18,9 +18,8 @@
 //    workgroups find their job by binary search over wg_begin, and each XCD
 //    (blockIdx is dealt round-robin over the 8 XCDs) owns a contiguous range
 //    of tiles, so consecutive tiles of one job share an L2.  A launch may use
-//    fewer workgroups than tiles (each then loops over its share): the
-//    executor caps bulk launches below the CU count so that the critical
-//    path's small launches always find free CUs.
+//    fewer workgroups than tiles (gemm_launch's max_wg; each workgroup then
+//    loops over its share).
 //  * SYRK is the same kernel with a triangle mask: only workgroup tiles that
 //    touch the requested triangle are launched, and diagonal tiles store
 //    masked.

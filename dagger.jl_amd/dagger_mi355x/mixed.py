@@ -234,13 +234,15 @@ class MixedCholesky:
              C.byref(o), self.stream)
 
     # ---------------------------------------------------------------- solve
-    def solve(self, B, tol=1e-13, max_iter=30, X=None):
+    def solve(self, B, tol=1e-13, max_iter=30, X=None, force_gmres=False):
         """Solve A X = B (B an FP64 DArray N x nrhs, Blocks(nb, nb)).
 
         Returns (X, info) with info = dict(iterations, berr (per column),
-        method, history, escalated).  A factor built with the 2-term BF16
-        update that does not reach tol is rebuilt with the 3-term update
-        (escalated = True)."""
+        method, history, escalated, shift).  Classic refinement runs until
+        tol, max_iter or a stall; then GMRES-IR if tol is not met
+        (force_gmres: straight after the first solve).  A factor built with
+        the 2-term BF16 update that does not reach tol is rebuilt with the
+        3-term update (escalated = True)."""
         if B.dtype != np.float64 or B.m != self.A.m:
             raise ValueError("B must be an FP64 DArray with A's rows")
         s = self.stream
@@ -260,7 +262,7 @@ class MixedCholesky:
             nx = _colnorms(Xd, nrm, s)
             berr = nr / (self.normA * nx + nb_)
             hist.append(float(berr.max()))
-            if berr.max() <= tol or it >= max_iter:
+            if berr.max() <= tol or it >= max_iter or force_gmres:
                 break
             if it >= 3 and hist[-1] > 0.5 * hist[-3]:
                 break  # stalled: kappa * u32 too close to 1
@@ -283,7 +285,7 @@ class MixedCholesky:
             # again from scratch.
             self.options = self.options.replace(f32_update=6)
             self.refactor()
-            Xd, info = self.solve(B, tol=tol, max_iter=max_iter, X=X)
+            Xd, info = self.solve(B, tol=tol, max_iter=max_iter, X=X, force_gmres=force_gmres)
             info["iterations"] += it
             info["escalated"] = True
             return Xd, info

@@ -50,18 +50,18 @@ def _closed_form_berr(h, normA, x, b):
     return np.abs(r2).max(0) / (normA * np.abs(x2).max(0) + np.abs(b2).max(0))
 
 
-def _mixed_case(dm, n, nb, kappa, nrhs_list, seed):
+def _mixed_case(dm, n, nb, kappa, nrhs_list, seed, options=None, force_gmres=False):
     from dagger_mi355x import mixed
     D, _ = mixed.householder_generate(n, dm.Blocks(nb, nb), kappa, seed=seed, nrefl=64)
     h = gen.HouseholderSPD(n, kappa, seed, nrefl=64)
     normA = _norm_inf(h)
-    M = mixed.MixedCholesky(D)
+    M = mixed.MixedCholesky(D, options=options)
     out = []
     for nrhs in nrhs_list:
         xt = gen.randn(n * nrhs, seed + nrhs).reshape(nrhs, n).T.copy()
         b = h.matvec(xt)
         DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
-        X, info = M.solve(DB, tol=1e-13, max_iter=30)
+        X, info = M.solve(DB, tol=1e-13, max_iter=30, force_gmres=force_gmres)
         x = X.collect()
         berr = _closed_form_berr(h, normA, x, b)
         out.append((nrhs, berr, info, x, xt))
@@ -101,6 +101,43 @@ def test_mixed_kappa1e8_reports_method_and_meets_tolerance(dm):
     assert berr.max() <= 1e-13, (berr.max(), info)
     assert info["method"] in ("ir", "gmres-ir")
     assert info["shift"] in (0.0, 4.0, 32.0, 256.0)
+
+
+def test_mixed_gmres_ir_meets_tolerance(dm):
+    """GMRES-IR itself (FP64 GMRES, the FP32 factor as preconditioner) at
+    kappa = 1e8, N = 16384: refinement skipped after the first solve."""
+    [(nrhs, berr, info, x, xt)] = _mixed_case(dm, 16384, 512, 1e8, (1,), 88, force_gmres=True)
+    assert info["method"] == "gmres-ir", info
+    assert berr.max() <= 1e-13, (berr.max(), info)
+
+
+def test_mixed_shift_when_the_fp32_copy_is_singular(dm):
+    """An SPD A whose FP32 copy is exactly singular: I with the 2 x 2 block
+    [1, 1 - 1e-9; 1 - 1e-9, 1] at rows 300, 301 (eigenvalues 2 - 1e-9 and
+    1e-9).  1 - 1e-9 rounds to 1 in FP32, so the factorization of the copy
+    meets the pivot 1 - 1 = 0 in any arithmetic; the scaled diagonal shift
+    (diagonal times 1 + 4 u32) makes the pivot about 4.8e-7, and refinement
+    against the unshifted FP64 A (kappa 2e9; classic refinement stalls on
+    the 1e-9 direction, GMRES-IR resolves it) reaches 1e-13."""
+    from dagger_mi355x import mixed
+    n, nb = 1024, 512
+    A = np.eye(n)
+    A[300, 301] = A[301, 300] = 1.0 - 1e-9
+    assert np.float32(A[300, 301]) == np.float32(1.0)
+    xt = gen.randn(n, 93).reshape(n, 1)
+    b = A @ xt
+    D = dm.DArray.from_array(A, dm.Blocks(nb, nb))
+    DB = dm.DArray.from_array(b, dm.Blocks(nb, nb))
+    M = mixed.MixedCholesky(D)
+    assert M.shift == 4.0, M.shift
+    X, info = M.solve(DB, tol=1e-13, max_iter=30)
+    x = X.collect()
+    berr = np.abs(b - A @ x).max() / (np.abs(A).sum(1).max() * np.abs(x).max() + np.abs(b).max())
+    assert info["shift"] == 4.0 and info["method"] == "gmres-ir", info
+    assert berr <= 1e-13, (berr, info)
+    mixed.release_factor_buffers(M.F)
+    for d in (M.F, DB, D):
+        d.unsafe_free()
 
 
 def _batched_check(dm, dtype, count=4096, seed=2024, mode="hybrid"):
