@@ -79,7 +79,8 @@ struct Dag {
   bool planned = false;
   dm_plan_opts opts{1, 0, 0, 0};
   // the critical stream's launches of a look-ahead schedule raise their
-  // waves' priority over the bulk launches sharing the CUs
+  // waves' priority over the bulk launches sharing the CUs (set by plan():
+  // plans with factorization tasks)
   bool crit_prio = true;
   int num_levels = 0;
   std::vector<Group> groups;

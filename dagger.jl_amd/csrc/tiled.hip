@@ -412,11 +412,6 @@ int setup_potrs_inv(Dag& dag, char up, const TileView& ta, const TileView& tb) {
   dm_tiled yt{ybuf->p, tb.m, tb.n, tb.mb, tb.nb, tb.dtype};
   TileView ty(yt);
   build_potrs_inv(dag, up, ta, tb, ty, [&](int k) { return (void*)plan->inv(k); });
-  // A sweep's period is set by its bulk launches (64-column products that
-  // share the CUs with the critical launches), not by the critical chain:
-  // no wave priority for the critical launches (N = 32768, 64 right-hand
-  // sides: 6.55 ms instead of 6.75 ms).
-  dag.crit_prio = false;
   dag.pre_hook = [plan, ybuf](Dag* self, hipStream_t s) { return plan->run(s, self->abort_dev); };
   return 0;
 }
