@@ -155,6 +155,52 @@ __global__ __launch_bounds__(256) void col_abs_max_kernel(const double* __restri
   if (threadIdx.x == 0) out[j] = part[0];
 }
 
+// Column dot products of two tile-packed m x n matrices of the same shape
+// and blocks: one workgroup per column, out[j] = sum_i X_ij Y_ij in a fixed
+// order (strided partial sums, then a tree), so the result is deterministic.
+__global__ __launch_bounds__(256) void col_dot_kernel(const double* __restrict__ X,
+                                                      const double* __restrict__ Y, int64_t m,
+                                                      int64_t n, int mb, int nb,
+                                                      double* __restrict__ out) {
+  __shared__ double part[256];
+  const int64_t j = blockIdx.x;
+  const int64_t tj = j / nb, jj = j - tj * nb;
+  const int64_t cols = (tj * nb + nb <= n) ? nb : n - tj * nb;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < m; i += 256) {
+    const int64_t ti = i / mb, ii = i - ti * mb;
+    const int64_t rows = (ti * mb + mb <= m) ? mb : m - ti * mb;
+    const int64_t e = tj * nb * m + ti * mb * cols + jj * rows + ii;
+    acc = fma(X[e], Y[e], acc);
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = part[0];
+}
+
+// Y[:, j] = (scale * a[j]) X[:, j] + beta Y[:, j] on tile-packed matrices of
+// the same shape and blocks (a == nullptr: a[j] = 1; beta == 0: Y is not
+// read).  Grid: (row blocks of 256, columns); X may be Y.
+__global__ __launch_bounds__(256) void col_axpby_kernel(const double* __restrict__ a,
+                                                        double scale, const double* X,
+                                                        double beta, double* Y, int64_t m,
+                                                        int64_t n, int mb, int nb) {
+  const int64_t j = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const int64_t tj = j / nb, jj = j - tj * nb;
+  const int64_t cols = (tj * nb + nb <= n) ? nb : n - tj * nb;
+  const int64_t ti = i / mb, ii = i - ti * mb;
+  const int64_t rows = (ti * mb + mb <= m) ? mb : m - ti * mb;
+  const int64_t e = tj * nb * m + ti * mb * cols + jj * rows + ii;
+  const double c = scale * (a ? a[j] : 1.0);
+  Y[e] = beta == 0.0 ? c * X[e] : c * X[e] + beta * Y[e];
+}
+
 static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
 
 int convert_slab(void* dst, int dst_dtype, const void* src, int src_dtype, size_t count,
@@ -381,6 +427,42 @@ int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s) {
   hipLaunchKernelGGL(col_abs_max_kernel, dim3((unsigned)X->n), dim3(256), 0, stream_of(s),
                      (const double*)X->base, X->m, X->n, X->mb, X->nb, out_dev);
   DM_LAUNCH_CHECK("col_abs_max_kernel");
+  return 0;
+}
+
+static bool same_layout(const dm_tiled* X, const dm_tiled* Y) {
+  return X->m == Y->m && X->n == Y->n && X->mb == Y->mb && X->nb == Y->nb;
+}
+
+int dm_tiled_col_dot(const dm_tiled* X, const dm_tiled* Y, double* out_dev, dm_stream_t s) {
+  if (!X || !Y || X->dtype != DM_F64 || Y->dtype != DM_F64)
+    return arg_fail(1, "X and Y must be FP64 tiled matrices");
+  if (!same_layout(X, Y)) return arg_fail(2, "X and Y differ in shape or blocks");
+  if (!out_dev) return arg_fail(3, "NULL output");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (X->n <= 0) return 0;
+  if (X->m <= 0) return fill_i32_launch((int*)out_dev, 0, (int)(2 * X->n), stream_of(s));
+  hipLaunchKernelGGL(col_dot_kernel, dim3((unsigned)X->n), dim3(256), 0, stream_of(s),
+                     (const double*)X->base, (const double*)Y->base, X->m, X->n, X->mb, X->nb,
+                     out_dev);
+  DM_LAUNCH_CHECK("col_dot_kernel");
+  return 0;
+}
+
+int dm_tiled_col_axpby(const double* a_dev, double scale, const dm_tiled* X, double beta,
+                       const dm_tiled* Y, dm_stream_t s) {
+  if (!X || !Y || X->dtype != DM_F64 || Y->dtype != DM_F64)
+    return arg_fail(3, "X and Y must be FP64 tiled matrices");
+  if (!same_layout(X, Y)) return arg_fail(5, "X and Y differ in shape or blocks");
+  if (X->n > 65535) return arg_fail(3, "at most 65535 columns");
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (X->m <= 0 || X->n <= 0) return 0;
+  hipLaunchKernelGGL(col_axpby_kernel, dim3(nblocks((size_t)X->m), (unsigned)X->n), dim3(256), 0,
+                     stream_of(s), a_dev, scale, (const double*)X->base, beta,
+                     (double*)Y->base, X->m, X->n, X->mb, X->nb);
+  DM_LAUNCH_CHECK("col_axpby_kernel");
   return 0;
 }
 

@@ -141,6 +141,9 @@ _PROTOS = {
     "dm_tiled_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp, vp]),
     "dm_tiled_identity": (i32, [C.POINTER(dm_tiled), vp]),
     "dm_tiled_col_norm_inf": (i32, [C.POINTER(dm_tiled), vp, vp]),
+    "dm_tiled_col_dot": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp, vp]),
+    "dm_tiled_col_axpby": (i32, [vp, C.c_double, C.POINTER(dm_tiled), C.c_double,
+                                 C.POINTER(dm_tiled), vp]),
     "dm_tiled_axpy_s2d": (i32, [C.POINTER(dm_tiled), C.POINTER(dm_tiled), vp]),
     "dm_tiled_scale_diag": (i32, [C.POINTER(dm_tiled), f64, vp]),
     "dm_add_diag_d": (i32, [vp, i64, i64, vp, vp]),

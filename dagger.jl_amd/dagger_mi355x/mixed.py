@@ -292,86 +292,128 @@ class MixedCholesky:
         return Xd, dict(iterations=it, berr=berr, method=method, history=hist, escalated=False,
                         shift=self.shift)
 
-    def _gmres_ir(self, B, X, tol, max_outer):
-        """GMRES-IR: outer FP64 refinement; each correction solves
-        (M^-1 A) d = M^-1 r by GMRES in FP64 with M = F32^T F32.  Vectors
-        live on the host (n-length), products with A and M^-1 on the device."""
-        n = self.A.m
-        s = self.stream
+    def _gmres_ir(self, B, X, tol, max_outer, restart=30, inner_tol=1e-4):
+        """GMRES-IR over all right-hand sides at once: outer FP64 refinement
+        X += d, each correction solving (M^-1 A) d = M^-1 r by restarted
+        GMRES in FP64 with M = F32^T F32 (one GMRES per column, advanced in
+        lockstep).  The Krylov vectors stay on the device as N x nrhs blocks:
+        A and M^-1 are applied to all columns in one product / one solve,
+        modified Gram-Schmidt runs on column dot products and scaled updates
+        (dm_tiled_col_dot / dm_tiled_col_axpby, coefficients on the device);
+        only each column's (restart+1) x restart Hessenberg, its Givens
+        rotations and the small triangular solve are on the host.  Returns
+        the number of outer iterations."""
+        n, r, s = self.A.m, B.n, self.stream
         blk = Blocks(self.nb, self.nb)
-        v64 = DArray(np.float64, (n, 1), blk)
-        w64 = DArray(np.float64, (n, 1), blk)
-        d32 = self._buffers(1)["D"]
-        x_all = X.collect(s)
-        b_all = B.collect(s)
+        buf = self._buffers(r)
+        R, D, nrm = buf["R"], buf["D"], buf["nrm"]
+        V = [DArray(np.float64, (n, r), blk) for _ in range(restart + 1)]
+        W = DArray(np.float64, (n, r), blk)
+        hbuf = DeviceBuffer(8 * (restart + 2) * r)
+        cbuf = [DeviceBuffer(8 * r) for _ in range(restart + 1)]
+        tV = [v.tiled() for v in V]
+        tW, tX, ta = W.tiled(), X.tiled(), self.A.tiled()
+        o = self.options.c()
+        keep = []  # host coefficient arrays until the next synchronizing copy
 
-        def A_mv(v):
-            v64.copy_from_host(v.reshape(n, 1), s)
-            ta, tv, tw = self.A.tiled(), v64.tiled(), w64.tiled()
-            o = self.options.c()
-            call("dm_gemm_tiled", b"N", b"N", 1.0, C.byref(ta), C.byref(tv), 0.0, C.byref(tw),
-                 C.byref(o), s)
-            return w64.collect(s).ravel()
+        def coef(vals, slot=0):
+            arr = np.ascontiguousarray(vals, dtype=np.float64)
+            keep.append(arr)
+            call("dm_memcpy_h2d", C.c_void_p(cbuf[slot].ptr), arr.ctypes.data_as(C.c_void_p),
+                 C.c_size_t(8 * r), s)
+            return C.c_void_p(cbuf[slot].ptr)
 
-        def Minv(v):
-            v64.copy_from_host(v.reshape(n, 1), s)
-            _convert(d32, v64, s)
-            self._solve32(d32)
-            _convert(v64, d32, s)
-            return v64.collect(s).ravel()
+        def hdev(i):
+            return C.c_void_p(hbuf.ptr + 8 * i * r)
 
+        def host(count):  # rows 0..count-1 of the device coefficients (synchronizes)
+            out = np.empty(count * r)
+            call("dm_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(hbuf.ptr),
+                 C.c_size_t(8 * count * r), s)
+            keep.clear()
+            return out.reshape(count, r)
+
+        def precond_into(dst_t):  # dst = M^-1 R (R holds the operand)
+            _convert(D, R, s)
+            self._solve32(D)
+            td = D.tiled()
+            call("dm_tiled_convert", C.byref(dst_t), C.byref(td), s)
+
+        nb_ = _colnorms(B, nrm, s)
         total = 0
-        for j in range(B.n):
-            x = x_all[:, j].copy()
-            b = b_all[:, j]
-            for outer in range(max_outer):
-                r = b - A_mv(x)
-                if np.abs(r).max() <= tol * (self.normA * np.abs(x).max() + np.abs(b).max()):
+        try:
+            for _ in range(max_outer):
+                self.residual(B, X, R)
+                berr = _colnorms(R, nrm, s) / (self.normA * _colnorms(X, nrm, s) + nb_)
+                if berr.max() <= tol:
                     break
-                x += _gmres(lambda v: Minv(A_mv(v)), Minv(r), restart=30, tol=1e-4)
+                precond_into(tV[0])
+                call("dm_tiled_col_dot", C.byref(tV[0]), C.byref(tV[0]), hdev(0), s)
+                beta = np.sqrt(np.maximum(host(1)[0], 0.0))
+                live = (beta > 0) & (berr > tol)
+                safe = np.where(beta > 0, beta, 1.0)
+                call("dm_tiled_col_axpby", coef(np.where(live, 1.0 / safe, 0.0)), 1.0,
+                     C.byref(tV[0]), 0.0, C.byref(tV[0]), s)
+                H = np.zeros((r, restart + 1, restart))
+                cs, sn = np.zeros((r, restart)), np.zeros((r, restart))
+                g = np.zeros((r, restart + 1))
+                g[:, 0] = np.where(live, beta, 0.0)
+                used = 0
+                for k in range(restart):
+                    # W = M^-1 A V_k for every column at once
+                    call("dm_gemm_tiled", b"N", b"N", 1.0, C.byref(ta), C.byref(tV[k]), 0.0,
+                         C.byref(tW), C.byref(o), s)
+                    call("dm_memcpy_d2d", C.c_void_p(R.buffer.ptr), C.c_void_p(W.buffer.ptr),
+                         C.c_size_t(8 * n * r), s)
+                    precond_into(tW)
+                    # modified Gram-Schmidt: h_i = V_i . W, W -= h_i V_i (per column)
+                    for i in range(k + 1):
+                        call("dm_tiled_col_dot", C.byref(tV[i]), C.byref(tW), hdev(i), s)
+                        call("dm_tiled_col_axpby", hdev(i), -1.0, C.byref(tV[i]), 1.0,
+                             C.byref(tW), s)
+                    call("dm_tiled_col_dot", C.byref(tW), C.byref(tW), hdev(k + 1), s)
+                    h = host(k + 2)
+                    hk1 = np.sqrt(np.maximum(h[k + 1], 0.0))
+                    call("dm_tiled_col_axpby",
+                         coef(np.where(live & (hk1 > 0), 1.0 / np.where(hk1 > 0, hk1, 1.0), 0.0)),
+                         1.0, C.byref(tW), 0.0, C.byref(tV[k + 1]), s)
+                    for j in np.nonzero(live)[0]:
+                        col = np.append(h[:k + 1, j], hk1[j])
+                        for i in range(k):
+                            t = cs[j, i] * col[i] + sn[j, i] * col[i + 1]
+                            col[i + 1] = -sn[j, i] * col[i] + cs[j, i] * col[i + 1]
+                            col[i] = t
+                        rho = np.hypot(col[k], col[k + 1])
+                        cs[j, k], sn[j, k] = (col[k] / rho, col[k + 1] / rho) if rho > 0 else (1.0, 0.0)
+                        col[k], col[k + 1] = rho, 0.0
+                        H[j, :k + 2, k] = col
+                        g[j, k + 1] = -sn[j, k] * g[j, k]
+                        g[j, k] = cs[j, k] * g[j, k]
+                    used = k + 1
+                    if np.all(~live | (np.abs(g[:, k + 1]) <= inner_tol * safe)):
+                        break
+                # y_j = H_j^-1 g_j, then X += sum_i V_i diag(y_i)
+                Y = np.zeros((used, r))
+                for j in np.nonzero(live)[0]:
+                    Hj = np.triu(H[j, :used, :used])
+                    if np.all(np.diag(Hj) != 0):
+                        Y[:, j] = np.linalg.solve(Hj, g[j, :used])
+                    else:
+                        Y[:, j] = np.linalg.lstsq(Hj, g[j, :used], rcond=None)[0]
+                for i in range(used):
+                    call("dm_tiled_col_axpby", coef(Y[i], slot=i), 1.0, C.byref(tV[i]), 1.0,
+                         C.byref(tX), s)
                 total += 1
-            x_all[:, j] = x
-        X.copy_from_host(x_all, s)
+        finally:
+            call("dm_stream_sync", s)
+            keep.clear()
+            for v in V:
+                v.unsafe_free()
+            W.unsafe_free()
+            hbuf.free()
+            for c in cbuf:
+                c.free()
         return total
-
-
-def _gmres(op, rhs, restart=30, tol=1e-4):
-    """One GMRES cycle of at most `restart` steps in FP64 (modified
-    Gram-Schmidt, Givens rotations)."""
-    n = rhs.shape[0]
-    beta = np.linalg.norm(rhs)
-    if beta == 0:
-        return np.zeros(n)
-    Q = np.zeros((restart + 1, n))
-    Hm = np.zeros((restart + 1, restart))
-    cs, sn = np.zeros(restart), np.zeros(restart)
-    g = np.zeros(restart + 1)
-    g[0] = beta
-    Q[0] = rhs / beta
-    k = 0
-    for k in range(restart):
-        w = op(Q[k])
-        for i in range(k + 1):
-            Hm[i, k] = w @ Q[i]
-            w = w - Hm[i, k] * Q[i]
-        Hm[k + 1, k] = np.linalg.norm(w)
-        if Hm[k + 1, k] > 0:
-            Q[k + 1] = w / Hm[k + 1, k]
-        for i in range(k):
-            t = cs[i] * Hm[i, k] + sn[i] * Hm[i + 1, k]
-            Hm[i + 1, k] = -sn[i] * Hm[i, k] + cs[i] * Hm[i + 1, k]
-            Hm[i, k] = t
-        rho = np.hypot(Hm[k, k], Hm[k + 1, k])
-        cs[k], sn[k] = Hm[k, k] / rho, Hm[k + 1, k] / rho
-        Hm[k, k] = rho
-        Hm[k + 1, k] = 0.0
-        g[k + 1] = -sn[k] * g[k]
-        g[k] = cs[k] * g[k]
-        if abs(g[k + 1]) <= tol * beta:
-            break
-    m = k + 1
-    y = np.linalg.solve(np.triu(Hm[:m, :m]), g[:m])
-    return Q[:m].T @ y
 
 
 class HouseholderSpec:

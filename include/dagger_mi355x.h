@@ -411,6 +411,16 @@ int dm_tiled_convert(const dm_tiled* dst, const dm_tiled* src, dm_stream_t s);
 int dm_tiled_norm_inf(const dm_tiled* A, double* out_dev, double* work_dev, dm_stream_t s);
 /* out_dev[j] = max_i |X_ij| for each column j (FP64). */
 int dm_tiled_col_norm_inf(const dm_tiled* X, double* out_dev, dm_stream_t s);
+/* out_dev[j] = sum_i X_ij Y_ij for each column j (FP64, X and Y of the same
+ * shape and blocks; deterministic order).  Column-blocked GMRES-IR: the
+ * Gram-Schmidt coefficients and norms of n right-hand sides at once. */
+int dm_tiled_col_dot(const dm_tiled* X, const dm_tiled* Y, double* out_dev, dm_stream_t s);
+/* Y[:, j] = (scale * a_dev[j]) X[:, j] + beta Y[:, j] for each column j
+ * (FP64, same shape and blocks; a_dev NULL: a_dev[j] = 1; beta = 0: Y is
+ * not read; X may be Y; at most 65535 columns).  Column-blocked GMRES-IR:
+ * orthogonalization, normalization and the solution update. */
+int dm_tiled_col_axpby(const double* a_dev, double scale, const dm_tiled* X, double beta,
+                       const dm_tiled* Y, dm_stream_t s);
 /* A[i,i] *= factor on a square tiled matrix with square blocks (FP64 or
  * FP32): the scaled diagonal shift A + c u diag(A) that keeps a
  * low-precision copy of an ill-conditioned SPD matrix positive definite

@@ -522,6 +522,26 @@ def quick():
     # two-deep prefetch k-loop): ragged M, an odd count of 32-deep k-steps,
     # a ragged last k-step (the other transpositions: group "tiles")
     prefetch_gemms(rng, (("T", "N", 4100, 64, 528),))
+    # column dot products and scaled column updates (blocked GMRES-IR) on a
+    # ragged tiling: 300 x 5 in Blocks(64, 2)
+    Xh, Yh = rng.standard_normal((300, 5)), rng.standard_normal((300, 5))
+    DX = dm.DArray.from_array(Xh, dm.Blocks(64, 2))
+    DY = dm.DArray.from_array(Yh, dm.Blocks(64, 2))
+    tx, ty = DX.tiled(), DY.tiled()
+    dots = Dev(np.zeros(5))
+    call("dm_tiled_col_dot", C.byref(tx), C.byref(ty), dots.p, None)
+    check(np.allclose(dots.get(), (Xh * Yh).sum(0), rtol=1e-13, atol=0), "col dot")
+    coefs = Dev(np.array([0.5, -2.0, 0.0, 3.0, 1.25]))
+    call("dm_tiled_col_axpby", coefs.p, -1.0, C.byref(tx), 1.0, C.byref(ty), None)
+    Yh = Yh - Xh * np.array([0.5, -2.0, 0.0, 3.0, 1.25])
+    check(np.allclose(DY.collect(), Yh, rtol=1e-15, atol=1e-15), "col axpby")
+    call("dm_tiled_col_axpby", coefs.p, 2.0, C.byref(tx), 0.0, C.byref(tx), None)  # in place
+    check(np.array_equal(DX.collect(), Xh * (2.0 * np.array([0.5, -2.0, 0.0, 3.0, 1.25]))),
+          "col axpby in place")
+    DZ = dm.DArray.from_array(np.full((300, 5), np.nan), dm.Blocks(64, 2))
+    tz = DZ.tiled()
+    call("dm_tiled_col_axpby", None, 1.0, C.byref(ty), 0.0, C.byref(tz), None)
+    check(np.array_equal(DZ.collect(), DY.collect()), "col axpby beta 0 ignores Y")
     # ishermitian with tiles wider than one 64 x 64 sub-block (ragged
     # 150 = 2 x 64 + 22): the pair check walks several sub-blocks per pair
     Ah = gen.spd(300, 6)
