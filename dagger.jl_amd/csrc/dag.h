@@ -78,9 +78,9 @@ struct Dag {
 
   bool planned = false;
   dm_plan_opts opts{1, 0, 0, 0};
-  // the critical stream's launches of a look-ahead schedule raise their
-  // waves' priority over the bulk launches sharing the CUs (set by plan():
-  // plans with factorization tasks)
+  // which side of a look-ahead schedule raises its waves' priority: the
+  // critical stream's launches (plans with factorization tasks; set by
+  // plan()) or the bulk launches
   bool crit_prio = true;
   int num_levels = 0;
   std::vector<Group> groups;
