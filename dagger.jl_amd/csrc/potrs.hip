@@ -182,6 +182,10 @@ int TrtriPlan<T>::build_list(bool up, int count, int nb_, const std::function<co
     tab_bytes += align_up<T>(sizeof(GemmJob<T>) * t.size(), 256);
   }
   DM_HIP(hipMalloc(&mem, inv_bytes + scratch_bytes + tab_bytes));
+  // The inverse tiles' other triangle stays zero: the leaves write their
+  // whole square (zeros included) and the level products only blocks inside
+  // the triangle, so it is cleared once here instead of on every run.
+  DM_HIP(hipMemset(mem, 0, inv_bytes));
   char* base = (char*)mem;
   T* invb = (T*)base;
   T* scr = (T*)(base + inv_bytes);
@@ -229,8 +233,7 @@ int TrtriPlan<T>::build_list(bool up, int count, int nb_, const std::function<co
 
 template <typename T>
 int TrtriPlan<T>::run(hipStream_t s, const int* abort) const {
-  int rc = fill_i32_launch((int*)mem, 0, inv_bytes / 4, s);
-  if (rc) return rc;
+  int rc = 0;
   if (nleaves) {
     if (upper)
       hipLaunchKernelGGL((trtri_leaf_kernel<T, true>), dim3(nleaves), dim3(64), 0, s, leaves_dev,
